@@ -1,0 +1,152 @@
+/*
+ * nsphere_gpu.h -- C-ABI of the MI355X-native NSphere per-step evolution loop.
+ *
+ * One context owns one GPU's copy of the particle state (SoA r, v_r, ell in
+ * float64 plus a uint32 id, kept in radius-rank order as the reference keeps
+ * particles[5][npts]) and one HIP stream.  A context is not thread-safe.
+ *
+ * Every entry point returns 0 on success or a positive NS_ERR_* code; the
+ * message of the last failure on the calling thread is ns_gpu_last_error().
+ * Nothing here terminates the process (the drop-in layer in nsphere_dropin.h
+ * does, as the reference does on failure).
+ *
+ * Reference interfaces replaced (kris-sigurdson/NSphere v0.1.2, src/c/nsphere.c):
+ *   ns_gpu_sort_particles  <- sort_particles / sort_particles_with_alg
+ *                             (declared 880, 903; defined 8662-8665, 8605-8652)
+ *   ns_gpu_inverse_map     <- the inverse-map loop, 5789-5794
+ *   ns_gpu_integrate       <- the per-particle loop, 5796-5838, which calls
+ *                             doAdaptiveFullLeap (1648-1735) and
+ *                             doAdaptiveFullLeviCivita (2231-2334)
+ *   ns_gpu_step            <- the whole method-5 body, 5775-5838
+ *   ns_gpu_upload/download <- the caller-owned double *particles[5] layout,
+ *                             4656-4682 (r, v_r, ell, id, mu; mu is not used)
+ */
+#ifndef NSPHERE_GPU_H
+#define NSPHERE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NS_OK = 0,
+    NS_ERR_ARG = 1,     /* invalid argument (null pointer, bad size or option) */
+    NS_ERR_HIP = 2,     /* a HIP runtime call failed                           */
+    NS_ERR_NOMEM = 3,   /* device or host allocation failed                    */
+    NS_ERR_STATE = 4,   /* call out of order (e.g. step before upload)         */
+    NS_ERR_NODEV = 5    /* no usable gfx950 device                             */
+};
+
+/* Parameters of one step.  Field meaning follows the method-5 constants of
+ * nsphere.c:5777-5782 and the force constants of nsphere.c:364-373, 513:
+ * gravity on rank i is -(vel_conv_sq*G) * (i/npts) * halo_mass / r^2. */
+typedef struct ns_step_params {
+    double dt;
+    double G;             /* G_CONST = 4.3e-6 kpc (km/s)^2 / Msun by default       */
+    double halo_mass;     /* HALO_MASS = 1e12 Msun by default                      */
+    double vel_conv_sq;   /* VEL_CONV_SQ = (1.02271e-3)^2 by default               */
+    double radius_tol;    /* 1e-7                                                  */
+    double velocity_tol;  /* 1e-7                                                  */
+    double alpha;         /* r_crit = alpha ell^2 / (vel_conv_sq G M_enc), 0.05   */
+    int32_t max_subdiv;   /* 4096*4096*16                                          */
+    int32_t out_type;     /* 0 coarse, 1 fine, 2 4*fine-3*coarse (default)         */
+    int32_t N_taumin;     /* 10                                                    */
+    int32_t identity_gravity; /* 1: gravity switched off (nsphere.c:476)           */
+} ns_step_params;
+
+/* Fill *p with the reference's method-5 values in its kpc/Myr/Msun units. */
+void ns_step_params_default(ns_step_params *p);
+/* Same loop constants in G = M = 1 units (vel_conv_sq = 1, dt = 1e-3). */
+void ns_step_params_unit(ns_step_params *p);
+
+/* Per-particle refinement record of the last integrate (one byte each,
+ * in the rank order of that step):
+ *   bits 0-5 deepest level K reached (N = 2^K subdivisions), saturating at 63;
+ *   bit 6    the adaptive loop hit max_subdiv without converging;
+ *   bit 7    the particle took the Levi-Civita branch. */
+#define NS_LVL_MASK 0x3F
+#define NS_LVL_NOCONV 0x40
+#define NS_LVL_LC 0x80
+
+/* Options for ns_gpu_set_option. */
+enum {
+    NS_OPT_TIEBREAK = 1,   /* 0: stable on input order, exactly as quadsort
+                              (default); 1: exact ties broken by particle id    */
+    NS_OPT_PROFILE = 2,    /* 1: time every kernel with HIP events             */
+    NS_OPT_INTEGRATOR = 3  /* 0: strict IEEE order of the reference (default);
+                              1: fast (FMA-contracted, reciprocal reuse)        */
+};
+
+typedef struct ns_gpu_ctx ns_gpu_ctx;
+
+const char *ns_gpu_last_error(void);
+/* Library version string and the gfx target it was built for. */
+const char *ns_gpu_version(void);
+
+int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out);
+int ns_gpu_ctx_destroy(ns_gpu_ctx *ctx);
+int ns_gpu_set_option(ns_gpu_ctx *ctx, int option, int64_t value);
+int64_t ns_gpu_npts(const ns_gpu_ctx *ctx);
+
+/* particles[0..3] = r, v_r, ell, id (as double), the reference's layout;
+ * particles[4] (mu) is neither read nor written. */
+int ns_gpu_upload(ns_gpu_ctx *ctx, double *const particles[5]);
+int ns_gpu_download(ns_gpu_ctx *ctx, double *particles[5]);
+/* Typed SoA host arrays (any of the outputs may be NULL). */
+int ns_gpu_upload_soa(ns_gpu_ctx *ctx, const double *r, const double *v,
+                      const double *ell, const uint32_t *id);
+int ns_gpu_download_soa(ns_gpu_ctx *ctx, double *r, double *v, double *ell,
+                        uint32_t *id);
+/* Device pointers of the current state (valid until the next step). */
+int ns_gpu_device_state(ns_gpu_ctx *ctx, double **r, double **v, double **ell,
+                        uint32_t **id);
+
+/* sort_particles semantics: reorder all columns by radius, in place. */
+int ns_gpu_sort_particles(ns_gpu_ctx *ctx);
+/* inverse_map[id] = rank for the current order (host_out may be NULL to
+ * keep it on the device only). */
+int ns_gpu_inverse_map(ns_gpu_ctx *ctx, int32_t *host_out);
+/* The per-particle loop at frozen rank (no sort): particles must already be
+ * in radius-rank order, as after ns_gpu_sort_particles. */
+int ns_gpu_integrate(ns_gpu_ctx *ctx, const ns_step_params *p);
+/* One method-5 step: sort, then integrate at the new ranks.  Asynchronous:
+ * returns once the work is queued on the context's stream. */
+int ns_gpu_step(ns_gpu_ctx *ctx, const ns_step_params *p);
+/* nsteps steps back to back, queued without host synchronisation. */
+int ns_gpu_run(ns_gpu_ctx *ctx, const ns_step_params *p, int64_t nsteps);
+int ns_gpu_sync(ns_gpu_ctx *ctx);
+
+/* Level records of the last integrate (npts bytes, rank order). */
+int ns_gpu_levels(ns_gpu_ctx *ctx, uint8_t *host_out);
+/* Shell-model energy of the current state with the rank-based mass of
+ * nsphere.c:513: K = sum m (v^2 + ell^2/r^2)/2,
+ * W = -sum (vel_conv_sq G) (i/npts) M m / r_i, m = M/npts, float64 sums. */
+int ns_gpu_energy(ns_gpu_ctx *ctx, const ns_step_params *p, double *K, double *W);
+
+/* Kernel timing (NS_OPT_PROFILE=1): for each kernel family, the number of
+ * launches and total milliseconds measured with HIP events on the context's
+ * stream since the last reset.  names[i] point at static strings. */
+#define NS_MAX_KERNEL_STATS 16
+int ns_gpu_kernel_stats(ns_gpu_ctx *ctx, int *count, const char **names,
+                        int64_t *launches, double *total_ms);
+int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);
+
+/* Seeded initial conditions in G = M = a = 1 units (host, deterministic per
+ * particle through a Philox-4x32-10 counter stream keyed by seed):
+ * kind 0 Plummer (inverse-CDF radii, Aarseth-Henon-Wielen speeds),
+ * kind 1 Hernquist (inverse-CDF radii, Gaussian velocities with the isotropic
+ * Jeans dispersion), kind 2 half Plummer (even ids) half Hernquist (odd ids).
+ * Radii beyond rmax are redrawn; a fraction lowl_frac of particles (Bernoulli
+ * pick) get ell scaled by lowl_scale.  id[i] = i. */
+int ns_ic_generate(int kind, int64_t n, uint64_t seed, double rmax, double lowl_frac,
+                   double lowl_scale, double *r, double *v, double *ell, uint32_t *id);
+/* Philox-4x32-10 block (exposed for known-answer tests). */
+void ns_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""nsphere_amd -- MI355X-native per-step evolution loop of NSphere.
+
+Host-side mirror of the reference's hot-path interface over the C-ABI of
+libnsphere_gpu.so (include/nsphere_gpu.h):
+
+* ``sort_particles(particles, npts)`` -- nsphere.c:8662-8665 (in place, all
+  columns move together, stable on radius);
+* ``method5_step(particles, npts, dt, inverse_map)`` -- the method-5 loop body,
+  nsphere.c:5775-5838 (sort, inverse map, adaptive integration at frozen rank);
+* ``GpuContext`` -- device-resident state for running many steps without
+  host round trips.
+
+Particles use the reference's layout: ``particles`` is a (5, npts) float64
+array holding r, v_r, ell, id (as a double) and mu (ignored).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import (NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV, NS_OPT_INTEGRATOR, NS_OPT_PROFILE,
+                   NS_OPT_TIEBREAK, NsError, StepParams, check, lib)
+
+__all__ = [
+    "GpuContext", "StepParams", "NsError", "default_params", "unit_params", "generate_ic",
+    "sort_particles", "method5_step", "philox4x32_10",
+    "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
+    "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR",
+]
+
+IC_PLUMMER, IC_HERNQUIST, IC_MIX = 0, 1, 2
+
+
+def default_params(**kw) -> StepParams:
+    """Method-5 constants in the reference's kpc / Myr / Msun units."""
+    p = StepParams()
+    lib.ns_step_params_default(ctypes.byref(p))
+    return p.replace(**kw) if kw else p
+
+
+def unit_params(**kw) -> StepParams:
+    """Method-5 constants with G = M = 1 and no velocity conversion."""
+    p = StepParams()
+    lib.ns_step_params_unit(ctypes.byref(p))
+    return p.replace(**kw) if kw else p
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _c_f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def generate_ic(kind: int, n: int, seed: int, rmax: float = 100.0, lowl_frac: float = 0.0,
+                lowl_scale: float = 1e-3):
+    """Seeded Plummer (0) / Hernquist (1) / mixed (2) sphere in G = M = a = 1 units.
+
+    Returns (r, v_r, ell, id) as float64, float64, float64, uint32 arrays.
+    """
+    r = np.empty(n, np.float64)
+    v = np.empty(n, np.float64)
+    ell = np.empty(n, np.float64)
+    ids = np.empty(n, np.uint32)
+    check(lib.ns_ic_generate(int(kind), int(n), ctypes.c_uint64(seed), float(rmax), float(lowl_frac),
+                             float(lowl_scale), _ptr(r), _ptr(v), _ptr(ell), _ptr(ids)))
+    return r, v, ell, ids
+
+
+def philox4x32_10(ctr, key) -> tuple:
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib.ns_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+class GpuContext:
+    """One GPU's copy of the particle state plus its HIP stream."""
+
+    def __init__(self, npts: int, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib.ns_gpu_ctx_create(int(npts), int(device), ctypes.byref(h)))
+        self._h = h
+        self.npts = int(npts)
+
+    # -- lifetime -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.ns_gpu_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- options --------------------------------------------------------
+    def set_option(self, option: int, value: int) -> None:
+        check(lib.ns_gpu_set_option(self._h, int(option), int(value)))
+
+    # -- data movement --------------------------------------------------
+    def upload(self, r, v, ell, ids) -> None:
+        r, v, ell = _c_f64(r), _c_f64(v), _c_f64(ell)
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        for a in (r, v, ell, ids):
+            if a.shape != (self.npts,):
+                raise ValueError(f"expected arrays of shape ({self.npts},), got {a.shape}")
+        check(lib.ns_gpu_upload_soa(self._h, _ptr(r), _ptr(v), _ptr(ell), _ptr(ids)))
+
+    def upload_particles(self, particles: np.ndarray) -> None:
+        """Upload the reference's particles[5][npts] layout."""
+        cols = [np.ascontiguousarray(particles[k], dtype=np.float64) for k in range(5)]
+        arr = (ctypes.POINTER(ctypes.c_double) * 5)(
+            *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols])
+        check(lib.ns_gpu_upload(self._h, arr))
+
+    def download(self):
+        r = np.empty(self.npts, np.float64)
+        v = np.empty(self.npts, np.float64)
+        ell = np.empty(self.npts, np.float64)
+        ids = np.empty(self.npts, np.uint32)
+        check(lib.ns_gpu_download_soa(self._h, _ptr(r), _ptr(v), _ptr(ell), _ptr(ids)))
+        return r, v, ell, ids
+
+    def download_particles(self, particles: np.ndarray) -> None:
+        """Write r, v_r, ell, id back into a (5, npts) float64 array in place."""
+        if particles.dtype != np.float64 or not particles.flags.c_contiguous:
+            raise ValueError("particles must be a C-contiguous float64 (5, npts) array")
+        arr = (ctypes.POINTER(ctypes.c_double) * 5)(
+            *[particles[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for k in range(5)])
+        check(lib.ns_gpu_download(self._h, arr))
+
+    # -- the loop -------------------------------------------------------
+    def sort_particles(self) -> None:
+        check(lib.ns_gpu_sort_particles(self._h))
+
+    def inverse_map(self) -> np.ndarray:
+        out = np.empty(self.npts, np.int32)
+        check(lib.ns_gpu_inverse_map(self._h, _ptr(out)))
+        return out
+
+    def integrate(self, params: StepParams) -> None:
+        check(lib.ns_gpu_integrate(self._h, ctypes.byref(params)))
+
+    def step(self, params: StepParams) -> None:
+        check(lib.ns_gpu_step(self._h, ctypes.byref(params)))
+
+    def run(self, params: StepParams, nsteps: int) -> None:
+        check(lib.ns_gpu_run(self._h, ctypes.byref(params), int(nsteps)))
+
+    def sync(self) -> None:
+        check(lib.ns_gpu_sync(self._h))
+
+    def levels(self) -> np.ndarray:
+        out = np.empty(self.npts, np.uint8)
+        check(lib.ns_gpu_levels(self._h, _ptr(out)))
+        return out
+
+    def energy(self, params: StepParams):
+        k = ctypes.c_double()
+        w = ctypes.c_double()
+        check(lib.ns_gpu_energy(self._h, ctypes.byref(params), ctypes.byref(k), ctypes.byref(w)))
+        return k.value, w.value
+
+    def kernel_stats(self) -> dict:
+        n = ctypes.c_int()
+        names = (ctypes.c_char_p * 16)()
+        launches = (ctypes.c_int64 * 16)()
+        ms = (ctypes.c_double * 16)()
+        check(lib.ns_gpu_kernel_stats(self._h, ctypes.byref(n), names, launches, ms))
+        return {names[i].decode(): (launches[i], ms[i]) for i in range(n.value) if launches[i]}
+
+    def reset_kernel_stats(self) -> None:
+        check(lib.ns_gpu_kernel_stats_reset(self._h))
+
+
+# -- reference-interface mirror -------------------------------------------
+
+_ctx_cache: dict = {}
+
+
+def _ctx_for(npts: int) -> GpuContext:
+    ctx = _ctx_cache.get(npts)
+    if ctx is None:
+        _ctx_cache.clear()
+        ctx = GpuContext(npts)
+        _ctx_cache[npts] = ctx
+    return ctx
+
+
+def sort_particles(particles: np.ndarray, npts: int) -> None:
+    """sort_particles(particles, npts), nsphere.c:8662-8665: stable radial sort
+    of all columns of a (5, npts) float64 array, in place, on the GPU."""
+    ctx = _ctx_for(npts)
+    ctx.upload_particles(particles)
+    ctx.sort_particles()
+    ctx.download_particles(particles)
+
+
+def method5_step(particles: np.ndarray, npts: int, params: StepParams,
+                 inverse_map: np.ndarray | None = None) -> np.ndarray:
+    """One method-5 step (nsphere.c:5775-5838) on a (5, npts) array in place.
+
+    Returns the level records (rank order).  When ``inverse_map`` is given it
+    receives inverse_map[id] = rank as in nsphere.c:5789-5794.
+    """
+    ctx = _ctx_for(npts)
+    ctx.upload_particles(particles)
+    ctx.step(params)
+    ctx.sync()
+    if inverse_map is not None:
+        inverse_map[:] = ctx.inverse_map()
+    ctx.download_particles(particles)
+    return ctx.levels()

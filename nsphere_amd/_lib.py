@@ -1,0 +1,111 @@
+"""ctypes binding of libnsphere_gpu.so (the C-ABI declared in include/nsphere_gpu.h).
+
+The product path has no fallback: if the native library is missing or was
+built for another target, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsphere_gpu.so")
+
+NS_OK, NS_ERR_ARG, NS_ERR_HIP, NS_ERR_NOMEM, NS_ERR_STATE, NS_ERR_NODEV = range(6)
+NS_OPT_TIEBREAK, NS_OPT_PROFILE, NS_OPT_INTEGRATOR = 1, 2, 3
+NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80
+NS_MAX_KERNEL_STATS = 16
+
+
+class NativeLibraryMissing(ImportError):
+    pass
+
+
+class NsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class StepParams(ctypes.Structure):
+    """Mirror of ns_step_params (method-5 constants, nsphere.c:5777-5782)."""
+
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("G", ctypes.c_double),
+        ("halo_mass", ctypes.c_double),
+        ("vel_conv_sq", ctypes.c_double),
+        ("radius_tol", ctypes.c_double),
+        ("velocity_tol", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("max_subdiv", ctypes.c_int32),
+        ("out_type", ctypes.c_int32),
+        ("N_taumin", ctypes.c_int32),
+        ("identity_gravity", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+    def replace(self, **kw) -> "StepParams":
+        p = StepParams()
+        ctypes.pointer(p)[0] = self
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryMissing(
+            f"{LIB_PATH} is not built; run `python -m nsphere_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    c = ctypes
+    vp, i64, i32, dbl = c.c_void_p, c.c_int64, c.c_int, c.c_double
+    pd = c.POINTER(c.c_double)
+    sig = {
+        "ns_gpu_last_error": (c.c_char_p, []),
+        "ns_gpu_version": (c.c_char_p, []),
+        "ns_step_params_default": (None, [c.POINTER(StepParams)]),
+        "ns_step_params_unit": (None, [c.POINTER(StepParams)]),
+        "ns_gpu_ctx_create": (i32, [i64, i32, c.POINTER(vp)]),
+        "ns_gpu_ctx_destroy": (i32, [vp]),
+        "ns_gpu_set_option": (i32, [vp, i32, i64]),
+        "ns_gpu_npts": (i64, [vp]),
+        "ns_gpu_upload": (i32, [vp, c.POINTER(pd)]),
+        "ns_gpu_download": (i32, [vp, c.POINTER(pd)]),
+        "ns_gpu_upload_soa": (i32, [vp, vp, vp, vp, vp]),
+        "ns_gpu_download_soa": (i32, [vp, vp, vp, vp, vp]),
+        "ns_gpu_device_state": (i32, [vp, c.POINTER(vp), c.POINTER(vp), c.POINTER(vp), c.POINTER(vp)]),
+        "ns_gpu_sort_particles": (i32, [vp]),
+        "ns_gpu_inverse_map": (i32, [vp, vp]),
+        "ns_gpu_integrate": (i32, [vp, c.POINTER(StepParams)]),
+        "ns_gpu_step": (i32, [vp, c.POINTER(StepParams)]),
+        "ns_gpu_run": (i32, [vp, c.POINTER(StepParams), i64]),
+        "ns_gpu_sync": (i32, [vp]),
+        "ns_gpu_levels": (i32, [vp, vp]),
+        "ns_gpu_energy": (i32, [vp, c.POINTER(StepParams), pd, pd]),
+        "ns_gpu_kernel_stats": (i32, [vp, c.POINTER(c.c_int), c.POINTER(c.c_char_p),
+                                      c.POINTER(c.c_int64), c.POINTER(c.c_double)]),
+        "ns_gpu_kernel_stats_reset": (i32, [vp]),
+        "ns_ic_generate": (i32, [i32, i64, c.c_uint64, dbl, dbl, dbl, vp, vp, vp, vp]),
+        "ns_philox4x32_10": (None, [c.POINTER(c.c_uint32), c.POINTER(c.c_uint32), c.POINTER(c.c_uint32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc: int) -> None:
+    if rc != NS_OK:
+        msg = lib.ns_gpu_last_error()
+        raise NsError(rc, msg.decode() if msg else "unknown error")
+
+
+def exported_symbols() -> list[str]:
+    return [name for name in dir(lib) if name.startswith("ns_")]

@@ -1,0 +1,678 @@
+// ns_api.hip -- context, memory and the C-ABI entry points of nsphere_amd.
+//
+// A context holds one GPU's copy of the particle state as SoA arrays in HBM,
+// double-buffered (the step gathers from buffer `cur` into buffer `cur^1` in
+// the new rank order), plus the radix-sort workspace.  Bytes per particle:
+// 2 x (3 x 8 + 4) state + 2 x (8 + 4) sort ping-pong + 4 permutation + 1 level
+// + 4 inverse map = 89 B (about 1.1 GB at 1.2e7 particles).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "ns_device.h"
+#include "ns_internal.h"
+#include "ns_kernels.h"
+
+struct ns_gpu_ctx {
+    int device = 0;
+    int64_t n = 0;
+    hipStream_t stream = nullptr;
+    int cur = 0;
+    double *r[2] = {}, *v[2] = {}, *l[2] = {};
+    uint32_t *id[2] = {};
+    uint8_t *lvl = nullptr;
+    uint32_t *perm = nullptr;
+    int32_t *inv = nullptr;
+    ns::SortWork sw = {};
+    double *energy_partial = nullptr;
+    double *energy_out = nullptr;  // device, 2 doubles
+    unsigned long long *budget_hits = nullptr;
+    bool uploaded = false;
+    bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
+    bool levels_valid = false;
+    int tiebreak = 0;
+    int integrator = 0;
+    int64_t budget = (int64_t)1 << 26;
+    ns::Profiler prof;
+};
+
+namespace ns {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+void clear_error() { g_err[0] = 0; }
+
+const char *kernel_name(int id)
+{
+    static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_onesweep",
+                                         "gather_state", "integrate", "inverse_map", "energy",
+                                         "sort_tiefix"};
+    return (id >= 0 && id < K_COUNT) ? names[id] : "?";
+}
+
+hipEvent_t Profiler::take()
+{
+    if (npool_ > 0)
+        return pool_[--npool_];
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+void Profiler::begin(int kid)
+{
+    if (!on)
+        return;
+    open_ = take();
+    open_kid_ = kid;
+    if (open_)
+        (void)hipEventRecord(open_, stream);
+}
+
+void Profiler::end(int kid)
+{
+    if (!on || !open_ || open_kid_ != kid)
+        return;
+    hipEvent_t b = take();
+    if (!b)
+        return;
+    (void)hipEventRecord(b, stream);
+    if (npend_ == cap_) {
+        int nc = cap_ ? 2 * cap_ : 256;
+        Pending *np = (Pending *)realloc(pend_, (size_t)nc * sizeof(Pending));
+        if (!np)
+            return;
+        pend_ = np;
+        cap_ = nc;
+    }
+    pend_[npend_++] = Pending{kid, open_, b};
+    open_ = nullptr;
+    if (npend_ >= 4096)
+        fold();
+}
+
+hipError_t Profiler::fold()
+{
+    hipError_t err = hipSuccess;
+    for (int i = 0; i < npend_; i++) {
+        hipError_t e = hipEventSynchronize(pend_[i].b);
+        float ms_ = 0.f;
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&ms_, pend_[i].a, pend_[i].b);
+        if (e == hipSuccess) {
+            launches[pend_[i].kid]++;
+            ms[pend_[i].kid] += ms_;
+        } else {
+            err = e;
+        }
+        for (hipEvent_t ev : {pend_[i].a, pend_[i].b}) {
+            if (npool_ == pool_cap_) {
+                int nc = pool_cap_ ? 2 * pool_cap_ : 512;
+                hipEvent_t *np = (hipEvent_t *)realloc(pool_, (size_t)nc * sizeof(hipEvent_t));
+                if (!np) {
+                    (void)hipEventDestroy(ev);
+                    continue;
+                }
+                pool_ = np;
+                pool_cap_ = nc;
+            }
+            pool_[npool_++] = ev;
+        }
+    }
+    npend_ = 0;
+    return err;
+}
+
+void Profiler::reset()
+{
+    fold();
+    for (int k = 0; k < K_COUNT; k++) {
+        launches[k] = 0;
+        ms[k] = 0.0;
+    }
+}
+
+void Profiler::release()
+{
+    fold();
+    for (int i = 0; i < npool_; i++)
+        (void)hipEventDestroy(pool_[i]);
+    free(pool_);
+    free(pend_);
+    pool_ = nullptr;
+    pend_ = nullptr;
+    npool_ = pool_cap_ = npend_ = cap_ = 0;
+}
+
+} // namespace ns
+
+using ns::set_error;
+
+namespace {
+
+#define HIP_TRY(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_error(NS_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                            \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+hipError_t dalloc(T **p, int64_t count)
+{
+    size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
+    return hipMalloc((void **)p, bytes);
+}
+
+void free_all(ns_gpu_ctx *c)
+{
+    void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
+                    c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
+                    c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
+                    c->energy_out, c->budget_hits};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+}
+
+int check_ctx(const ns_gpu_ctx *c, const char *fn)
+{
+    if (!c)
+        return set_error(NS_ERR_ARG, "%s: null context", fn);
+    return NS_OK;
+}
+
+int check_params(const ns_step_params *p, const char *fn)
+{
+    if (!p)
+        return set_error(NS_ERR_ARG, "%s: null params", fn);
+    if (!(p->dt > 0.0) || p->out_type < 0 || p->out_type > 2)
+        return set_error(NS_ERR_ARG, "%s: dt must be > 0 and out_type in {0,1,2}", fn);
+    return NS_OK;
+}
+
+ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
+{
+    ns::StepArgs a{};
+    a.n = c->n;
+    a.dt = p->dt;
+    a.G = p->G;
+    a.halo_mass = p->halo_mass;
+    a.vel_conv_sq = p->vel_conv_sq;
+    a.radius_tol = p->radius_tol;
+    a.velocity_tol = p->velocity_tol;
+    a.alpha = p->alpha;
+    a.max_subdiv = p->max_subdiv;
+    a.out_type = p->out_type;
+    a.N_taumin = p->N_taumin;
+    a.identity_gravity = p->identity_gravity;
+    a.budget = c->budget;
+    a.budget_hits = c->budget_hits;
+    return a;
+}
+
+// Sort r[cur] into c->perm (keys from the integrate epilogue when valid).
+int sort_perm(ns_gpu_ctx *c)
+{
+    if (!c->keys_valid) {
+        c->prof.begin(ns::K_KEYS);
+        HIP_TRY(ns::launch_radius_keys(c->r[c->cur], c->sw.keys[0], c->n, c->stream));
+        c->prof.end(ns::K_KEYS);
+    }
+    HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->tiebreak ? c->id[c->cur] : nullptr, c->perm, c->stream,
+                                &c->prof));
+    c->keys_valid = false;  // the passes overwrite keys[0]
+    return NS_OK;
+}
+
+int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
+{
+    ns::StepArgs a = make_args(c, p);
+    // gather: previous order in buffer cur -> new rank order in cur^1;
+    // otherwise in place (each thread reads and writes only its own index).
+    const int s = c->cur, d = gather ? (c->cur ^ 1) : c->cur;
+    a.r0 = c->r[s];
+    a.v0 = c->v[s];
+    a.l0 = c->l[s];
+    a.id0 = c->id[s];
+    a.perm = gather ? c->perm : nullptr;
+    a.r1 = c->r[d];
+    a.v1 = c->v[d];
+    a.l1 = gather ? c->l[d] : nullptr;
+    a.id1 = gather ? c->id[d] : nullptr;
+    a.keys = c->sw.keys[0];
+    a.lvl = c->lvl;
+    c->prof.begin(ns::K_INTEGRATE);
+    HIP_TRY(ns::launch_integrate(a, gather, c->stream));
+    c->prof.end(ns::K_INTEGRATE);
+    c->cur = d;
+    c->keys_valid = true;
+    c->levels_valid = true;
+    return NS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *ns_gpu_last_error(void) { return ns::g_err; }
+
+const char *ns_gpu_version(void) { return "nsphere_amd 0.1 (gfx950, HIP)"; }
+
+void ns_step_params_default(ns_step_params *p)
+{
+    if (!p)
+        return;
+    const double kmsec_to_kpcmyr = 1.02271e-3;
+    p->dt = 1.0e-3;
+    p->G = 4.3e-6;
+    p->halo_mass = 1.0e12;
+    p->vel_conv_sq = kmsec_to_kpcmyr * kmsec_to_kpcmyr;
+    p->radius_tol = 1.0e-7;
+    p->velocity_tol = 1.0e-7;
+    p->alpha = 0.05;
+    p->max_subdiv = 4096 * 4096 * 16;
+    p->out_type = 2;
+    p->N_taumin = 10;
+    p->identity_gravity = 0;
+}
+
+void ns_step_params_unit(ns_step_params *p)
+{
+    if (!p)
+        return;
+    ns_step_params_default(p);
+    p->G = 1.0;
+    p->halo_mass = 1.0;
+    p->vel_conv_sq = 1.0;
+}
+
+int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
+{
+    ns::clear_error();
+    if (!out)
+        return set_error(NS_ERR_ARG, "ns_gpu_ctx_create: null out");
+    *out = nullptr;
+    if (npts < 0 || npts > (int64_t)INT32_MAX)
+        return set_error(NS_ERR_ARG, "ns_gpu_ctx_create: npts %lld out of range [0, 2^31)", (long long)npts);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(NS_ERR_NODEV, "ns_gpu_ctx_create: no HIP device");
+    if (device < 0 || device >= ndev)
+        return set_error(NS_ERR_ARG, "ns_gpu_ctx_create: device %d not in [0,%d)", device, ndev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return set_error(NS_ERR_NODEV, "ns_gpu_ctx_create: cannot query device %d", device);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(NS_ERR_NODEV, "ns_gpu_ctx_create: device %d is %s, library built for gfx950", device,
+                         prop.gcnArchName);
+
+    DeviceGuard g(device);
+    ns_gpu_ctx *c = new (std::nothrow) ns_gpu_ctx();
+    if (!c)
+        return set_error(NS_ERR_NOMEM, "ns_gpu_ctx_create: host allocation");
+    c->device = device;
+    c->n = npts;
+    const int64_t n = npts;
+    const int64_t tiles = ns::sort_tiles(n) > 0 ? ns::sort_tiles(n) : 1;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) {
+        if (e == hipSuccess) e = dalloc(&c->r[b], n);
+        if (e == hipSuccess) e = dalloc(&c->v[b], n);
+        if (e == hipSuccess) e = dalloc(&c->l[b], n);
+        if (e == hipSuccess) e = dalloc(&c->id[b], n);
+        if (e == hipSuccess) e = dalloc(&c->sw.keys[b], n);
+        if (e == hipSuccess) e = dalloc(&c->sw.vals[b], n);
+    }
+    if (e == hipSuccess) e = dalloc(&c->lvl, n);
+    if (e == hipSuccess) e = dalloc(&c->perm, n);
+    if (e == hipSuccess) e = dalloc(&c->inv, n);
+    if (e == hipSuccess) e = dalloc(&c->sw.hist, 2 * ns::kKeyPasses * ns::kRadix);
+    if (e == hipSuccess) e = dalloc(&c->sw.lookback, tiles * ns::kRadix);
+    if (e == hipSuccess) e = dalloc(&c->sw.counters, 64);
+    if (e == hipSuccess) e = dalloc(&c->energy_partial, 2 * ns::energy_blocks(n));
+    if (e == hipSuccess) e = dalloc(&c->energy_out, 2);
+    if (e == hipSuccess) e = dalloc(&c->budget_hits, 1);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        int code = (e == hipErrorOutOfMemory) ? NS_ERR_NOMEM : NS_ERR_HIP;
+        set_error(code, "ns_gpu_ctx_create(npts=%lld): %s", (long long)npts, hipGetErrorString(e));
+        free_all(c);
+        if (c->stream)
+            (void)hipStreamDestroy(c->stream);
+        delete c;
+        return code;
+    }
+    c->sw.epoch = 1;
+    c->prof.stream = c->stream;
+    *out = c;
+    return NS_OK;
+}
+
+int ns_gpu_ctx_destroy(ns_gpu_ctx *c)
+{
+    if (!c)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->prof.release();
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NS_OK;
+}
+
+int64_t ns_gpu_npts(const ns_gpu_ctx *c) { return c ? c->n : -1; }
+
+int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
+{
+    if (int rc = check_ctx(c, "ns_gpu_set_option"))
+        return rc;
+    switch (option) {
+    case NS_OPT_TIEBREAK:
+        if (value != 0 && value != 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_TIEBREAK takes 0 or 1");
+        if (value == 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_TIEBREAK=1 (id tiebreak) is not available in this build");
+        c->tiebreak = (int)value;
+        return NS_OK;
+    case NS_OPT_PROFILE:
+        c->prof.on = value != 0;
+        return NS_OK;
+    case NS_OPT_INTEGRATOR:
+        if (value != 0)
+            return set_error(NS_ERR_ARG, "NS_OPT_INTEGRATOR: only 0 (strict) is available in this build");
+        c->integrator = (int)value;
+        return NS_OK;
+    default:
+        return set_error(NS_ERR_ARG, "ns_gpu_set_option: unknown option %d", option);
+    }
+}
+
+int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const double *ell, const uint32_t *id)
+{
+    if (int rc = check_ctx(c, "ns_gpu_upload_soa"))
+        return rc;
+    if (c->n > 0 && (!r || !v || !ell || !id))
+        return set_error(NS_ERR_ARG, "ns_gpu_upload_soa: null array");
+    DeviceGuard g(c->device);
+    const size_t nd = (size_t)c->n * sizeof(double), ni = (size_t)c->n * sizeof(uint32_t);
+    c->cur = 0;
+    if (c->n > 0) {
+        HIP_TRY(hipMemcpyAsync(c->r[0], r, nd, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->v[0], v, nd, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->l[0], ell, nd, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->id[0], id, ni, hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->keys_valid = false;
+    c->levels_valid = false;
+    return NS_OK;
+}
+
+int ns_gpu_upload(ns_gpu_ctx *c, double *const particles[5])
+{
+    if (int rc = check_ctx(c, "ns_gpu_upload"))
+        return rc;
+    if (!particles || (c->n > 0 && (!particles[0] || !particles[1] || !particles[2] || !particles[3])))
+        return set_error(NS_ERR_ARG, "ns_gpu_upload: null particle column");
+    uint32_t *ids = (uint32_t *)malloc((size_t)(c->n > 0 ? c->n : 1) * sizeof(uint32_t));
+    if (!ids)
+        return set_error(NS_ERR_NOMEM, "ns_gpu_upload: host allocation");
+    for (int64_t i = 0; i < c->n; i++) {
+        const double x = particles[3][i];
+        if (!(x >= 0.0 && x < 4294967296.0)) {
+            free(ids);
+            return set_error(NS_ERR_ARG, "ns_gpu_upload: id %g at %lld is not a uint32", x, (long long)i);
+        }
+        ids[i] = (uint32_t)x;
+    }
+    int rc = ns_gpu_upload_soa(c, particles[0], particles[1], particles[2], ids);
+    free(ids);
+    return rc;
+}
+
+int ns_gpu_download_soa(ns_gpu_ctx *c, double *r, double *v, double *ell, uint32_t *id)
+{
+    if (int rc = check_ctx(c, "ns_gpu_download_soa"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_download_soa: no state uploaded");
+    DeviceGuard g(c->device);
+    const size_t nd = (size_t)c->n * sizeof(double), ni = (size_t)c->n * sizeof(uint32_t);
+    const int s = c->cur;
+    if (c->n > 0) {
+        if (r) HIP_TRY(hipMemcpyAsync(r, c->r[s], nd, hipMemcpyDeviceToHost, c->stream));
+        if (v) HIP_TRY(hipMemcpyAsync(v, c->v[s], nd, hipMemcpyDeviceToHost, c->stream));
+        if (ell) HIP_TRY(hipMemcpyAsync(ell, c->l[s], nd, hipMemcpyDeviceToHost, c->stream));
+        if (id) HIP_TRY(hipMemcpyAsync(id, c->id[s], ni, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int ns_gpu_download(ns_gpu_ctx *c, double *particles[5])
+{
+    if (int rc = check_ctx(c, "ns_gpu_download"))
+        return rc;
+    if (!particles || (c->n > 0 && (!particles[0] || !particles[1] || !particles[2] || !particles[3])))
+        return set_error(NS_ERR_ARG, "ns_gpu_download: null particle column");
+    uint32_t *ids = (uint32_t *)malloc((size_t)(c->n > 0 ? c->n : 1) * sizeof(uint32_t));
+    if (!ids)
+        return set_error(NS_ERR_NOMEM, "ns_gpu_download: host allocation");
+    int rc = ns_gpu_download_soa(c, particles[0], particles[1], particles[2], ids);
+    if (rc == NS_OK)
+        for (int64_t i = 0; i < c->n; i++)
+            particles[3][i] = (double)ids[i];
+    free(ids);
+    return rc;
+}
+
+int ns_gpu_device_state(ns_gpu_ctx *c, double **r, double **v, double **ell, uint32_t **id)
+{
+    if (int rc = check_ctx(c, "ns_gpu_device_state"))
+        return rc;
+    if (r) *r = c->r[c->cur];
+    if (v) *v = c->v[c->cur];
+    if (ell) *ell = c->l[c->cur];
+    if (id) *id = c->id[c->cur];
+    return NS_OK;
+}
+
+int ns_gpu_sort_particles(ns_gpu_ctx *c)
+{
+    if (int rc = check_ctx(c, "ns_gpu_sort_particles"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_sort_particles: no state uploaded");
+    DeviceGuard g(c->device);
+    if (int rc = sort_perm(c))
+        return rc;
+    const int s = c->cur, d = c->cur ^ 1;
+    c->prof.begin(ns::K_GATHER);
+    HIP_TRY(ns::launch_gather_state(c->r[s], c->v[s], c->l[s], c->id[s], c->perm, c->r[d], c->v[d], c->l[d],
+                                    c->id[d], c->n, c->stream));
+    c->prof.end(ns::K_GATHER);
+    c->cur = d;
+    c->keys_valid = false;
+    c->levels_valid = false;
+    return NS_OK;
+}
+
+int ns_gpu_inverse_map(ns_gpu_ctx *c, int32_t *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_inverse_map"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_inverse_map: no state uploaded");
+    DeviceGuard g(c->device);
+    c->prof.begin(ns::K_INVMAP);
+    HIP_TRY(ns::launch_inverse_map(c->id[c->cur], c->inv, c->n, c->stream));
+    c->prof.end(ns::K_INVMAP);
+    if (host_out && c->n > 0) {
+        HIP_TRY(hipMemcpyAsync(host_out, c->inv, (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return NS_OK;
+}
+
+int ns_gpu_integrate(ns_gpu_ctx *c, const ns_step_params *p)
+{
+    if (int rc = check_ctx(c, "ns_gpu_integrate"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_integrate"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_integrate: no state uploaded");
+    DeviceGuard g(c->device);
+    return integrate(c, p, false);
+}
+
+int ns_gpu_step(ns_gpu_ctx *c, const ns_step_params *p)
+{
+    if (int rc = check_ctx(c, "ns_gpu_step"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_step"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_step: no state uploaded");
+    DeviceGuard g(c->device);
+    if (int rc = sort_perm(c))
+        return rc;
+    return integrate(c, p, true);
+}
+
+int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
+{
+    if (nsteps < 0)
+        return set_error(NS_ERR_ARG, "ns_gpu_run: nsteps < 0");
+    for (int64_t s = 0; s < nsteps; s++)
+        if (int rc = ns_gpu_step(c, p))
+            return rc;
+    return NS_OK;
+}
+
+int ns_gpu_sync(ns_gpu_ctx *c)
+{
+    if (int rc = check_ctx(c, "ns_gpu_sync"))
+        return rc;
+    DeviceGuard g(c->device);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long hits = 0;
+    HIP_TRY(hipMemcpy(&hits, c->budget_hits, sizeof hits, hipMemcpyDeviceToHost));
+    if (hits)
+        return set_error(NS_ERR_STATE,
+                         "ns_gpu_sync: %llu particle-steps exceeded the per-particle work cap of %lld force "
+                         "evaluations (their records carry NS_LVL_NOCONV)",
+                         hits, (long long)c->budget);
+    return NS_OK;
+}
+
+int ns_gpu_levels(ns_gpu_ctx *c, uint8_t *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_levels"))
+        return rc;
+    if (!host_out && c->n > 0)
+        return set_error(NS_ERR_ARG, "ns_gpu_levels: null output");
+    if (!c->levels_valid)
+        return set_error(NS_ERR_STATE, "ns_gpu_levels: no integrate since the last upload/sort");
+    DeviceGuard g(c->device);
+    if (c->n > 0) {
+        HIP_TRY(hipMemcpyAsync(host_out, c->lvl, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return NS_OK;
+}
+
+int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
+{
+    if (int rc = check_ctx(c, "ns_gpu_energy"))
+        return rc;
+    if (!p || !K || !W)
+        return set_error(NS_ERR_ARG, "ns_gpu_energy: null argument");
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_energy: no state uploaded");
+    DeviceGuard g(c->device);
+    if (c->n == 0) {
+        *K = 0.0;
+        *W = 0.0;
+        return NS_OK;
+    }
+    const double m = p->halo_mass / (double)c->n;
+    c->prof.begin(ns::K_ENERGY);
+    HIP_TRY(ns::launch_energy(c->r[c->cur], c->v[c->cur], c->l[c->cur], c->n, m, p->vel_conv_sq * p->G,
+                              p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out, c->stream));
+    c->prof.end(ns::K_ENERGY);
+    double out[2];
+    HIP_TRY(hipMemcpyAsync(out, c->energy_out, sizeof out, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *K = out[0];
+    *W = out[1];
+    return NS_OK;
+}
+
+int ns_gpu_kernel_stats(ns_gpu_ctx *c, int *count, const char **names, int64_t *launches, double *total_ms)
+{
+    if (int rc = check_ctx(c, "ns_gpu_kernel_stats"))
+        return rc;
+    if (!count)
+        return set_error(NS_ERR_ARG, "ns_gpu_kernel_stats: null count");
+    DeviceGuard g(c->device);
+    HIP_TRY(c->prof.fold());
+    *count = ns::K_COUNT;
+    for (int k = 0; k < ns::K_COUNT; k++) {
+        if (names) names[k] = ns::kernel_name(k);
+        if (launches) launches[k] = c->prof.launches[k];
+        if (total_ms) total_ms[k] = c->prof.ms[k];
+    }
+    return NS_OK;
+}
+
+int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
+{
+    if (int rc = check_ctx(c, "ns_gpu_kernel_stats_reset"))
+        return rc;
+    DeviceGuard g(c->device);
+    c->prof.reset();
+    return NS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,105 @@
+// ns_device.h -- device-side helpers and kernel launchers shared by the
+// HIP translation units of nsphere_amd (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nsphere_gpu.h"
+
+namespace ns {
+
+// ---------------------------------------------------------------- sort sizes
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kKeyPasses = 64 / kRadixBits;
+constexpr int kSortThreads = 256;                      // 4 wave64 per workgroup
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 16;                         // keys per thread
+constexpr int kSortTile = kSortThreads * kSortItems;   // 4096 keys per tile
+constexpr int kHistThreads = 256;
+constexpr int kHistItems = 16;
+
+inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+// Order-preserving 64-bit radius key: unsigned order of the key equals the
+// order of compare_particles (nsphere.c:8297-8306) on non-NaN radii; -0 and
+// +0 map to one key (they compare equal there), every NaN maps to one key
+// after +inf.
+__host__ __device__ inline uint64_t radius_key(double r)
+{
+    uint64_t u;
+    if (r != r) {
+        u = 0x7FF8000000000000ull;
+    } else if (r == 0.0) {
+        u = 0;
+    } else {
+        u = (uint64_t)__builtin_bit_cast(uint64_t, r);
+    }
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Device workspace of one radix sort.
+struct SortWork {
+    uint64_t *keys[2];   // ping-pong key buffers (keys[0] holds the input)
+    uint32_t *vals[2];   // ping-pong payload buffers
+    uint32_t *hist;      // kKeyPasses * kRadix digit counts
+    uint64_t *lookback;  // sort_tiles(n) * kRadix decoupled-lookback words
+    uint32_t *counters;  // kKeyPasses tile tickets (+ spare)
+    uint32_t epoch;      // lookback epoch of the next pass (host-owned)
+};
+
+// Launchers (ns_sort.hip).  All asynchronous on `stream`.
+hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream);
+// Sorts keys[0] (stable); payload = input position, or ids (tiebreak) when
+// tie_ids != nullptr.  On return perm_out[k] = input position of rank k.
+// perm_out may alias neither vals buffer.
+struct Profiler;
+hipError_t radix_sort_perm(SortWork &w, int64_t n, const uint32_t *tie_ids, uint32_t *perm_out,
+                           hipStream_t stream, Profiler *prof);
+
+// Kernel-family ids for profiling.
+enum KernelId : int {
+    K_KEYS = 0,
+    K_SORT_ZERO,
+    K_SORT_HIST,
+    K_SORT_PASS,
+    K_GATHER,
+    K_INTEGRATE,
+    K_INVMAP,
+    K_ENERGY,
+    K_TIEFIX,
+    K_COUNT
+};
+const char *kernel_name(int id);
+
+// Per-kernel-family timing with HIP events on the launching stream.  begin()
+// and end() bracket one launch; fold() waits for recorded pairs and adds
+// their durations.  Disabled (no events) unless `on`.
+struct Profiler {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    int64_t launches[K_COUNT] = {};
+    double ms[K_COUNT] = {};
+    void begin(int kid);
+    void end(int kid);
+    hipError_t fold();
+    void reset();
+    void release();
+
+  private:
+    struct Pending {
+        int kid;
+        hipEvent_t a, b;
+    };
+    Pending *pend_ = nullptr;
+    int npend_ = 0, cap_ = 0;
+    hipEvent_t open_ = nullptr;
+    int open_kid_ = -1;
+    hipEvent_t *pool_ = nullptr;
+    int npool_ = 0, pool_cap_ = 0;
+    hipEvent_t take();
+};
+
+} // namespace ns

@@ -1,0 +1,392 @@
+// ns_sort.hip -- stable LSD radix sort of radius keys for MI355X (gfx950).
+//
+// Replaces the reference's radial sort (sort_particles_with_alg,
+// nsphere.c:8605-8652, with the stable quadsort of nsphere.c:8356-8359 and the
+// comparator of nsphere.c:8297-8306).  An LSD radix sort is stable, so keys
+// that tie keep their input (previous-step) order -- the order quadsort gives.
+//
+// Design (one HIP stream, wave64):
+//   * one upfront pass reads every key once and builds all eight 8-bit digit
+//     histograms (LDS-privatised, run-length-combined counters);
+//   * eight "onesweep" passes, one kernel each: a workgroup takes a 4096-key
+//     tile by atomic ticket, ranks its keys stably per digit with wave
+//     ballots (a 64-lane match of the 8 digit bits), publishes its digit
+//     counts and resolves its global digit offsets by decoupled lookback over
+//     earlier tiles, stages the tile in LDS in digit order and writes it out
+//     so that each digit run leaves as one coalesced segment;
+//   * pass 0 synthesises the payload (input positions) instead of reading
+//     it, and the last pass writes only the payload (the permutation).
+#include <hip/hip_runtime.h>
+
+#include "ns_device.h"
+
+namespace ns {
+
+namespace {
+
+// Lookback word: high 32 bits = epoch << 2 | status, low 32 bits = count.
+constexpr uint32_t kStatusAggregate = 1;
+constexpr uint32_t kStatusPrefix = 2;
+
+__device__ inline uint64_t lb_pack(uint32_t epoch, uint32_t status, uint32_t count)
+{
+    return ((uint64_t)((epoch << 2) | status) << 32) | count;
+}
+
+__device__ inline void lb_store(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t lb_load(const uint64_t *p)
+{
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t lanemask_lt()
+{
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_wave, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += y;
+    }
+    if (lane == 63)
+        s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; w++) {
+        uint32_t c = s_wave[w];
+        before += (w < wave) ? c : 0u;
+        all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+__global__ void k_radius_keys(const double *__restrict__ r, uint64_t *__restrict__ keys, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        keys[i] = radius_key(r[i]);
+}
+
+__global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ void k_sort_zero(uint32_t *__restrict__ hist, int nhist, uint32_t *__restrict__ counters,
+                            int ncounters)
+{
+    for (int i = threadIdx.x; i < nhist; i += blockDim.x)
+        hist[i] = 0;
+    for (int i = threadIdx.x; i < ncounters; i += blockDim.x)
+        counters[i] = 0;
+}
+
+// All eight digit histograms in one read of the keys.  Each thread takes
+// kHistItems consecutive keys and adds runs of equal digits with one LDS
+// atomic, which keeps the nearly-constant high digits of nearly-sorted radii
+// from serialising on one bank.
+__global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const uint64_t *__restrict__ keys,
+                                                                  int64_t n,
+                                                                  uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t s_hist[kKeyPasses * kRadix];
+    for (int i = threadIdx.x; i < kKeyPasses * kRadix; i += kHistThreads)
+        s_hist[i] = 0;
+    __syncthreads();
+
+    const int64_t per_block = (int64_t)kHistThreads * kHistItems;
+    for (int64_t base = (int64_t)blockIdx.x * per_block; base < n; base += (int64_t)gridDim.x * per_block) {
+        const int64_t first = base + (int64_t)threadIdx.x * kHistItems;
+        uint64_t k[kHistItems];
+        int cnt = 0;
+        if (first + kHistItems <= n) {
+            const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(keys + first);
+#pragma unroll
+            for (int j = 0; j < kHistItems / 2; j++) {
+                ulonglong2 t = src[j];
+                k[2 * j] = t.x;
+                k[2 * j + 1] = t.y;
+            }
+            cnt = kHistItems;
+        } else {
+#pragma unroll
+            for (int j = 0; j < kHistItems; j++) {
+                k[j] = (first + j < n) ? keys[first + j] : 0;
+            }
+            cnt = first < n ? (int)(n - first) : 0;
+        }
+        if (cnt > 0) {
+#pragma unroll
+            for (int p = 0; p < kKeyPasses; p++) {
+                uint32_t cur = (uint32_t)(k[0] >> (kRadixBits * p)) & (kRadix - 1);
+                uint32_t run = 1;
+#pragma unroll
+                for (int j = 1; j < kHistItems; j++) {
+                    if (j < cnt) {
+                        uint32_t d = (uint32_t)(k[j] >> (kRadixBits * p)) & (kRadix - 1);
+                        if (d == cur) {
+                            run++;
+                        } else {
+                            atomicAdd(&s_hist[p * kRadix + cur], run);
+                            cur = d;
+                            run = 1;
+                        }
+                    }
+                }
+                atomicAdd(&s_hist[p * kRadix + cur], run);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kKeyPasses * kRadix; i += kHistThreads) {
+        uint32_t c = s_hist[i];
+        if (c)
+            atomicAdd(&hist[i], c);
+    }
+}
+
+// One onesweep pass over digit `shift`.  FIRST: payload = input position.
+// LAST: keys are not written.
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+    const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int64_t n, int shift,
+    const uint32_t *__restrict__ hist_pass, uint64_t *__restrict__ lookback,
+    uint32_t *__restrict__ tile_counter, uint32_t epoch)
+{
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
+    __shared__ uint32_t s_off[kRadix];    // global base minus tile-local start, per digit
+    __shared__ uint32_t s_lstart[kRadix]; // tile-local digit start
+    __shared__ uint32_t s_dbase[kRadix];  // global exclusive digit base
+    __shared__ uint32_t s_wave[kSortWaves];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+
+    if (tid == 0)
+        s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads)
+        (&s_wcnt[0][0])[i] = 0;
+    {
+        uint32_t tot;
+        uint32_t b = block_exclusive_scan(hist_pass[tid], s_wave, &tot);
+        s_dbase[tid] = b;
+    }
+    __syncthreads();
+
+    const uint32_t tile = s_tile;
+    const int64_t tile_base = (int64_t)tile * kSortTile;
+    const int64_t wave_base = tile_base + (int64_t)wave * (kSortItems * 64);
+
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+        const bool valid = idx < n;
+        key[j] = valid ? keys_in[idx] : ~0ull;
+        if (FIRST)
+            val[j] = (uint32_t)idx;
+        else
+            val[j] = valid ? vals_in[idx] : 0u;
+    }
+
+    // Stable per-wave ranking: items of one wave are visited in index order
+    // (round j covers indices wave_base + 64 j + lane).
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+        uint32_t old = 0;
+        if (valid && lane == leader)
+            old = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
+        old = __shfl(old, leader, 64);
+        rank[j] = old + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+
+    // Per digit (thread tid owns digit tid): wave-exclusive offsets, tile
+    // total, decoupled lookback for the tile's global offset.
+    {
+        const int d = tid;
+        uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; w++) {
+            uint32_t c = s_wcnt[w][d];
+            s_wcnt[w][d] = total;
+            total += c;
+        }
+        uint64_t *mine = lookback + (size_t)tile * kRadix + d;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            lb_store(mine, lb_pack(epoch, kStatusPrefix, total));
+        } else {
+            lb_store(mine, lb_pack(epoch, kStatusAggregate, total));
+            int64_t t = (int64_t)tile - 1;
+            while (true) {
+                const uint64_t wv = lb_load(lookback + (size_t)t * kRadix + d);
+                const uint32_t hi = (uint32_t)(wv >> 32);
+                const uint32_t status = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
+                if (status == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)wv;
+                if (status == kStatusPrefix)
+                    break;
+                t--;
+            }
+            lb_store(mine, lb_pack(epoch, kStatusPrefix, excl + total));
+        }
+        uint32_t tot;
+        const uint32_t lstart = block_exclusive_scan(total, s_wave, &tot);
+        s_lstart[d] = lstart;
+        s_off[d] = s_dbase[d] + excl - lstart;
+    }
+    __syncthreads();
+
+    // Stage the tile in digit order.
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+            const uint32_t pos = s_lstart[d] + s_wcnt[wave][d] + rank[j];
+            s_keys[pos] = key[j];
+            s_vals[pos] = val[j];
+        }
+    }
+    __syncthreads();
+
+    const int64_t remain = n - tile_base;
+    const int count = remain < kSortTile ? (int)remain : kSortTile;
+#pragma unroll 4
+    for (int j = tid; j < count; j += kSortThreads) {
+        const uint64_t k = s_keys[j];
+        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        const uint32_t dest = s_off[d] + (uint32_t)j;
+        if (!LAST)
+            keys_out[dest] = k;
+        vals_out[dest] = s_vals[j];
+    }
+}
+
+template <bool FIRST, bool LAST>
+hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass,
+                       hipStream_t stream)
+{
+    const int64_t tiles = sort_tiles(n);
+    const uint32_t epoch = w.epoch++;
+    hipLaunchKernelGGL((k_onesweep<FIRST, LAST>), dim3((unsigned)tiles), dim3(kSortThreads), 0, stream,
+                       w.keys[src], FIRST ? nullptr : w.vals[src], LAST ? nullptr : w.keys[src ^ 1],
+                       vals_out, n, pass * kRadixBits, w.hist + pass * kRadix, w.lookback,
+                       w.counters + pass, epoch);
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    const int threads = 256;
+    hipLaunchKernelGGL(k_radius_keys, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0,
+                       stream, r, keys, n);
+    return hipGetLastError();
+}
+
+hipError_t radix_sort_perm(SortWork &w, int64_t n, const uint32_t *tie_ids, uint32_t *perm_out,
+                           hipStream_t stream, Profiler *prof)
+{
+    (void)tie_ids;
+    if (n <= 0)
+        return hipSuccess;
+    hipError_t e;
+    // The epoch lives in 30 bits of the lookback word; wrap by clearing
+    // (a fill kernel, not a memset node, so that the sort stays capturable).
+    if (w.epoch + kKeyPasses >= (1u << 30) || w.epoch == 0) {
+        const int64_t words = sort_tiles(n) * kRadix;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream,
+                           w.lookback, words, (uint64_t)0);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        w.epoch = 1;
+    }
+    if (prof)
+        prof->begin(K_SORT_ZERO);
+    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kKeyPasses * kRadix,
+                       w.counters, kKeyPasses);
+    if (prof)
+        prof->end(K_SORT_ZERO);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+
+    {
+        const int64_t per_block = (int64_t)kHistThreads * kHistItems;
+        int64_t blocks = (n + per_block - 1) / per_block;
+        if (blocks > 2048)
+            blocks = 2048;
+        if (prof)
+            prof->begin(K_SORT_HIST);
+        hipLaunchKernelGGL(k_sort_histogram, dim3((unsigned)blocks), dim3(kHistThreads), 0, stream,
+                           w.keys[0], n, w.hist);
+        if (prof)
+            prof->end(K_SORT_HIST);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+
+    int src = 0;
+    for (int p = 0; p < kKeyPasses; p++) {
+        const bool first = p == 0, last = p == kKeyPasses - 1;
+        uint32_t *vout = last ? perm_out : w.vals[src ^ 1];
+        if (prof)
+            prof->begin(K_SORT_PASS);
+        if (first)
+            e = launch_pass<true, false>(w, src, vout, n, p, stream);
+        else if (last)
+            e = launch_pass<false, true>(w, src, vout, n, p, stream);
+        else
+            e = launch_pass<false, false>(w, src, vout, n, p, stream);
+        if (prof)
+            prof->end(K_SORT_PASS);
+        if (e != hipSuccess)
+            return e;
+        src ^= 1;
+    }
+    return hipSuccess;
+}
+
+} // namespace ns
