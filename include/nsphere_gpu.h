@@ -123,7 +123,11 @@ int ns_gpu_sync(ns_gpu_ctx *ctx);
 int ns_gpu_levels(ns_gpu_ctx *ctx, uint8_t *host_out);
 /* Shell-model energy of the current state with the rank-based mass of
  * nsphere.c:513: K = sum m (v^2 + ell^2/r^2)/2,
- * W = -sum (vel_conv_sq G) (i/npts) M m / r_i, m = M/npts, float64 sums. */
+ * W = -sum (vel_conv_sq G) (i/npts) M m / r_(i), m = M/npts, float64 sums,
+ * where i is the particle's rank among the CURRENT radii (the call sorts
+ * first; the state itself is not reordered).  The reference has no total-
+ * energy routine; W is the potential whose gradient is its force, so E is
+ * conserved by the exact dynamics and continuous when shells cross. */
 int ns_gpu_energy(ns_gpu_ctx *ctx, const ns_step_params *p, double *K, double *W);
 
 /* Kernel timing (NS_OPT_PROFILE=1): for each kernel family, the number of

@@ -636,10 +636,15 @@ int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
         *W = 0.0;
         return NS_OK;
     }
+    // Ranks of the current radii (after a step the arrays hold the order of
+    // that step's sort, not of the new radii).
+    if (int rc = sort_perm(c))
+        return rc;
     const double m = p->halo_mass / (double)c->n;
     c->prof.begin(ns::K_ENERGY);
-    HIP_TRY(ns::launch_energy(c->r[c->cur], c->v[c->cur], c->l[c->cur], c->n, m, p->vel_conv_sq * p->G,
-                              p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out, c->stream));
+    HIP_TRY(ns::launch_energy(c->r[c->cur], c->v[c->cur], c->l[c->cur], c->perm, c->n, m,
+                              p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity, c->energy_partial,
+                              c->energy_out, c->stream));
     c->prof.end(ns::K_ENERGY);
     double out[2];
     HIP_TRY(hipMemcpyAsync(out, c->energy_out, sizeof out, hipMemcpyDeviceToHost, c->stream));

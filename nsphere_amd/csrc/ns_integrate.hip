@@ -378,16 +378,20 @@ __global__ void k_inverse_map(const uint32_t *__restrict__ id, int32_t *__restri
 
 constexpr int kEnergyThreads = 256;
 
-// Per-block float64 partial sums of K and W in a fixed tree order.
+// Per-block float64 partial sums of K and W in a fixed tree order.  Rank i
+// is the position in radius order: particle perm[i] (perm may be null when
+// the arrays are already sorted).
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
     const double *__restrict__ r, const double *__restrict__ v, const double *__restrict__ ell,
-    int64_t n, double m, double gpre, double halo_mass, int idg, double *__restrict__ partial)
+    const uint32_t *__restrict__ perm, int64_t n, double m, double gpre, double halo_mass, int idg,
+    double *__restrict__ partial)
 {
     __shared__ double sk[kEnergyThreads], sw[kEnergyThreads];
     double k = 0.0, w = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kEnergyThreads;
     for (int64_t i = (int64_t)blockIdx.x * kEnergyThreads + threadIdx.x; i < n; i += stride) {
-        const double rr = r[i], vv = v[i], ll = ell[i];
+        const int64_t p = perm ? (int64_t)perm[i] : i;
+        const double rr = r[p], vv = v[p], ll = ell[p];
         k += m * 0.5 * (vv * vv + (ll * ll) / (rr * rr));
         if (!idg) {
             const double menc = ((double)i / (double)n) * halo_mass;
@@ -468,13 +472,13 @@ int energy_blocks(int64_t n)
     return b < 1 ? 1 : (int)b;
 }
 
-hipError_t launch_energy(const double *r, const double *v, const double *ell, int64_t n, double m,
-                         double gpre, double halo_mass, int idg, double *partial, double *out,
-                         hipStream_t stream)
+hipError_t launch_energy(const double *r, const double *v, const double *ell, const uint32_t *perm,
+                         int64_t n, double m, double gpre, double halo_mass, int idg, double *partial,
+                         double *out, hipStream_t stream)
 {
     const int blocks = energy_blocks(n);
-    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, n, m,
-                       gpre, halo_mass, idg, partial);
+    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, n,
+                       m, gpre, halo_mass, idg, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

@@ -30,8 +30,8 @@ hipError_t launch_gather_state(const double *r0, const double *v0, const double 
                                int64_t n, hipStream_t stream);
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream);
 int energy_blocks(int64_t n);
-hipError_t launch_energy(const double *r, const double *v, const double *ell, int64_t n, double m,
-                         double gpre, double halo_mass, int idg, double *partial, double *out,
-                         hipStream_t stream);
+hipError_t launch_energy(const double *r, const double *v, const double *ell, const uint32_t *perm,
+                         int64_t n, double m, double gpre, double halo_mass, int idg, double *partial,
+                         double *out, hipStream_t stream);
 
 } // namespace ns

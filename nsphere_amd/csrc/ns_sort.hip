@@ -33,14 +33,18 @@ __device__ inline uint64_t lb_pack(uint32_t epoch, uint32_t status, uint32_t cou
     return ((uint64_t)((epoch << 2) | status) << 32) | count;
 }
 
+// The lookback word carries its whole message (status and count), so it needs
+// no ordering against other memory: relaxed agent-scope atomics keep it
+// coherent across the eight XCD L2s (sc1 accesses) without the L2 writeback
+// and invalidate that release/acquire would issue on every access.
 __device__ inline void lb_store(uint64_t *p, uint64_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ inline uint64_t lb_load(const uint64_t *p)
 {
-    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ inline uint64_t lanemask_lt()
