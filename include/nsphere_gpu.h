@@ -76,8 +76,17 @@ enum {
     NS_OPT_TIEBREAK = 1,   /* 0: stable on input order, exactly as quadsort
                               (default); 1: exact ties broken by particle id    */
     NS_OPT_PROFILE = 2,    /* 1: time every kernel with HIP events             */
-    NS_OPT_INTEGRATOR = 3  /* 0: strict IEEE order of the reference (default);
-                              1: fast (FMA-contracted, reciprocal reuse)        */
+    NS_OPT_INTEGRATOR = 3, /* 0 plain: one lane per particle, levels in turn,
+                                 strict IEEE order of the reference (default);
+                              1 binned: levels 0-3 per lane, deeper particles
+                                 on a whole wave with one level per lane;
+                                 bit-identical to 0;
+                              2 binned with the fast arithmetic form (one
+                                 reciprocal per force, fused multiply-adds)   */
+    NS_OPT_BUDGET = 4      /* force evaluations allowed per particle and step
+                              (default 2^26); a particle that needs more is
+                              stopped, flagged NS_LVL_NOCONV and counted, and
+                              ns_gpu_sync reports it                           */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;

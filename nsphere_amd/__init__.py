@@ -19,7 +19,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV, NS_OPT_INTEGRATOR, NS_OPT_PROFILE,
+from ._lib import (FORM_BINNED, FORM_BINNED_FAST, FORM_PLAIN, NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV,
+                   NS_OPT_BUDGET, NS_OPT_INTEGRATOR, NS_OPT_PROFILE,
                    NS_OPT_TIEBREAK, NsError, StepParams, check, lib)
 
 __all__ = [

@@ -31,6 +31,7 @@ struct ns_gpu_ctx {
     double *energy_partial = nullptr;
     double *energy_out = nullptr;  // device, 2 doubles
     unsigned long long *budget_hits = nullptr;
+    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool levels_valid = false;
@@ -202,7 +203,7 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits};
+                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_count};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -241,6 +242,10 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.identity_gravity = p->identity_gravity;
     a.budget = c->budget;
     a.budget_hits = c->budget_hits;
+    a.q_lf = c->q_lf;
+    a.q_lc = c->q_lc;
+    a.q_count = c->q_count;
+    a.cap_N = 8;  // levels 0..3 inline in the bulk kernel
     return a;
 }
 
@@ -276,7 +281,7 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
     a.keys = c->sw.keys[0];
     a.lvl = c->lvl;
     c->prof.begin(ns::K_INTEGRATE);
-    HIP_TRY(ns::launch_integrate(a, gather, c->stream));
+    HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->stream));
     c->prof.end(ns::K_INTEGRATE);
     c->cur = d;
     c->keys_valid = true;
@@ -366,6 +371,9 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->energy_partial, 2 * ns::energy_blocks(n));
     if (e == hipSuccess) e = dalloc(&c->energy_out, 2);
     if (e == hipSuccess) e = dalloc(&c->budget_hits, 1);
+    if (e == hipSuccess) e = dalloc(&c->q_lf, n);
+    if (e == hipSuccess) e = dalloc(&c->q_lc, n);
+    if (e == hipSuccess) e = dalloc(&c->q_count, 2);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));
@@ -416,9 +424,14 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         c->prof.on = value != 0;
         return NS_OK;
     case NS_OPT_INTEGRATOR:
-        if (value != 0)
-            return set_error(NS_ERR_ARG, "NS_OPT_INTEGRATOR: only 0 (strict) is available in this build");
+        if (value < ns::FORM_PLAIN || value > ns::FORM_BINNED_FAST)
+            return set_error(NS_ERR_ARG, "NS_OPT_INTEGRATOR takes 0 (plain), 1 (binned) or 2 (binned, fast)");
         c->integrator = (int)value;
+        return NS_OK;
+    case NS_OPT_BUDGET:
+        if (value < 2)
+            return set_error(NS_ERR_ARG, "NS_OPT_BUDGET must be >= 2 force evaluations");
+        c->budget = value;
         return NS_OK;
     default:
         return set_error(NS_ERR_ARG, "ns_gpu_set_option: unknown option %d", option);

@@ -22,9 +22,21 @@ struct StepArgs {
     int64_t max_subdiv;
     int64_t budget;              // force evaluations allowed per particle per step
     int32_t out_type, N_taumin, identity_gravity;
+    // binned integrator: queues of ranks handed from the bulk kernel to the
+    // deep kernel (leapfrog particles past cap_N, Levi-Civita particles)
+    uint32_t *q_lf, *q_lc;
+    uint32_t *q_count;           // [0] leapfrog queue length, [1] LC queue length
+    int64_t cap_N;               // deepest N the bulk kernel tries inline
 };
 
-hipError_t launch_integrate(const StepArgs &a, bool gather, hipStream_t stream);
+// Integrator forms (NS_OPT_INTEGRATOR).
+enum IntegratorForm : int {
+    FORM_PLAIN = 0,        // one lane per particle, sequential levels, strict IEEE
+    FORM_BINNED = 1,       // bulk lanes + deep queue with wave-parallel levels, strict IEEE
+    FORM_BINNED_FAST = 2   // as BINNED with the fast arithmetic form
+};
+
+hipError_t launch_integrate(const StepArgs &a, bool gather, int form, hipStream_t stream);
 hipError_t launch_gather_state(const double *r0, const double *v0, const double *l0, const uint32_t *id0,
                                const uint32_t *perm, double *r1, double *v1, double *l1, uint32_t *id1,
                                int64_t n, hipStream_t stream);

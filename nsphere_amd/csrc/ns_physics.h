@@ -1,0 +1,457 @@
+// ns_physics.h -- per-particle physics of the method-5 step, device side.
+//
+// Restates, for one particle at frozen rank, the reference's force helpers
+// (nsphere.c:502-599, 1757-1787), doMicroLeapfrog / doAdaptiveFullLeap
+// (nsphere.c:1550-1735) and the adaptive Levi-Civita family
+// (nsphere.c:1990-2334).  Two arithmetic forms:
+//   STRICT (FAST = false): the reference's operation order with IEEE division
+//     and no contraction (the library is compiled with -ffp-contract=off), so
+//     results equal the strict CPU restatement bit for bit;
+//   FAST: one reciprocal (v_rcp_f64 refined by two Newton steps) per force
+//     evaluation and fused multiply-adds; same algorithm and control flow,
+//     results within a few ulp per operation of the strict form.
+// Exact rewrites used by both forms (no result changes):
+//   * fine pass at level k (4N+1 micro-steps) == coarse pass at level k+1
+//     (2(2N)+1 micro-steps): same step sizes and operations;
+//   * a Levi-Civita micro-step's force at rho_next is the next micro-step's
+//     force at rho_curr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ns_device.h"
+#include "ns_kernels.h"
+
+namespace ns {
+
+// Returned by a capped leapfrog loop that must continue in the deep queue.
+constexpr int kEscalate = -1;
+
+struct Phys {
+    double gm;  // -(vel_conv_sq G) (i/npts) M : gravity numerator at frozen rank
+    double l2;  // ell * ell
+    bool idg;   // identity gravity (nsphere.c:476)
+};
+
+__device__ inline Phys make_phys(const StepArgs &a, int64_t i, double ell)
+{
+    Phys ph;
+    ph.idg = a.identity_gravity != 0;
+    ph.gm = -(a.vel_conv_sq * a.G) * ((double)i / (double)a.n) * a.halo_mass;
+    ph.l2 = ell * ell;
+    return ph;
+}
+
+// r_crit switch of nsphere.c:5803-5828.
+__device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r, double ell)
+{
+    double r_crit = 0.0;
+    if (fabs(ell) > 1.0e-30) {
+        double M_enc;
+        if (i == 0)
+            M_enc = 0.1 * (1.0 / (double)a.n) * a.halo_mass;
+        else
+            M_enc = ((double)i / (double)a.n) * a.halo_mass;
+        const double grav_part = (a.vel_conv_sq * a.G) * M_enc;
+        r_crit = (ell * ell) * a.alpha / grav_part;
+    }
+    return r > 1.0e-30 && r < r_crit;
+}
+
+__device__ inline double fast_rcp(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// gravitational_force + effective_angular_force (nsphere.c:502-535).
+template <bool FAST>
+__device__ inline double accel(const Phys &ph, double r)
+{
+    if constexpr (!FAST) {
+        const double r2 = r * r;
+        const double f = ph.idg ? 0.0 : ph.gm / r2;
+        return f + ph.l2 / (r2 * r);
+    } else {
+        const double inv = fast_rcp(r);
+        return fma(ph.l2, inv, ph.idg ? 0.0 : ph.gm) * (inv * inv);
+    }
+}
+
+// forceLCfun (nsphere.c:1777-1787 with 563-599): -GM/rho^2 + ell^2/rho^4.
+template <bool FAST>
+__device__ inline double force_lc(const Phys &ph, double rho)
+{
+    if constexpr (!FAST) {
+        const double rr = rho * rho;
+        const double g = ph.idg ? 0.0 : ph.gm / rr;
+        return g + ph.l2 / (rr * rho * rho);
+    } else {
+        const double inv = fast_rcp(rho);
+        const double inv2 = inv * inv;
+        return fma(ph.l2, inv2, ph.idg ? 0.0 : ph.gm) * inv2;
+    }
+}
+
+template <bool FAST>
+__device__ inline void drift_kick(const Phys &ph, double ms, double kk, double &r, double &v)
+{
+    if constexpr (!FAST) {
+        r += ms * v;
+        v += kk * accel<false>(ph, r);
+    } else {
+        r = fma(ms, v, r);
+        v = fma(kk, accel<true>(ph, r), v);
+    }
+}
+
+template <bool FAST>
+__device__ inline void kick(const Phys &ph, double kk, double r, double &v)
+{
+    if constexpr (!FAST)
+        v += kk * accel<false>(ph, r);
+    else
+        v = fma(kk, accel<true>(ph, r), v);
+}
+
+// Force-evaluation budget per particle and step: a guard against inputs on
+// which the reference would loop for hours (NaN state, r -> 0 in the LC
+// branch); the default is far above any converging particle's cost.
+struct Budget {
+    int64_t left;
+    bool hit;
+    __device__ inline bool take(int64_t n)
+    {
+        if (n > left) {
+            hit = true;
+            return false;
+        }
+        left -= n;
+        return true;
+    }
+};
+
+// doMicroLeapfrog, nsphere.c:1550-1607: N subdivisions, fine = (4N+1) form.
+template <bool FAST>
+__device__ inline void micro_leap(const Phys &ph, double r_in, double v_in, double h, int64_t N, bool fine,
+                                  double &r_out, double &v_out)
+{
+    const double Nd = (double)N;
+    const double hk = fine ? h / (4.0 * Nd) : h / (2.0 * Nd);
+    const double ms = fine ? h / (2.0 * Nd) : h / (1.0 * Nd);
+    const int64_t pairs = fine ? 2 * N : N;
+    double r = r_in, v = v_in;
+    kick<FAST>(ph, hk, r, v);
+    for (int64_t pp = 1; pp <= pairs - 1; pp++)
+        drift_kick<FAST>(ph, ms, ms, r, v);
+    drift_kick<FAST>(ph, ms, hk, r, v);
+    r_out = r;
+    v_out = v;
+}
+
+__device__ inline void leap_output(int out_type, double rc, double vc, double rf, double vf, double &r_out,
+                                   double &v_out)
+{
+    if (out_type == 0) {
+        r_out = rc;
+        v_out = vc;
+    } else if (out_type == 1) {
+        r_out = rf;
+        v_out = vf;
+    } else {
+        r_out = 4.0 * rf - 3.0 * rc;
+        v_out = 4.0 * vf - 3.0 * vc;
+    }
+}
+
+__device__ inline int level_of(int64_t N)
+{
+    const int k = 63 - __clzll((unsigned long long)N);  // N is a power of two
+    return k > NS_LVL_MASK ? NS_LVL_MASK : k;
+}
+
+// doAdaptiveFullLeap, nsphere.c:1648-1735, one lane.  With cap_N > 0 the
+// loop stops before a level whose N exceeds cap_N and returns kEscalate.
+template <bool FAST>
+__device__ int adaptive_leap(const Phys &ph, const StepArgs &a, double r_in, double v_in, double &r_out,
+                             double &v_out, Budget &bud, int64_t cap_N)
+{
+    const double h = a.dt;
+    double rc = r_in, vc = v_in, rf = r_in, vf = v_in;
+    int64_t N = 1;
+    int last = 0;
+    if (N <= a.max_subdiv) {
+        if (!bud.take(2)) {
+            r_out = r_in;
+            v_out = v_in;
+            return NS_LVL_NOCONV | NS_LVL_MASK;
+        }
+        micro_leap<FAST>(ph, r_in, v_in, h, N, false, rc, vc);
+    }
+    while (N <= a.max_subdiv) {
+        if (cap_N > 0 && N > cap_N)
+            return kEscalate;
+        if (!bud.take(2 * N + 1)) {
+            leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
+            return last | NS_LVL_NOCONV;
+        }
+        micro_leap<FAST>(ph, r_in, v_in, h, N, true, rf, vf);
+        last = level_of(N);
+        const double rd = fabs(rf - rc) / (fabs(rf) + 1.0e-30);
+        const double vd = fabs(vf - vc) / (fabs(vf) + 1.0e-30);
+        if ((rd < a.radius_tol) && (vd < a.velocity_tol)) {
+            leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
+            return last;
+        }
+        N *= 2;
+        if (N <= a.max_subdiv) {  // fine(N) is coarse(2N)
+            rc = rf;
+            vc = vf;
+        }
+    }
+    leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
+    return last | NS_LVL_NOCONV;
+}
+
+// doAdaptiveFullLeap, wave-cooperative form (all 64 lanes call it with the
+// same particle).  Lane k runs the fine pass at N = 2^k (k = 0..kmax, kmax =
+// floor(log2 max_subdiv)); lane 63 runs the coarse pass at N = 1.  All lanes
+// step in lockstep, so after 2^(k+1) iterations lanes k-1 and k hold coarse
+// and fine of level k and the reference's test for level k is evaluated; the
+// first level that passes is returned, exactly as the sequential loop would
+// return it, after 2^(K+1) iterations instead of about 2^(K+2).
+template <bool FAST>
+__device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in, double v_in, double &r_out,
+                                  double &v_out, Budget &bud)
+{
+    const int lane = threadIdx.x & 63;
+    const double h = a.dt;
+    if (a.max_subdiv < 1) {
+        leap_output(a.out_type, r_in, v_in, r_in, v_in, r_out, v_out);
+        return NS_LVL_NOCONV;
+    }
+    int kmax = 63 - __clzll((unsigned long long)a.max_subdiv);
+    if (kmax > 62)
+        kmax = 62;
+    const bool coarse_lane = lane == 63;
+    const bool active = coarse_lane || lane <= kmax;
+    const int64_t N = coarse_lane ? 1 : ((int64_t)1 << (lane <= kmax ? lane : 0));
+    const double Nd = (double)N;
+    const double hk = coarse_lane ? h / (2.0 * Nd) : h / (4.0 * Nd);
+    const double ms = coarse_lane ? h / (1.0 * Nd) : h / (2.0 * Nd);
+    const int64_t pairs = coarse_lane ? N : 2 * N;
+    double r = r_in, v = v_in;
+    if (!bud.take(2)) {
+        r_out = r_in;
+        v_out = v_in;
+        return NS_LVL_NOCONV | NS_LVL_MASK;
+    }
+    if (active)
+        kick<FAST>(ph, hk, r, v);
+    int64_t t = 0;
+    double prev_r = 0.0, prev_v = 0.0;  // fine of the previous level (shifted coarse)
+    for (int k = 0; k <= kmax; k++) {
+        const int64_t Nk = (int64_t)1 << k;
+        if (!bud.take(2 * Nk + 1)) {
+            if (k == 0) {
+                const double rc = __shfl(r, 63, 64), vc = __shfl(v, 63, 64);
+                leap_output(a.out_type, rc, vc, r_in, v_in, r_out, v_out);
+                return NS_LVL_NOCONV;
+            }
+            leap_output(a.out_type, prev_r, prev_v, prev_r, prev_v, r_out, v_out);
+            return (k - 1) | NS_LVL_NOCONV;
+        }
+        const int64_t target = (int64_t)2 << k;
+        for (; t < target; t++) {
+            if (active && t < pairs)
+                drift_kick<FAST>(ph, ms, (t < pairs - 1) ? ms : hk, r, v);
+        }
+        const int src_c = (k == 0) ? 63 : k - 1;
+        const double rc = __shfl(r, src_c, 64), vc = __shfl(v, src_c, 64);
+        const double rf = __shfl(r, k, 64), vf = __shfl(v, k, 64);
+        const double rd = fabs(rf - rc) / (fabs(rf) + 1.0e-30);
+        const double vd = fabs(vf - vc) / (fabs(vf) + 1.0e-30);
+        if ((rd < a.radius_tol) && (vd < a.velocity_tol)) {
+            leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
+            return level_of(Nk);
+        }
+        if (k == kmax) {
+            leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
+            return level_of(Nk) | NS_LVL_NOCONV;
+        }
+        prev_r = rf;
+        prev_v = vf;
+    }
+    // unreachable: the loop returns at k == kmax
+    r_out = r_in;
+    v_out = v_in;
+    return NS_LVL_NOCONV;
+}
+
+// doMicroLeviCivita, nsphere.c:1990-2036 (force at rho_next reused).
+template <bool FAST>
+__device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, double t_in, int64_t sub,
+                                double h_tau, double &rho_out, double &v_out, double &t_out)
+{
+    const double dtau = h_tau / (double)sub;
+    double rho = rho_in, v = v_in, t = t_in;
+    double f = force_lc<FAST>(ph, rho);
+    for (int64_t ss = 0; ss < sub; ss++) {
+        if constexpr (!FAST) {
+            const double v_half = v + 0.5 * dtau * f;
+            const double rho_next = rho + dtau * (0.5 * rho * v_half);
+            const double f2 = force_lc<false>(ph, rho_next);
+            const double v_next = v_half + 0.5 * dtau * f2;
+            const double rho_mid = 0.5 * (rho + rho_next);
+            const double dt_phys = dtau * (rho_mid * rho_mid);
+            t += dt_phys;
+            rho = rho_next;
+            v = v_next;
+            f = f2;
+        } else {
+            const double hd = 0.5 * dtau;
+            const double v_half = fma(hd, f, v);
+            const double rho_next = fma(dtau, 0.5 * rho * v_half, rho);
+            const double f2 = force_lc<true>(ph, rho_next);
+            const double rho_mid = 0.5 * (rho + rho_next);
+            t = fma(dtau, rho_mid * rho_mid, t);
+            v = fma(hd, f2, v_half);
+            rho = rho_next;
+            f = f2;
+        }
+    }
+    rho_out = rho;
+    v_out = v;
+    t_out = t;
+}
+
+// doSingleTauStepAdaptiveLeviCivita, nsphere.c:2077-2186.
+template <bool FAST>
+__device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
+                             double h, double &rho_out, double &v_out, double &t_out, Budget &bud)
+{
+    int64_t N = 1;
+    int last = 0;
+    double rhoC = 0, vC = 0, tC = 0, rhoF = 0, vF = 0, tF = 0;
+    if (N <= a.max_subdiv) {
+        if (!bud.take(2 * N + 2)) {
+            rho_out = rho_in;
+            v_out = v_in;
+            t_out = t_in;
+            return NS_LVL_NOCONV | NS_LVL_MASK;
+        }
+        micro_lc<FAST>(ph, rho_in, v_in, t_in, 2 * N + 1, h, rhoC, vC, tC);
+    }
+    while (N <= a.max_subdiv) {
+        if (!bud.take(4 * N + 2)) {
+            rho_out = rhoC;
+            v_out = vC;
+            t_out = tC;
+            return last | NS_LVL_NOCONV;
+        }
+        micro_lc<FAST>(ph, rho_in, v_in, t_in, 4 * N + 1, h, rhoF, vF, tF);
+        last = level_of(N);
+        const double rF = rhoF * rhoF;
+        const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
+        const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
+        if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
+            if (a.out_type == 0) {
+                rho_out = rhoC;
+                v_out = vC;
+                t_out = tC;
+            } else if (a.out_type == 1) {
+                rho_out = rhoF;
+                v_out = vF;
+                t_out = tF;
+            } else {
+                double rho_rich = 4.0 * rhoF - 3.0 * rhoC;
+                const double v_rich = 4.0 * vF - 3.0 * vC;
+                const double t_rich = 4.0 * tF - 3.0 * tC;
+                if (rho_rich < 0.0)
+                    rho_rich = 0.0;
+                rho_out = rho_rich;
+                v_out = v_rich;
+                t_out = t_rich;
+            }
+            return last;
+        }
+        N *= 2;
+        if (N <= a.max_subdiv) {
+            rhoC = rhoF;
+            vC = vF;
+            tC = tF;
+        }
+    }
+    // not converged: the fine pass at the overflowed N (nsphere.c:2168-2185)
+    if (!bud.take(4 * N + 2)) {
+        rho_out = rhoF;
+        v_out = vF;
+        t_out = tF;
+        return last | NS_LVL_NOCONV;
+    }
+    micro_lc<FAST>(ph, rho_in, v_in, t_in, 4 * N + 1, h, rho_out, v_out, t_out);
+    return level_of(N) | NS_LVL_NOCONV;
+}
+
+// doAdaptiveFullLeviCivita, nsphere.c:2231-2334.
+template <bool FAST>
+__device__ int adaptive_lc(const Phys &ph, const StepArgs &a, double r_in, double v_in, double &r_out,
+                           double &v_out, Budget &bud)
+{
+    int rec = NS_LVL_LC;
+    if (r_in < 1.0e-30) {
+        r_out = r_in;
+        v_out = v_in;
+        return rec;
+    }
+    const double dt = a.dt;
+    double rho_current = sqrt(r_in);
+    double v_current = v_in;
+    double t_cur = 0.0;
+    double delta_tau;
+    if (r_in > 1.0e-30 && a.N_taumin > 0)
+        delta_tau = dt / (2.0 * r_in * (double)a.N_taumin);
+    else
+        delta_tau = dt / 100.0;
+    int64_t step_count = 0;
+    const int64_t step_max = 100000000;
+    while (true) {
+        if (t_cur >= dt)
+            break;
+        double rho_next, v_next, t_next;
+        const int lv = single_tau_lc<FAST>(ph, a, rho_current, v_current, t_cur, delta_tau, rho_next, v_next,
+                                           t_next, bud);
+        if ((lv & NS_LVL_MASK) > (rec & NS_LVL_MASK))
+            rec = (rec & ~NS_LVL_MASK) | (lv & NS_LVL_MASK);
+        rec |= (lv & NS_LVL_NOCONV);
+        if (t_next > dt) {
+            double alpha = 0.0;
+            if (fabs(t_next - t_cur) > 1.0e-30)
+                alpha = (dt - t_cur) / (t_next - t_cur);
+            const double rho_final = rho_current + alpha * (rho_next - rho_current);
+            const double v_final = v_current + alpha * (v_next - v_current);
+            r_out = rho_final * rho_final;
+            v_out = v_final;
+            return rec;
+        }
+        rho_current = rho_next;
+        v_current = v_next;
+        t_cur = t_next;
+        step_count++;
+        if (step_count > step_max || bud.hit) {
+            r_out = rho_current * rho_current;
+            v_out = v_current;
+            return rec | NS_LVL_NOCONV;
+        }
+    }
+    r_out = rho_current * rho_current;
+    v_out = v_current;
+    return rec;
+}
+
+} // namespace ns

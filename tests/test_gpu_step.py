@@ -147,3 +147,73 @@ def test_million_particles_ten_steps(ns):
     assert abs((K1 + W1) - e_end) <= 1e-12 * abs(e_end)
     drift_per_step = ((K1 + W1) - (K0 + W0)) / abs(K0 + W0) / 10
     assert 0 < drift_per_step < 1e-6
+
+
+def run_form(ns, form, r, v, ell, ids, p, steps, budget=None):
+    with ns.GpuContext(r.size) as ctx:
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
+        if budget is not None:
+            ctx.set_option(ns.NS_OPT_BUDGET, budget)
+        ctx.upload(r, v, ell, ids)
+        for _ in range(steps):
+            ctx.step(p)
+        try:
+            ctx.sync()
+        except ns.NsError:
+            if budget is None:
+                raise
+        return (*ctx.download(), ctx.levels())
+
+
+@pytest.mark.parametrize("kind,n,seed", [(0, 100_000, 51), (2, 60_000, 52)])
+def test_binned_form_bit_identical(ns, kind, n, seed):
+    """Levels 0-3 per lane + wave-parallel deep levels == plain form, bit for bit."""
+    r, v, ell, ids = sorted_ic(ns, kind, n, seed, lowl=0.1)
+    p = ns.unit_params()
+    a = run_form(ns, ns.FORM_PLAIN, r, v, ell, ids, p, 4)
+    b = run_form(ns, ns.FORM_BINNED, r, v, ell, ids, p, 4)
+    for x, y, what in zip(a, b, ("r", "v", "ell", "id", "levels")):
+        assert_bits(y, x, what)
+    assert (a[4] & ns.NS_LVL_MASK).max() >= 4, "workload should reach the deep queue"
+
+
+@pytest.mark.parametrize("variant", [
+    dict(max_subdiv=1), dict(max_subdiv=16), dict(max_subdiv=0), dict(out_type=0), dict(out_type=1),
+    dict(radius_tol=1e-13, velocity_tol=1e-13, max_subdiv=1 << 12), dict(identity_gravity=1),
+])
+def test_binned_variants_bit_identical(ns, variant):
+    r, v, ell, ids = sorted_ic(ns, 2, 30_000, 61, lowl=0.2)
+    p = ns.unit_params(**variant)
+    a = run_form(ns, ns.FORM_PLAIN, r, v, ell, ids, p, 1)
+    b = run_form(ns, ns.FORM_BINNED, r, v, ell, ids, p, 1)
+    lv_o = oracle.integrate(r.copy(), v.copy(), ell, oracle.from_step_params(p))
+    for x, y, what in zip(a, b, ("r", "v", "ell", "id", "levels")):
+        assert_bits(y, x, what)
+
+
+@pytest.mark.parametrize("budget", [2, 40, 1000])
+def test_budget_cap_same_in_both_forms(ns, budget):
+    """The per-particle work cap stops both forms at the same level."""
+    r, v, ell, ids = sorted_ic(ns, 0, 20_000, 71, lowl=0.1)
+    p = ns.unit_params(radius_tol=1e-13, velocity_tol=1e-13)
+    a = run_form(ns, ns.FORM_PLAIN, r, v, ell, ids, p, 1, budget=budget)
+    b = run_form(ns, ns.FORM_BINNED, r, v, ell, ids, p, 1, budget=budget)
+    for x, y, what in zip(a, b, ("r", "v", "ell", "id", "levels")):
+        assert_bits(y, x, what)
+    assert ((a[4] & ns.NS_LVL_NOCONV) != 0).any()
+
+
+def test_fast_form_close_to_strict(ns):
+    """Fast arithmetic (one reciprocal per force, FMAs): same algorithm, so
+    radii and velocities agree to the integrator tolerance and levels agree
+    except where a convergence test sits on the tolerance boundary."""
+    r, v, ell, ids = sorted_ic(ns, 0, 200_000, 81, lowl=0.05)
+    p = ns.unit_params()
+    a = run_form(ns, ns.FORM_BINNED, r, v, ell, ids, p, 3)
+    b = run_form(ns, ns.FORM_BINNED_FAST, r, v, ell, ids, p, 3)
+    assert np.array_equal(a[3], b[3]), "same sort order after 3 steps"
+    dr = np.abs(b[0] - a[0]) / np.abs(a[0])
+    dv = np.abs(b[1] - a[1]) / np.maximum(np.abs(a[1]), 1e-3)
+    assert np.median(dr) < 1e-14 and dr.max() < 1e-6
+    assert np.median(dv) < 1e-13 and dv.max() < 1e-5
+    assert np.mean(a[4] != b[4]) < 1e-3
