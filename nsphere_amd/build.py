@@ -23,7 +23,9 @@ LIB = os.path.join(PKG, "libnsphere_gpu.so")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["ns_api.hip", "ns_sort.hip", "ns_integrate.hip"]
-HOST_SOURCES = ["ns_ic.cpp"]
+HOST_SOURCES = ["ns_ic.cpp", "ns_dropin.cpp"]
+DRIVER_SRC = os.path.join(ROOT, "tests", "c", "dropin_driver.c")
+DRIVER_BIN = os.path.join(ROOT, "tests", "c", "dropin_driver")
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -70,6 +72,10 @@ def build(verbose: bool = False) -> str:
         tmp_lib = LIB + ".tmp"
         _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp_lib, "-lgomp"])
         os.replace(tmp_lib, LIB)
+    if os.path.exists(DRIVER_SRC):
+        # a plain C program using the reference's signatures (nsphere_dropin.h)
+        _run(["gcc", "-O2", "-std=c11", "-Wall", "-I", INCLUDE, DRIVER_SRC, "-o", DRIVER_BIN,
+              "-L", PKG, "-lnsphere_gpu", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../nsphere_amd"])
     if verbose:
         print(f"built {LIB}")
     return LIB
