@@ -56,14 +56,14 @@ def cpu_info() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(state, params, budget_s: float) -> dict:
+def cpu_baseline(state, params, budget_s: float, work_cap: int) -> dict:
     """The oracle with the reference's -Ofast flags, OpenMP on the host cores
     and the reference's default chunked sort, timed on whole steps of the same
-    workload from the same initial state."""
+    workload from the same initial state, under the same work cap."""
     import oracle
 
     threads = oracle.lib(True).orc_num_threads()
-    po = oracle.from_step_params(params)
+    po = oracle.from_step_params(params, work_cap)
     r, v, ell, ids = (a.copy() for a in state)
     n = r.size
     steps = 0
@@ -81,33 +81,43 @@ def cpu_baseline(state, params, budget_s: float) -> dict:
         "kind": "port",
         "sample": f"{steps} whole steps of the N={n} workload from the benchmark's initial state "
                   f"({el:.1f} s); C restatement with the reference's -Ofast flags, chunked sort "
-                  f"(--sort 1); {threads} OpenMP threads on {cpu_info()}",
+                  f"(--sort 1), work cap {work_cap}; {threads} OpenMP threads on {cpu_info()}",
     }
 
 
-def measure(ns, form: str, state, p, steps: int, warmup: int) -> dict:
+def measure(ns, form: str, state, p, steps: int, warmup: int, work_cap: int) -> dict:
     n = state[0].size
     ctx = ns.GpuContext(n)
+    chunk = 10
     try:
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
+        ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
         ctx.upload(*state)
         K0, W0 = ctx.energy(p)
         ctx.run(p, warmup)
         ctx.sync()
+        ctx.cap_hits(reset=True)
         ctx.set_option(ns.NS_OPT_PROFILE, 1)
         ctx.reset_kernel_stats()
         ctx.sync()
         t0 = time.perf_counter()
-        ctx.run(p, steps)
-        ctx.sync()
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            ctx.run(p, k)
+            ctx.sync()  # progress line every `chunk` steps (one host sync per chunk)
+            done += k
+            if done % 50 == 0 or done == steps:
+                print(f"[bench] {form}: {done}/{steps} steps, {time.perf_counter() - t0:.2f}s",
+                      file=sys.stderr, flush=True)
         el = time.perf_counter() - t0
         stats = ctx.kernel_stats()
         ctx.set_option(ns.NS_OPT_PROFILE, 0)
+        hits = ctx.cap_hits()
         K1, W1 = ctx.energy(p)
         lv = ctx.levels()
     finally:
         ctx.close()
-    print(f"[bench] {form}: {steps} steps in {el:.3f}s", file=sys.stderr, flush=True)
     kernels = {}
     for name, (launches, ms) in stats.items():
         avg_ms = ms / max(launches, 1)
@@ -124,6 +134,7 @@ def measure(ns, form: str, state, p, steps: int, warmup: int) -> dict:
         "kernels": kernels,
         "E0": K0 + W0,
         "E1": K1 + W1,
+        "cap_hits": hits,
         "levels": {"hist": np.bincount(lev, minlength=8)[:24].tolist(),
                    "levi_civita": int(((lv & ns.NS_LVL_LC) != 0).sum())},
     }
@@ -154,6 +165,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--budget", type=int, default=1 << 18,
+                    help="force evaluations allowed per particle-step, the same for every form and "
+                         "the CPU baseline (the reference has no cap; see DESIGN.md)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -174,10 +188,10 @@ def main() -> int:
         state = ctx.download()
     print(f"[bench] N={n} initial state in {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
 
-    chosen = measure(ns, args.form, state, p, args.steps, args.warmup)
+    chosen = measure(ns, args.form, state, p, args.steps, args.warmup, args.budget)
     plain = None
     if not args.no_plain and args.form != "plain":
-        plain = measure(ns, "plain", state, p, args.plain_steps or args.steps, args.warmup)
+        plain = measure(ns, "plain", state, p, args.plain_steps or args.steps, args.warmup, args.budget)
 
     dom = max(chosen["stats"].items(), key=lambda kv: kv[1][1])[0]
     out = {
@@ -201,17 +215,19 @@ def main() -> int:
         "kernels": chosen["kernels"],
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),
         "levels": chosen["levels"],
+        "work_cap": {"force_evals_per_particle_step": args.budget, "capped_particle_steps": chosen["cap_hits"]},
     }
     if plain is not None:
         out["plain_fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                              "kernels": plain["kernels"],
-                             "dE_over_E": (plain["E1"] - plain["E0"]) / abs(plain["E0"])}
+                             "dE_over_E": (plain["E1"] - plain["E0"]) / abs(plain["E0"]),
+                             "capped_particle_steps": plain["cap_hits"]}
         out["speedup_vs_plain"] = chosen["value"] / plain["value"]
         if (args.plain_steps or args.steps) == args.steps:
             out["energy_vs_plain"] = abs(chosen["E1"] - plain["E1"]) / abs(plain["E1"])
     if not args.no_cpu:
         print("[bench] timing CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds, args.budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

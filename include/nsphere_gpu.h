@@ -85,8 +85,10 @@ enum {
                                  reciprocal per force, fused multiply-adds)   */
     NS_OPT_BUDGET = 4      /* force evaluations allowed per particle and step
                               (default 2^26); a particle that needs more is
-                              stopped, flagged NS_LVL_NOCONV and counted, and
-                              ns_gpu_sync reports it                           */
+                              stopped, flagged NS_LVL_NOCONV and counted
+                              (ns_gpu_cap_hits).  Not in the reference, whose
+                              Levi-Civita tau steps converge only to first
+                              order and can run 2^28 subdivisions each       */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;
@@ -127,6 +129,9 @@ int ns_gpu_step(ns_gpu_ctx *ctx, const ns_step_params *p);
 /* nsteps steps back to back, queued without host synchronisation. */
 int ns_gpu_run(ns_gpu_ctx *ctx, const ns_step_params *p, int64_t nsteps);
 int ns_gpu_sync(ns_gpu_ctx *ctx);
+/* Particle-steps stopped by the work cap (NS_OPT_BUDGET) since the context
+ * was created or last reset; synchronises the context's stream. */
+int ns_gpu_cap_hits(ns_gpu_ctx *ctx, uint64_t *count, int reset);
 
 /* Level records of the last integrate (npts bytes, rank order). */
 int ns_gpu_levels(ns_gpu_ctx *ctx, uint8_t *host_out);

@@ -162,6 +162,12 @@ class GpuContext:
     def sync(self) -> None:
         check(lib.ns_gpu_sync(self._h))
 
+    def cap_hits(self, reset: bool = False) -> int:
+        """Particle-steps stopped by the work cap (NS_OPT_BUDGET)."""
+        n = ctypes.c_uint64()
+        check(lib.ns_gpu_cap_hits(self._h, ctypes.byref(n), int(reset)))
+        return n.value
+
     def levels(self) -> np.ndarray:
         out = np.empty(self.npts, np.uint8)
         check(lib.ns_gpu_levels(self._h, _ptr(out)))

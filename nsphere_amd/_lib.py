@@ -84,6 +84,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_step": (i32, [vp, c.POINTER(StepParams)]),
         "ns_gpu_run": (i32, [vp, c.POINTER(StepParams), i64]),
         "ns_gpu_sync": (i32, [vp]),
+        "ns_gpu_cap_hits": (i32, [vp, c.POINTER(c.c_uint64), i32]),
         "ns_gpu_levels": (i32, [vp, vp]),
         "ns_gpu_energy": (i32, [vp, c.POINTER(StepParams), pd, pd]),
         "ns_gpu_kernel_stats": (i32, [vp, c.POINTER(c.c_int), c.POINTER(c.c_char_p),

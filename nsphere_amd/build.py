@@ -1,6 +1,6 @@
 """Builds the native library of nsphere_amd in-tree for gfx950.
 
-    python -m nsphere_amd.build          # -> nsphere_amd/libnsphere_gpu.so
+    python nsphere_amd/build.py          # -> nsphere_amd/libnsphere_gpu.so
 
 hipcc compiles the HIP translation units (cross-compiles without a GPU);
 the host-only IC sampler is compiled with g++ and OpenMP.  All HIP code is

@@ -609,13 +609,24 @@ int ns_gpu_sync(ns_gpu_ctx *c)
         return rc;
     DeviceGuard g(c->device);
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int ns_gpu_cap_hits(ns_gpu_ctx *c, uint64_t *count, int reset)
+{
+    if (int rc = check_ctx(c, "ns_gpu_cap_hits"))
+        return rc;
+    if (!count)
+        return set_error(NS_ERR_ARG, "ns_gpu_cap_hits: null count");
+    DeviceGuard g(c->device);
     unsigned long long hits = 0;
-    HIP_TRY(hipMemcpy(&hits, c->budget_hits, sizeof hits, hipMemcpyDeviceToHost));
-    if (hits)
-        return set_error(NS_ERR_STATE,
-                         "ns_gpu_sync: %llu particle-steps exceeded the per-particle work cap of %lld force "
-                         "evaluations (their records carry NS_LVL_NOCONV)",
-                         hits, (long long)c->budget);
+    HIP_TRY(hipMemcpyAsync(&hits, c->budget_hits, sizeof hits, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *count = hits;
+    if (reset) {
+        HIP_TRY(hipMemsetAsync(c->budget_hits, 0, sizeof hits, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     return NS_OK;
 }
 

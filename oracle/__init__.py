@@ -31,15 +31,17 @@ class OrcParams(ctypes.Structure):
         ("out_type", ctypes.c_int32),
         ("N_taumin", ctypes.c_int32),
         ("identity_gravity", ctypes.c_int32),
+        ("budget", ctypes.c_int64),
     ]
 
 
 def params(dt=1e-3, G=1.0, halo_mass=1.0, vel_conv_sq=1.0, radius_tol=1e-7, velocity_tol=1e-7,
            alpha=0.05, max_subdiv=4096 * 4096 * 16, out_type=2, N_taumin=10,
-           identity_gravity=0) -> OrcParams:
-    """Method-5 constants (nsphere.c:5777-5782); G = M = 1 units by default."""
+           identity_gravity=0, budget=0) -> OrcParams:
+    """Method-5 constants (nsphere.c:5777-5782); G = M = 1 units by default.
+    budget: force evaluations per particle-step (0 = unlimited, as the reference)."""
     return OrcParams(dt, G, halo_mass, vel_conv_sq, radius_tol, velocity_tol, alpha, max_subdiv,
-                     out_type, N_taumin, identity_gravity)
+                     out_type, N_taumin, identity_gravity, budget)
 
 
 def reference_units(**kw) -> OrcParams:
@@ -49,9 +51,11 @@ def reference_units(**kw) -> OrcParams:
     return params(**d)
 
 
-def from_step_params(sp) -> OrcParams:
-    """Copy a nsphere_amd.StepParams (same field order) into OrcParams."""
-    return OrcParams(*[getattr(sp, f) for f, _ in OrcParams._fields_])
+def from_step_params(sp, budget: int = 0) -> OrcParams:
+    """Copy a nsphere_amd.StepParams into OrcParams (budget: the GPU context's
+    NS_OPT_BUDGET when it is to be mirrored, else 0 = unlimited)."""
+    vals = [getattr(sp, f) for f, _ in OrcParams._fields_ if f != "budget"]
+    return OrcParams(*vals, budget)
 
 
 def build() -> None:

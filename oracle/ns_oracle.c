@@ -101,10 +101,44 @@ static inline int level_of(int64_t N)
     return k > ORC_LVL_MASK ? ORC_LVL_MASK : k;
 }
 
+/* Work cap (not in the reference; budget 0 = unlimited). */
+typedef struct {
+    int64_t left;
+    int limited, hit;
+} work_cap;
+
+static inline work_cap cap_of(const orc_params *p)
+{
+    work_cap w = {p->budget, p->budget > 0, 0};
+    return w;
+}
+
+static inline int cap_take(work_cap *w, int64_t n)
+{
+    if (!w->limited)
+        return 1;
+    if (n > w->left) {
+        w->hit = 1;
+        return 0;
+    }
+    w->left -= n;
+    return 1;
+}
+
+static int adaptive_leap_capped(int i, int npts, double r_in, double v_in, double ell, double h,
+                                const orc_params *p, double *r_out, double *v_out, work_cap *w);
+
 /* nsphere.c:1648-1735: N = 1, 2, 4, ... <= max_subdiv; converge when both
  * relative differences are below tolerance; output per out_type. */
 int orc_adaptive_full_leap(int i, int npts, double r_in, double v_in, double ell,
                            double h, const orc_params *p, double *r_out, double *v_out)
+{
+    work_cap w = cap_of(p);
+    return adaptive_leap_capped(i, npts, r_in, v_in, ell, h, p, r_out, v_out, &w);
+}
+
+static int adaptive_leap_capped(int i, int npts, double r_in, double v_in, double ell, double h,
+                                const orc_params *p, double *r_out, double *v_out, work_cap *w)
 {
     int N = 1;
     double r_coarse = r_in, v_coarse = v_in;
@@ -112,7 +146,26 @@ int orc_adaptive_full_leap(int i, int npts, double r_in, double v_in, double ell
     int last_level = 0;
 
     while (N <= p->max_subdiv) {
+        if (N == 1 && !cap_take(w, 2)) {
+            *r_out = r_in;
+            *v_out = v_in;
+            return ORC_LVL_NOCONV | ORC_LVL_MASK;
+        }
         orc_micro_leapfrog(i, npts, r_in, v_in, ell, h, N, (2 * N + 1), p, &r_coarse, &v_coarse);
+        if (!cap_take(w, 2 * (int64_t)N + 1)) {
+            /* coarse(N) equals the previous level's fine pass bit for bit */
+            if (p->out_type == 0) {
+                *r_out = r_coarse;
+                *v_out = v_coarse;
+            } else if (p->out_type == 1) {
+                *r_out = r_fine;
+                *v_out = v_fine;
+            } else {
+                *r_out = 4.0 * r_fine - 3.0 * r_coarse;
+                *v_out = 4.0 * v_fine - 3.0 * v_coarse;
+            }
+            return last_level | ORC_LVL_NOCONV;
+        }
         orc_micro_leapfrog(i, npts, r_in, v_in, ell, h, N, (4 * N + 1), p, &r_fine, &v_fine);
         last_level = level_of(N);
 
@@ -180,15 +233,42 @@ void orc_micro_lc(int i, int npts, double rho_in, double v_in, double t_in,
 /* nsphere.c:2077-2186: coarse 2N+1 vs fine 4N+1 substeps over the same h;
  * metric |rhoF-rhoC| / (rhoF^2 + 1e-30); 4F-3C on rho, v and t with rho
  * floored at 0; on non-convergence the fine pass at the overflowed N. */
+static int single_tau_capped(int i, int npts, double rho_in, double v_in, double t_in,
+                             double h_guess, const orc_params *p, double ell, double *rho_out,
+                             double *v_out, double *t_out, work_cap *w);
+
 int orc_single_tau_step_lc(int i, int npts, double rho_in, double v_in, double t_in,
                            double h_guess, const orc_params *p, double ell,
                            double *rho_out, double *v_out, double *t_out)
 {
+    work_cap w = cap_of(p);
+    return single_tau_capped(i, npts, rho_in, v_in, t_in, h_guess, p, ell, rho_out, v_out, t_out,
+                             &w);
+}
+
+static int single_tau_capped(int i, int npts, double rho_in, double v_in, double t_in,
+                             double h_guess, const orc_params *p, double ell, double *rho_out,
+                             double *v_out, double *t_out, work_cap *w)
+{
     int64_t N = 1;
     int last_level = 0;
+    double rhoF = 0.0, vF = 0.0, tF = 0.0; /* last fine pass, kept for the capped fallback */
     while (N <= p->max_subdiv) {
-        double rhoC, vC, tC, rhoF, vF, tF;
+        double rhoC, vC, tC;
+        if (N == 1 && !cap_take(w, 2 * N + 2)) {
+            *rho_out = rho_in;
+            *v_out = v_in;
+            *t_out = t_in;
+            return ORC_LVL_NOCONV | ORC_LVL_MASK;
+        }
         orc_micro_lc(i, npts, rho_in, v_in, t_in, 2 * N + 1, h_guess, p, ell, &rhoC, &vC, &tC);
+        if (!cap_take(w, 4 * N + 2)) {
+            /* coarse(N) equals the previous level's fine pass bit for bit */
+            *rho_out = rhoC;
+            *v_out = vC;
+            *t_out = tC;
+            return last_level | ORC_LVL_NOCONV;
+        }
         orc_micro_lc(i, npts, rho_in, v_in, t_in, 4 * N + 1, h_guess, p, ell, &rhoF, &vF, &tF);
         last_level = level_of(N);
 
@@ -219,13 +299,13 @@ int orc_single_tau_step_lc(int i, int npts, double rho_in, double v_in, double t
         }
         N *= 2;
     }
-    {
-        double rhoF, vF, tF;
-        orc_micro_lc(i, npts, rho_in, v_in, t_in, 4 * N + 1, h_guess, p, ell, &rhoF, &vF, &tF);
+    if (!cap_take(w, 4 * N + 2)) {
         *rho_out = rhoF;
         *v_out = vF;
         *t_out = tF;
+        return last_level | ORC_LVL_NOCONV;
     }
+    orc_micro_lc(i, npts, rho_in, v_in, t_in, 4 * N + 1, h_guess, p, ell, rho_out, v_out, t_out);
     return level_of(N) | ORC_LVL_NOCONV;
 }
 
@@ -255,12 +335,13 @@ int orc_adaptive_full_lc(int i, int npts, double r_in, double v_in, double ell,
     int64_t step_count = 0;
     const int64_t step_max = 100000000;
     int64_t taken = 0;
+    work_cap w = cap_of(p); /* one cap for all tau steps of this particle-step */
     while (1) {
         if (t_cur >= dt)
             break;
         double rho_next, v_next, t_next;
-        int lv = orc_single_tau_step_lc(i, npts, rho_current, v_current, t_cur, delta_tau, p,
-                                        ell, &rho_next, &v_next, &t_next);
+        int lv = single_tau_capped(i, npts, rho_current, v_current, t_cur, delta_tau, p, ell,
+                                   &rho_next, &v_next, &t_next, &w);
         taken++;
         if ((lv & ORC_LVL_MASK) > (rec & ORC_LVL_MASK))
             rec = (rec & ~ORC_LVL_MASK) | (lv & ORC_LVL_MASK);
@@ -281,7 +362,7 @@ int orc_adaptive_full_lc(int i, int npts, double r_in, double v_in, double ell,
         v_current = v_next;
         t_cur = t_next;
         step_count++;
-        if (step_count > step_max) {
+        if (step_count > step_max || w.hit) {
             *r_out = rho_current * rho_current;
             *v_out = v_current;
             if (n_tau_steps)

@@ -36,6 +36,12 @@ typedef struct {
     int32_t out_type;    /* 0 coarse, 1 fine, 2 "Richardson" 4F-3C              */
     int32_t N_taumin;
     int32_t identity_gravity; /* use_identity_gravity, nsphere.c:476           */
+    /* Not in the reference: force evaluations allowed per particle and step
+     * (0 = unlimited, the reference's behaviour).  Counted as the GPU library
+     * counts them -- 2 for the first coarse leapfrog pass, 2N+1 per level,
+     * 2N+2 / 4N+2 for the Levi-Civita coarse / fine passes -- and checked at
+     * the same points, so capped results stay comparable bit for bit. */
+    int64_t budget;
 } orc_params;
 
 /* Per-particle refinement record written by the step (one byte):

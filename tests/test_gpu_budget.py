@@ -1,0 +1,73 @@
+"""The per-particle work cap, GPU vs oracle.
+
+The reference has no cap: on workloads with strong-field Levi-Civita particles
+its first-order tau steps can need 2^28 subdivisions each (minutes of CPU per
+particle-step).  The library caps the force evaluations per particle-step
+(NS_OPT_BUDGET); the oracle mirrors the same counting, so capped results must
+still agree bit for bit, in every integrator form.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ns():
+    import nsphere_amd
+    return nsphere_amd
+
+
+def assert_bits(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype == np.float64:
+        bad = np.flatnonzero(a.view(np.uint64) != b.view(np.uint64))
+    else:
+        bad = np.flatnonzero(a != b)
+    assert bad.size == 0, f"{what}: {bad.size} differ, first at {bad[0]}: {a[bad[0]]!r} vs {b[bad[0]]!r}"
+
+
+def plunging_ic(ns, n, seed):
+    """Plummer with a 20 % tail at ell x 1e-3 plus particles placed near the
+    centre on radial plunges, so that the Levi-Civita branch needs deep levels."""
+    r, v, ell, ids = ns.generate_ic(0, n, seed, 100.0, 0.2, 1e-3)
+    rng = np.random.default_rng(seed)
+    k = n // 200
+    pick = rng.choice(n, k, replace=False)
+    r[pick] = rng.uniform(1e-4, 3e-3, k)
+    v[pick] = rng.uniform(-0.6, 0.6, k)
+    ell[pick] = rng.uniform(1e-6, 1e-4, k)
+    q = oracle.sort_perm(r)
+    return r[q].copy(), v[q].copy(), ell[q].copy(), ids[q].copy()
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("budget", [2, 64, 4096, 1 << 16])
+def test_capped_step_matches_capped_oracle(ns, form, budget):
+    n = 20_000
+    r, v, ell, ids = plunging_ic(ns, n, 17)
+    p = ns.unit_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
+        ctx.set_option(ns.NS_OPT_BUDGET, budget)
+        ctx.upload(r, v, ell, ids)
+        ctx.step(p)
+        ctx.sync()
+        hits = ctx.cap_hits()
+        gr, gv, gl, gid = ctx.download()
+        glv = ctx.levels()
+    lv, _ = oracle.step(r, v, ell, ids, oracle.from_step_params(p, budget))
+    assert ((lv & oracle.LVL_NOCONV) != 0).any()
+    assert hits > 0
+    assert_bits(gid, ids, "ids")
+    if form == 2:
+        # fast arithmetic: same control flow, a few ulp per operation
+        ok = (lv == glv)
+        assert ok.mean() > 0.995
+        np.testing.assert_allclose(gr[ok], r[ok], rtol=1e-9)
+        return
+    assert_bits(gr, r, "r")
+    assert_bits(gv, v, "v")
+    assert_bits(glv, lv, "levels")
