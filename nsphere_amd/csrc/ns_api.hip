@@ -31,7 +31,7 @@ struct ns_gpu_ctx {
     double *energy_partial = nullptr;
     double *energy_out = nullptr;  // device, 2 doubles
     unsigned long long *budget_hits = nullptr;
-    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
+    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_lcw = nullptr, *q_count = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool levels_valid = false;
@@ -203,7 +203,7 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_count};
+                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_lcw, c->q_count};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -244,8 +244,10 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.budget_hits = c->budget_hits;
     a.q_lf = c->q_lf;
     a.q_lc = c->q_lc;
+    a.q_lcw = c->q_lcw;
     a.q_count = c->q_count;
-    a.cap_N = 8;  // levels 0..3 inline in the bulk kernel
+    a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
+    a.cap_lc = 8;  // Levi-Civita tau-step levels 0..3 lane-per-particle
     return a;
 }
 
@@ -373,7 +375,8 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->budget_hits, 1);
     if (e == hipSuccess) e = dalloc(&c->q_lf, n);
     if (e == hipSuccess) e = dalloc(&c->q_lc, n);
-    if (e == hipSuccess) e = dalloc(&c->q_count, 2);
+    if (e == hipSuccess) e = dalloc(&c->q_lcw, n);
+    if (e == hipSuccess) e = dalloc(&c->q_count, 4);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));

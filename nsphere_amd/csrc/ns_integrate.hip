@@ -29,6 +29,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kDeepLfBlocks = 64;     // 256 waves for the wave-per-particle queue
 constexpr int kDeepLcBlocks = 1024;   // 262144 lanes for the Levi-Civita queue
+constexpr int kLcWaveBlocks = 128;    // 512 waves for escalated Levi-Civita particles
 
 __device__ inline uint64_t lanemask_lt()
 {
@@ -157,9 +158,34 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
             const Phys ph = make_phys(a, i, ell);
             Budget bud{a.budget, false};
             double r_new, v_new;
-            const int rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
-            write_result(a, i, r_new, v_new, rec, bud.hit);
+            const int rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_lc);
+            const bool escalate = rec == kEscalate;
+            if (!escalate)
+                write_result(a, i, r_new, v_new, rec, bud.hit);
+            wave_push(a.q_lcw, a.q_count + 2, escalate, (uint32_t)i);
         }
+    }
+}
+
+// One wave per Levi-Civita particle whose tau steps need more than cap_lc:
+// every tau step's levels run one per lane (restarting the particle from its
+// step-start state, so the result is that of the lane form).
+template <bool GATHER, bool FAST>
+__global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t count = a.q_count[2];
+    const uint32_t waves = gridDim.x * (kThreads / 64);
+    for (uint32_t q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); q < count; q += waves) {
+        const int64_t i = a.q_lcw[q];
+        double r, v, ell;
+        load_particle<GATHER>(a, i, r, v, ell);
+        const Phys ph = make_phys(a, i, ell);
+        Budget bud{a.budget, false};
+        double r_new, v_new;
+        const int rec = adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud);
+        if (lane == 0)
+            write_result(a, i, r_new, v_new, rec, bud.hit);
     }
 }
 
@@ -254,7 +280,7 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream)
         hipLaunchKernelGGL((k_integrate_plain<GATHER, FAST>), grid, dim3(kThreads), 0, stream, a);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, 2);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, 3);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
@@ -263,6 +289,9 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream)
         return e;
     hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST>), dim3(kDeepLfBlocks + kDeepLcBlocks), dim3(kThreads), 0,
                        stream, a);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((k_integrate_lc_wave<GATHER, FAST>), dim3(kLcWaveBlocks), dim3(kThreads), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -22,11 +22,13 @@ struct StepArgs {
     int64_t max_subdiv;
     int64_t budget;              // force evaluations allowed per particle per step
     int32_t out_type, N_taumin, identity_gravity;
-    // binned integrator: queues of ranks handed from the bulk kernel to the
-    // deep kernel (leapfrog particles past cap_N, Levi-Civita particles)
-    uint32_t *q_lf, *q_lc;
-    uint32_t *q_count;           // [0] leapfrog queue length, [1] LC queue length
-    int64_t cap_N;               // deepest N the bulk kernel tries inline
+    // binned integrator: queues of ranks handed between its kernels
+    uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
+    uint32_t *q_lc;              // Levi-Civita particles (lane per particle)
+    uint32_t *q_lcw;             // Levi-Civita particles past cap_lc in some tau step (wave)
+    uint32_t *q_count;           // lengths of q_lf, q_lc, q_lcw
+    int64_t cap_N;               // deepest leapfrog N tried lane-per-particle
+    int64_t cap_lc;              // deepest Levi-Civita N tried lane-per-particle
 };
 
 // Integrator forms (NS_OPT_INTEGRATOR).

@@ -330,10 +330,35 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
     t_out = t;
 }
 
-// doSingleTauStepAdaptiveLeviCivita, nsphere.c:2077-2186.
+__device__ inline void lc_output(int out_type, double rhoC, double vC, double tC, double rhoF, double vF,
+                                 double tF, double &rho_out, double &v_out, double &t_out)
+{
+    if (out_type == 0) {
+        rho_out = rhoC;
+        v_out = vC;
+        t_out = tC;
+    } else if (out_type == 1) {
+        rho_out = rhoF;
+        v_out = vF;
+        t_out = tF;
+    } else {
+        double rho_rich = 4.0 * rhoF - 3.0 * rhoC;
+        const double v_rich = 4.0 * vF - 3.0 * vC;
+        const double t_rich = 4.0 * tF - 3.0 * tC;
+        if (rho_rich < 0.0)
+            rho_rich = 0.0;
+        rho_out = rho_rich;
+        v_out = v_rich;
+        t_out = t_rich;
+    }
+}
+
+// doSingleTauStepAdaptiveLeviCivita, nsphere.c:2077-2186.  With cap_N > 0
+// the loop stops before a level whose N exceeds cap_N and returns kEscalate.
 template <bool FAST>
 __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
-                             double h, double &rho_out, double &v_out, double &t_out, Budget &bud)
+                             double h, double &rho_out, double &v_out, double &t_out, Budget &bud,
+                             int64_t cap_N)
 {
     int64_t N = 1;
     int last = 0;
@@ -348,6 +373,8 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
         micro_lc<FAST>(ph, rho_in, v_in, t_in, 2 * N + 1, h, rhoC, vC, tC);
     }
     while (N <= a.max_subdiv) {
+        if (cap_N > 0 && N > cap_N)
+            return kEscalate;
         if (!bud.take(4 * N + 2)) {
             rho_out = rhoC;
             v_out = vC;
@@ -360,24 +387,7 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
         const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
         const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
         if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
-            if (a.out_type == 0) {
-                rho_out = rhoC;
-                v_out = vC;
-                t_out = tC;
-            } else if (a.out_type == 1) {
-                rho_out = rhoF;
-                v_out = vF;
-                t_out = tF;
-            } else {
-                double rho_rich = 4.0 * rhoF - 3.0 * rhoC;
-                const double v_rich = 4.0 * vF - 3.0 * vC;
-                const double t_rich = 4.0 * tF - 3.0 * tC;
-                if (rho_rich < 0.0)
-                    rho_rich = 0.0;
-                rho_out = rho_rich;
-                v_out = v_rich;
-                t_out = t_rich;
-            }
+            lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
             return last;
         }
         N *= 2;
@@ -398,10 +408,106 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
     return level_of(N) | NS_LVL_NOCONV;
 }
 
-// doAdaptiveFullLeviCivita, nsphere.c:2231-2334.
+// doSingleTauStepAdaptiveLeviCivita, wave-cooperative form (all 64 lanes,
+// same particle).  Lane k runs the fine pass of level k (4*2^k+1 micro-steps,
+// k = 0..kmax+1 -- the last one is the non-convergence fallback), lane 63 the
+// coarse pass of level 0 (3 micro-steps).  All lanes step in lockstep; after
+// 4*2^k+1 iterations lanes k-1 (or 63) and k hold coarse and fine of level k
+// and the reference's test is evaluated.  Same results as the lane form, after
+// 4*2^K+1 iterations instead of about 8*2^K.
 template <bool FAST>
+__device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
+                                  double h, double &rho_out, double &v_out, double &t_out, Budget &bud)
+{
+    if (a.max_subdiv < 1)  // degenerate: no level loop, only the fallback pass
+        return single_tau_lc<FAST>(ph, a, rho_in, v_in, t_in, h, rho_out, v_out, t_out, bud, 0);
+    const int lane = threadIdx.x & 63;
+    int kmax = 63 - __clzll((unsigned long long)a.max_subdiv);
+    if (kmax > 61)
+        kmax = 61;
+    const bool coarse_lane = lane == 63;
+    const bool active = coarse_lane || lane <= kmax + 1;
+    const int64_t sub = coarse_lane ? 3 : (active ? 4 * ((int64_t)1 << lane) + 1 : 1);
+    const double dtau = h / (double)sub;
+    double rho = rho_in, v = v_in, t = t_in;
+    double f = force_lc<FAST>(ph, rho);
+    if (!bud.take(4)) {
+        rho_out = rho_in;
+        v_out = v_in;
+        t_out = t_in;
+        return NS_LVL_NOCONV | NS_LVL_MASK;
+    }
+    int64_t it = 0;
+    auto advance_to = [&](int64_t target) {
+        for (; it < target; it++) {
+            if (!(active && it < sub))
+                continue;
+            if constexpr (!FAST) {
+                const double v_half = v + 0.5 * dtau * f;
+                const double rho_next = rho + dtau * (0.5 * rho * v_half);
+                const double f2 = force_lc<false>(ph, rho_next);
+                const double v_next = v_half + 0.5 * dtau * f2;
+                const double rho_mid = 0.5 * (rho + rho_next);
+                const double dt_phys = dtau * (rho_mid * rho_mid);
+                t += dt_phys;
+                rho = rho_next;
+                v = v_next;
+                f = f2;
+            } else {
+                const double hd = 0.5 * dtau;
+                const double v_half = fma(hd, f, v);
+                const double rho_next = fma(dtau, 0.5 * rho * v_half, rho);
+                const double f2 = force_lc<true>(ph, rho_next);
+                const double rho_mid = 0.5 * (rho + rho_next);
+                t = fma(dtau, rho_mid * rho_mid, t);
+                v = fma(hd, f2, v_half);
+                rho = rho_next;
+                f = f2;
+            }
+        }
+    };
+    for (int k = 0; k <= kmax; k++) {
+        const int64_t Nk = (int64_t)1 << k;
+        const int src_c = (k == 0) ? 63 : k - 1;
+        if (!bud.take(4 * Nk + 2)) {
+            // coarse of level k: the previous fine pass (or lane 63)
+            rho_out = __shfl(rho, src_c, 64);
+            v_out = __shfl(v, src_c, 64);
+            t_out = __shfl(t, src_c, 64);
+            return (k == 0 ? 0 : level_of(Nk >> 1)) | NS_LVL_NOCONV;
+        }
+        advance_to(4 * Nk + 1);
+        const double rhoC = __shfl(rho, src_c, 64), vC = __shfl(v, src_c, 64), tC = __shfl(t, src_c, 64);
+        const double rhoF = __shfl(rho, k, 64), vF = __shfl(v, k, 64), tF = __shfl(t, k, 64);
+        const double rF = rhoF * rhoF;
+        const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
+        const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
+        if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
+            lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
+            return level_of(Nk);
+        }
+    }
+    // not converged: the fine pass at the overflowed N = 2^(kmax+1)
+    const int64_t Nf = (int64_t)2 << kmax;
+    if (!bud.take(4 * Nf + 2)) {
+        rho_out = __shfl(rho, kmax, 64);
+        v_out = __shfl(v, kmax, 64);
+        t_out = __shfl(t, kmax, 64);
+        return level_of(Nf >> 1) | NS_LVL_NOCONV;
+    }
+    advance_to(4 * Nf + 1);
+    rho_out = __shfl(rho, kmax + 1, 64);
+    v_out = __shfl(v, kmax + 1, 64);
+    t_out = __shfl(t, kmax + 1, 64);
+    return level_of(Nf) | NS_LVL_NOCONV;
+}
+
+// doAdaptiveFullLeviCivita, nsphere.c:2231-2334.  WAVE: every tau step on the
+// whole wave (all lanes call with the same particle).  cap_N (lane form
+// only): escalate with kEscalate when a tau step needs N > cap_N.
+template <bool FAST, bool WAVE = false>
 __device__ int adaptive_lc(const Phys &ph, const StepArgs &a, double r_in, double v_in, double &r_out,
-                           double &v_out, Budget &bud)
+                           double &v_out, Budget &bud, int64_t cap_N = 0)
 {
     int rec = NS_LVL_LC;
     if (r_in < 1.0e-30) {
@@ -424,8 +530,15 @@ __device__ int adaptive_lc(const Phys &ph, const StepArgs &a, double r_in, doubl
         if (t_cur >= dt)
             break;
         double rho_next, v_next, t_next;
-        const int lv = single_tau_lc<FAST>(ph, a, rho_current, v_current, t_cur, delta_tau, rho_next, v_next,
-                                           t_next, bud);
+        int lv;
+        if constexpr (WAVE)
+            lv = single_tau_lc_wave<FAST>(ph, a, rho_current, v_current, t_cur, delta_tau, rho_next, v_next,
+                                          t_next, bud);
+        else
+            lv = single_tau_lc<FAST>(ph, a, rho_current, v_current, t_cur, delta_tau, rho_next, v_next, t_next,
+                                     bud, cap_N);
+        if (lv == kEscalate)
+            return kEscalate;
         if ((lv & NS_LVL_MASK) > (rec & NS_LVL_MASK))
             rec = (rec & ~NS_LVL_MASK) | (lv & NS_LVL_MASK);
         rec |= (lv & NS_LVL_NOCONV);
