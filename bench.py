@@ -30,19 +30,27 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle for each kernel family of one step
-# (float64 state, 64-bit keys, uint32 permutation / ids, 8-bit digits).
-BYTES_PER_PARTICLE = {
-    "sort_histogram": 8.0,                             # read the key
-    # average over the 8 passes: the first reads the key (8) and writes key +
-    # payload (12); six middle passes read 12 and write 12; the last reads 12
-    # and writes the 4 B permutation
-    "sort_onesweep": (20.0 + 6 * 24.0 + 16.0) / 8.0,
-    # read perm 4, r v ell 24, id 4; write r v ell 24, id 4, key 8, level 1
-    "integrate": 32.0 + 37.0,
-    "radius_keys": 16.0,
-}
+def bytes_per_particle(state_bytes: int) -> dict:
+    """Algorithmic HBM bytes per particle of each kernel family of one step,
+    for float64 (8) or float32 (4) state: keys as wide as the state, uint32
+    permutation and ids, 8-bit digits (so state_bytes passes)."""
+    s = float(state_bytes)
+    passes = state_bytes
+    first = s + (s + 4)              # read key; write key + payload
+    middle = 2 * (s + 4)             # read and write key + payload
+    last = (s + 4) + 4               # read key + payload; write the permutation
+    return {
+        "sort_histogram": s,                                     # read the key
+        "sort_onesweep": (first + (passes - 2) * middle + last) / passes,
+        # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
+        "integrate": (4 + 3 * s + 4) + (3 * s + 4 + s + 1),
+        "radius_keys": 2 * s,
+    }
+
+
+
 FORMS = {"plain": 0, "binned": 1, "binned-fast": 2}
+PRECISIONS = {"f64": 0, "f32": 1}
 
 
 def cpu_info() -> str:
@@ -85,11 +93,13 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int) -> dict:
     }
 
 
-def measure(ns, form: str, state, p, steps: int, warmup: int, work_cap: int) -> dict:
+def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int) -> dict:
     n = state[0].size
     ctx = ns.GpuContext(n)
     chunk = 10
+    bpp = bytes_per_particle(4 if precision == "f32" else 8)
     try:
+        ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
         ctx.upload(*state)
@@ -122,11 +132,12 @@ def measure(ns, form: str, state, p, steps: int, warmup: int, work_cap: int) -> 
     for name, (launches, ms) in stats.items():
         avg_ms = ms / max(launches, 1)
         d = {"launches": launches, "total_ms": round(ms, 4), "avg_us": round(avg_ms * 1e3, 3)}
-        if name in BYTES_PER_PARTICLE:
-            d["algorithmic_GBps"] = round(BYTES_PER_PARTICLE[name] * n / (avg_ms * 1e-3) / 1e9, 1)
+        if name in bpp:
+            d["algorithmic_GBps"] = round(bpp[name] * n / (avg_ms * 1e-3) / 1e9, 1)
         kernels[name] = d
     lev = lv & ns.NS_LVL_MASK
     return {
+        "bpp": bpp,
         "seconds": el,
         "value": n * steps / el,
         "ms_per_step": el / steps * 1e3,
@@ -140,12 +151,13 @@ def measure(ns, form: str, state, p, steps: int, warmup: int, work_cap: int) -> 
     }
 
 
-def roofline_for(stats: dict, n: int, kernel: str) -> dict | None:
-    if kernel not in stats or kernel not in BYTES_PER_PARTICLE:
+def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
+    stats = run["stats"]
+    if kernel not in stats or kernel not in run["bpp"]:
         return None
     launches, ms = stats[kernel]
     avg_s = ms / launches * 1e-3
-    bpp = BYTES_PER_PARTICLE[kernel]
+    bpp = run["bpp"][kernel]
     achieved = bpp * n / avg_s / 1e9
     return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
@@ -160,6 +172,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=12_000_000)
     ap.add_argument("--seed", type=int, default=20250401)
     ap.add_argument("--form", choices=sorted(FORMS), default="binned-fast")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="f32",
+                    help="state precision of the chosen form (f32: float32 state and 32-bit keys, "
+                         "float64 arithmetic); the plain form is always f64")
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="steps for the plain-form comparison (default: --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -188,10 +203,10 @@ def main() -> int:
         state = ctx.download()
     print(f"[bench] N={n} initial state in {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
 
-    chosen = measure(ns, args.form, state, p, args.steps, args.warmup, args.budget)
+    chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget)
     plain = None
-    if not args.no_plain and args.form != "plain":
-        plain = measure(ns, "plain", state, p, args.plain_steps or args.steps, args.warmup, args.budget)
+    if not args.no_plain and (args.form, args.precision) != ("plain", "f64"):
+        plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget)
 
     dom = max(chosen["stats"].items(), key=lambda kv: kv[1][1])[0]
     out = {
@@ -208,9 +223,9 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plummer_n1.2e7_lowl5pct_dt1e-3", "n": n, "dt": p.dt, "form": args.form,
-                   "lowl_frac": 0.05, "lowl_scale": 1e-3, "seed": args.seed},
-        "roofline": roofline_for(chosen["stats"], n, dom),
-        "roofline_sort": roofline_for(chosen["stats"], n, "sort_onesweep"),
+                   "state": args.precision, "lowl_frac": 0.05, "lowl_scale": 1e-3, "seed": args.seed},
+        "roofline": roofline_for(chosen, n, dom),
+        "roofline_sort": roofline_for(chosen, n, "sort_onesweep"),
         "dominant_kernel": dom,
         "kernels": chosen["kernels"],
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),

@@ -83,6 +83,11 @@ enum {
                                  bit-identical to 0;
                               2 binned with the fast arithmetic form (one
                                  reciprocal per force, fused multiply-adds)   */
+    NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
+                              float32 r, v_r, ell in HBM and 32-bit radius
+                              keys (4 sort passes), float64 arithmetic in the
+                              integrator and the energy.  Switching converts
+                              the current state (round to nearest / exact). */
     NS_OPT_BUDGET = 4      /* force evaluations allowed per particle and step
                               (default 2^26); a particle that needs more is
                               stopped, flagged NS_LVL_NOCONV and counted

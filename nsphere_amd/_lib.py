@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnsphere_gpu.so")
 
 NS_OK, NS_ERR_ARG, NS_ERR_HIP, NS_ERR_NOMEM, NS_ERR_STATE, NS_ERR_NODEV = range(6)
-NS_OPT_TIEBREAK, NS_OPT_PROFILE, NS_OPT_INTEGRATOR, NS_OPT_BUDGET = 1, 2, 3, 4
+NS_OPT_TIEBREAK, NS_OPT_PROFILE, NS_OPT_INTEGRATOR, NS_OPT_BUDGET, NS_OPT_PRECISION = 1, 2, 3, 4, 5
 FORM_PLAIN, FORM_BINNED, FORM_BINNED_FAST = 0, 1, 2
 NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80
 NS_MAX_KERNEL_STATS = 16
@@ -59,7 +59,7 @@ class StepParams(ctypes.Structure):
 def _load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise NativeLibraryMissing(
-            f"{LIB_PATH} is not built; run `python -m nsphere_amd.build` (hipcc, gfx950)")
+            f"{LIB_PATH} is not built; run `python nsphere_amd/build.py` (hipcc, gfx950)")
     lib = ctypes.CDLL(LIB_PATH)
     c = ctypes
     vp, i64, i32, dbl = c.c_void_p, c.c_int64, c.c_int, c.c_double

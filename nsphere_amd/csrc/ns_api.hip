@@ -23,6 +23,10 @@ struct ns_gpu_ctx {
     hipStream_t stream = nullptr;
     int cur = 0;
     double *r[2] = {}, *v[2] = {}, *l[2] = {};
+    // mixed-precision form: float32 state (allocated on first use); the
+    // float64 buffers then serve only as upload / download staging
+    float *rf[2] = {}, *vf[2] = {}, *lf[2] = {};
+    bool f32 = false;
     uint32_t *id[2] = {};
     uint8_t *lvl = nullptr;
     uint32_t *perm = nullptr;
@@ -203,10 +207,45 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_lcw, c->q_count};
+                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_lcw, c->q_count,
+                    c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1]};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
+}
+
+// Float32 state buffers of the mixed-precision form, allocated on first use.
+int ensure_f32(ns_gpu_ctx *c)
+{
+    for (int b = 0; b < 2; b++) {
+        float **bufs[] = {&c->rf[b], &c->vf[b], &c->lf[b]};
+        for (float **p : bufs) {
+            if (*p)
+                continue;
+            hipError_t e = dalloc(p, c->n);
+            if (e != hipSuccess)
+                return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP,
+                                 "float32 state allocation (npts=%lld): %s", (long long)c->n, hipGetErrorString(e));
+        }
+    }
+    return NS_OK;
+}
+
+// Float64 state of buffer b -> float32 state of buffer b (round to nearest), or back.
+int state_to_f32(ns_gpu_ctx *c, int b)
+{
+    HIP_TRY((ns::launch_convert<float, double>(c->r[b], c->rf[b], c->n, c->stream)));
+    HIP_TRY((ns::launch_convert<float, double>(c->v[b], c->vf[b], c->n, c->stream)));
+    HIP_TRY((ns::launch_convert<float, double>(c->l[b], c->lf[b], c->n, c->stream)));
+    return NS_OK;
+}
+
+int state_to_f64(ns_gpu_ctx *c, int b)
+{
+    HIP_TRY((ns::launch_convert<double, float>(c->rf[b], c->r[b], c->n, c->stream)));
+    HIP_TRY((ns::launch_convert<double, float>(c->vf[b], c->v[b], c->n, c->stream)));
+    HIP_TRY((ns::launch_convert<double, float>(c->lf[b], c->l[b], c->n, c->stream)));
+    return NS_OK;
 }
 
 int check_ctx(const ns_gpu_ctx *c, const char *fn)
@@ -256,11 +295,14 @@ int sort_perm(ns_gpu_ctx *c)
 {
     if (!c->keys_valid) {
         c->prof.begin(ns::K_KEYS);
-        HIP_TRY(ns::launch_radius_keys(c->r[c->cur], c->sw.keys[0], c->n, c->stream));
+        if (c->f32)
+            HIP_TRY(ns::launch_radius_keys(c->rf[c->cur], reinterpret_cast<uint32_t *>(c->sw.keys[0]), c->n,
+                                           c->stream));
+        else
+            HIP_TRY(ns::launch_radius_keys(c->r[c->cur], c->sw.keys[0], c->n, c->stream));
         c->prof.end(ns::K_KEYS);
     }
-    HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->tiebreak ? c->id[c->cur] : nullptr, c->perm, c->stream,
-                                &c->prof));
+    HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->f32 ? 32 : 64, c->perm, c->stream, &c->prof));
     c->keys_valid = false;  // the passes overwrite keys[0]
     return NS_OK;
 }
@@ -271,19 +313,28 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
     // gather: previous order in buffer cur -> new rank order in cur^1;
     // otherwise in place (each thread reads and writes only its own index).
     const int s = c->cur, d = gather ? (c->cur ^ 1) : c->cur;
-    a.r0 = c->r[s];
-    a.v0 = c->v[s];
-    a.l0 = c->l[s];
+    if (c->f32) {
+        a.r0 = c->rf[s];
+        a.v0 = c->vf[s];
+        a.l0 = c->lf[s];
+        a.r1 = c->rf[d];
+        a.v1 = c->vf[d];
+        a.l1 = gather ? c->lf[d] : nullptr;
+    } else {
+        a.r0 = c->r[s];
+        a.v0 = c->v[s];
+        a.l0 = c->l[s];
+        a.r1 = c->r[d];
+        a.v1 = c->v[d];
+        a.l1 = gather ? c->l[d] : nullptr;
+    }
     a.id0 = c->id[s];
     a.perm = gather ? c->perm : nullptr;
-    a.r1 = c->r[d];
-    a.v1 = c->v[d];
-    a.l1 = gather ? c->l[d] : nullptr;
     a.id1 = gather ? c->id[d] : nullptr;
     a.keys = c->sw.keys[0];
     a.lvl = c->lvl;
     c->prof.begin(ns::K_INTEGRATE);
-    HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->stream));
+    HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream));
     c->prof.end(ns::K_INTEGRATE);
     c->cur = d;
     c->keys_valid = true;
@@ -436,6 +487,25 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
             return set_error(NS_ERR_ARG, "NS_OPT_BUDGET must be >= 2 force evaluations");
         c->budget = value;
         return NS_OK;
+    case NS_OPT_PRECISION: {
+        if (value != 0 && value != 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_PRECISION takes 0 (float64 state) or 1 (float32 state)");
+        const bool want = value == 1;
+        if (want == c->f32)
+            return NS_OK;
+        DeviceGuard g(c->device);
+        if (want)
+            if (int rc = ensure_f32(c))
+                return rc;
+        if (c->uploaded && c->n > 0) {  // carry the current state over
+            if (int rc = want ? state_to_f32(c, c->cur) : state_to_f64(c, c->cur))
+                return rc;
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+        c->f32 = want;
+        c->keys_valid = false;
+        return NS_OK;
+    }
     default:
         return set_error(NS_ERR_ARG, "ns_gpu_set_option: unknown option %d", option);
     }
@@ -455,6 +525,9 @@ int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const dou
         HIP_TRY(hipMemcpyAsync(c->v[0], v, nd, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->l[0], ell, nd, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->id[0], id, ni, hipMemcpyHostToDevice, c->stream));
+        if (c->f32)
+            if (int rc = state_to_f32(c, 0))
+                return rc;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->uploaded = true;
@@ -495,6 +568,9 @@ int ns_gpu_download_soa(ns_gpu_ctx *c, double *r, double *v, double *ell, uint32
     const size_t nd = (size_t)c->n * sizeof(double), ni = (size_t)c->n * sizeof(uint32_t);
     const int s = c->cur;
     if (c->n > 0) {
+        if (c->f32)  // widen into the float64 staging buffers of the same index
+            if (int rc = state_to_f64(c, s))
+                return rc;
         if (r) HIP_TRY(hipMemcpyAsync(r, c->r[s], nd, hipMemcpyDeviceToHost, c->stream));
         if (v) HIP_TRY(hipMemcpyAsync(v, c->v[s], nd, hipMemcpyDeviceToHost, c->stream));
         if (ell) HIP_TRY(hipMemcpyAsync(ell, c->l[s], nd, hipMemcpyDeviceToHost, c->stream));
@@ -525,6 +601,8 @@ int ns_gpu_device_state(ns_gpu_ctx *c, double **r, double **v, double **ell, uin
 {
     if (int rc = check_ctx(c, "ns_gpu_device_state"))
         return rc;
+    if (c->f32)
+        return set_error(NS_ERR_STATE, "ns_gpu_device_state: the state is float32 (NS_OPT_PRECISION=1)");
     if (r) *r = c->r[c->cur];
     if (v) *v = c->v[c->cur];
     if (ell) *ell = c->l[c->cur];
@@ -543,8 +621,12 @@ int ns_gpu_sort_particles(ns_gpu_ctx *c)
         return rc;
     const int s = c->cur, d = c->cur ^ 1;
     c->prof.begin(ns::K_GATHER);
-    HIP_TRY(ns::launch_gather_state(c->r[s], c->v[s], c->l[s], c->id[s], c->perm, c->r[d], c->v[d], c->l[d],
-                                    c->id[d], c->n, c->stream));
+    if (c->f32)
+        HIP_TRY(ns::launch_gather_state<float>(c->rf[s], c->vf[s], c->lf[s], c->id[s], c->perm, c->rf[d], c->vf[d],
+                                               c->lf[d], c->id[d], c->n, c->stream));
+    else
+        HIP_TRY(ns::launch_gather_state<double>(c->r[s], c->v[s], c->l[s], c->id[s], c->perm, c->r[d], c->v[d],
+                                                c->l[d], c->id[d], c->n, c->stream));
     c->prof.end(ns::K_GATHER);
     c->cur = d;
     c->keys_valid = false;
@@ -669,9 +751,15 @@ int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
         return rc;
     const double m = p->halo_mass / (double)c->n;
     c->prof.begin(ns::K_ENERGY);
-    HIP_TRY(ns::launch_energy(c->r[c->cur], c->v[c->cur], c->l[c->cur], c->perm, c->n, m,
-                              p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity, c->energy_partial,
-                              c->energy_out, c->stream));
+    const int s = c->cur;
+    if (c->f32)
+        HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], c->perm, c->n, m, p->vel_conv_sq * p->G,
+                                         p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
+                                         c->stream));
+    else
+        HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], c->perm, c->n, m, p->vel_conv_sq * p->G,
+                                          p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
+                                          c->stream));
     c->prof.end(ns::K_ENERGY);
     double out[2];
     HIP_TRY(hipMemcpyAsync(out, c->energy_out, sizeof out, hipMemcpyDeviceToHost, c->stream));

@@ -40,7 +40,34 @@ __host__ __device__ inline uint64_t radius_key(double r)
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-// Device workspace of one radix sort.
+// The same total order on float32 radii (the state of the mixed-precision
+// form): a 32-bit key, so the sort needs four 8-bit passes instead of eight.
+__host__ __device__ inline uint32_t radius_key(float r)
+{
+    uint32_t u;
+    if (r != r) {
+        u = 0x7FC00000u;
+    } else if (r == 0.0f) {
+        u = 0;
+    } else {
+        u = (uint32_t)__builtin_bit_cast(uint32_t, r);
+    }
+    return (u >> 31) ? ~u : (u | 0x80000000u);
+}
+
+template <typename S>
+struct KeyOf;
+template <>
+struct KeyOf<double> {
+    using type = uint64_t;
+};
+template <>
+struct KeyOf<float> {
+    using type = uint32_t;
+};
+
+// Device workspace of one radix sort.  Key buffers hold 64-bit keys, or
+// 32-bit keys in their first half.
 struct SortWork {
     uint64_t *keys[2];   // ping-pong key buffers (keys[0] holds the input)
     uint32_t *vals[2];   // ping-pong payload buffers
@@ -52,12 +79,12 @@ struct SortWork {
 
 // Launchers (ns_sort.hip).  All asynchronous on `stream`.
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream);
-// Sorts keys[0] (stable); payload = input position, or ids (tiebreak) when
-// tie_ids != nullptr.  On return perm_out[k] = input position of rank k.
-// perm_out may alias neither vals buffer.
+hipError_t launch_radius_keys(const float *r, uint32_t *keys, int64_t n, hipStream_t stream);
+// Sorts the key_bits-wide (32 or 64) keys in keys[0], stably.  On return
+// perm_out[k] = input position of rank k (perm_out aliases no vals buffer).
 struct Profiler;
-hipError_t radix_sort_perm(SortWork &w, int64_t n, const uint32_t *tie_ids, uint32_t *perm_out,
-                           hipStream_t stream, Profiler *prof);
+hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_out, hipStream_t stream,
+                           Profiler *prof);
 
 // Kernel-family ids for profiling.
 enum KernelId : int {

@@ -7,15 +7,20 @@
 // is nearly sorted, so the gather is nearly sequential), writes ell and id in
 // the new rank order, integrates (r, v_r) at frozen rank i and writes r, v,
 // the radius key of the new r (the next sort's input) and a level record.
+// State is float64, or float32 in the mixed-precision form: loaded values
+// are widened to float64, all arithmetic is float64, results are rounded to
+// the state type once, and the key is taken from the stored value.
 //
 // FORM_PLAIN: one lane per particle runs the whole adaptive loop.  Its cost
 //   is set by the deepest particle's sequential chain.
 // FORM_BINNED(_FAST): a bulk kernel runs levels 0..3 lane-per-particle and
-//   hands the rest to queues; a deep kernel then runs each queued leapfrog
-//   particle on a whole wave with one adaptive level per lane (the levels of
-//   one particle are independent passes from the same start), and the
-//   Levi-Civita particles lane-per-particle.  Results are identical to the
-//   plain form (strict) because every level is the same computation.
+//   queues the rest; a deep kernel then runs each queued leapfrog particle on
+//   a whole wave with one adaptive level per lane (the levels of one particle
+//   are independent passes from the same start) and the Levi-Civita particles
+//   lane-per-particle up to level 3 per tau step; a last kernel runs the
+//   Levi-Civita particles that need more on a whole wave, one level per lane.
+//   Results are identical to the plain form (strict arithmetic) because every
+//   level is the same computation.
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
@@ -54,39 +59,47 @@ __device__ inline void wave_push(uint32_t *q, uint32_t *count, bool pred, uint32
         q[base + (uint32_t)__popcll(m & lanemask_lt())] = value;
 }
 
+template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit)
 {
-    a.r1[i] = r;
-    a.v1[i] = v;
+    const S rs = (S)r;
+    static_cast<S *>(a.r1)[i] = rs;
+    static_cast<S *>(a.v1)[i] = (S)v;
     if (a.keys)
-        a.keys[i] = radius_key(r);
+        static_cast<typename KeyOf<S>::type *>(a.keys)[i] = radius_key(rs);
     if (a.lvl)
         a.lvl[i] = (uint8_t)rec;
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
 }
 
-template <bool GATHER>
+template <bool GATHER, typename S>
 __device__ inline void load_particle(const StepArgs &a, int64_t i, double &r, double &v, double &ell)
 {
     const int64_t src = GATHER ? (int64_t)a.perm[i] : i;
-    r = a.r0[src];
-    v = a.v0[src];
-    ell = a.l0[src];
+    r = (double)static_cast<const S *>(a.r0)[src];
+    v = (double)static_cast<const S *>(a.v0)[src];
+    ell = (double)static_cast<const S *>(a.l0)[src];
 }
 
-template <bool GATHER, bool FAST>
+template <bool GATHER, typename S>
+__device__ inline void carry_ell_id(const StepArgs &a, int64_t i, double ell)
+{
+    if (GATHER) {
+        static_cast<S *>(a.l1)[i] = (S)ell;
+        a.id1[i] = a.id0[a.perm[i]];
+    }
+}
+
+template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n)
         return;
     double r, v, ell;
-    load_particle<GATHER>(a, i, r, v, ell);
-    if (GATHER) {
-        a.l1[i] = ell;
-        a.id1[i] = a.id0[a.perm[i]];
-    }
+    load_particle<GATHER, S>(a, i, r, v, ell);
+    carry_ell_id<GATHER, S>(a, i, ell);
     const Phys ph = make_phys(a, i, ell);
     Budget bud{a.budget, false};
     double r_new, v_new;
@@ -95,10 +108,10 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
         rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
     else
         rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
-    write_result(a, i, r_new, v_new, rec, bud.hit);
+    write_result<S>(a, i, r_new, v_new, rec, bud.hit);
 }
 
-template <bool GATHER, bool FAST>
+template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,11 +119,8 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
     bool to_lc = false, to_lf = false;
     if (valid) {
         double r, v, ell;
-        load_particle<GATHER>(a, i, r, v, ell);
-        if (GATHER) {
-            a.l1[i] = ell;
-            a.id1[i] = a.id0[a.perm[i]];
-        }
+        load_particle<GATHER, S>(a, i, r, v, ell);
+        carry_ell_id<GATHER, S>(a, i, ell);
         if (takes_levi_civita(a, i, r, ell)) {
             to_lc = true;
         } else {
@@ -121,7 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
             if (rec == kEscalate)
                 to_lf = true;
             else
-                write_result(a, i, r_new, v_new, rec, bud.hit);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
         }
     }
     wave_push(a.q_lc, a.q_count + 1, to_lc, (uint32_t)i);
@@ -130,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 
 // Blocks [0, kDeepLfBlocks): one wave per queued leapfrog particle.
 // Blocks [kDeepLfBlocks, ...): one lane per queued Levi-Civita particle.
-template <bool GATHER, bool FAST>
+template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 {
     if (blockIdx.x < kDeepLfBlocks) {
@@ -140,13 +150,13 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
         for (uint32_t q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); q < count; q += waves) {
             const int64_t i = a.q_lf[q];
             double r, v, ell;
-            load_particle<GATHER>(a, i, r, v, ell);
+            load_particle<GATHER, S>(a, i, r, v, ell);
             const Phys ph = make_phys(a, i, ell);
             Budget bud{a.budget, false};
             double r_new, v_new;
             const int rec = adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
             if (lane == 0)
-                write_result(a, i, r_new, v_new, rec, bud.hit);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
         }
     } else {
         const uint32_t count = a.q_count[1];
@@ -154,14 +164,14 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
         for (uint32_t q = (blockIdx.x - kDeepLfBlocks) * kThreads + threadIdx.x; q < count; q += lanes) {
             const int64_t i = a.q_lc[q];
             double r, v, ell;
-            load_particle<GATHER>(a, i, r, v, ell);
+            load_particle<GATHER, S>(a, i, r, v, ell);
             const Phys ph = make_phys(a, i, ell);
             Budget bud{a.budget, false};
             double r_new, v_new;
             const int rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_lc);
             const bool escalate = rec == kEscalate;
             if (!escalate)
-                write_result(a, i, r_new, v_new, rec, bud.hit);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
             wave_push(a.q_lcw, a.q_count + 2, escalate, (uint32_t)i);
         }
     }
@@ -170,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 // One wave per Levi-Civita particle whose tau steps need more than cap_lc:
 // every tau step's levels run one per lane (restarting the particle from its
 // step-start state, so the result is that of the lane form).
-template <bool GATHER, bool FAST>
+template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -179,13 +189,13 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
     for (uint32_t q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); q < count; q += waves) {
         const int64_t i = a.q_lcw[q];
         double r, v, ell;
-        load_particle<GATHER>(a, i, r, v, ell);
+        load_particle<GATHER, S>(a, i, r, v, ell);
         const Phys ph = make_phys(a, i, ell);
         Budget bud{a.budget, false};
         double r_new, v_new;
         const int rec = adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud);
         if (lane == 0)
-            write_result(a, i, r_new, v_new, rec, bud.hit);
+            write_result<S>(a, i, r_new, v_new, rec, bud.hit);
     }
 }
 
@@ -195,11 +205,11 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
         p[threadIdx.x] = 0;
 }
 
-__global__ void k_gather_state(const double *__restrict__ r0, const double *__restrict__ v0,
-                               const double *__restrict__ l0, const uint32_t *__restrict__ id0,
-                               const uint32_t *__restrict__ perm, double *__restrict__ r1,
-                               double *__restrict__ v1, double *__restrict__ l1, uint32_t *__restrict__ id1,
-                               int64_t n)
+template <typename S>
+__global__ void k_gather_state(const S *__restrict__ r0, const S *__restrict__ v0, const S *__restrict__ l0,
+                               const uint32_t *__restrict__ id0, const uint32_t *__restrict__ perm,
+                               S *__restrict__ r1, S *__restrict__ v1, S *__restrict__ l1,
+                               uint32_t *__restrict__ id1, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -209,6 +219,14 @@ __global__ void k_gather_state(const double *__restrict__ r0, const double *__re
         l1[i] = l0[p];
         id1[i] = id0[p];
     }
+}
+
+template <typename D, typename S>
+__global__ void k_convert(const S *__restrict__ src, D *__restrict__ dst, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = (D)src[i];
 }
 
 __global__ void k_inverse_map(const uint32_t *__restrict__ id, int32_t *__restrict__ inv, int64_t n)
@@ -226,10 +244,10 @@ constexpr int kEnergyThreads = 256;
 // Per-block float64 partial sums of K and W in a fixed tree order.  Rank i
 // is the position in radius order: particle perm[i] (perm may be null when
 // the arrays are already sorted).
+template <typename S>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
-    const double *__restrict__ r, const double *__restrict__ v, const double *__restrict__ ell,
-    const uint32_t *__restrict__ perm, int64_t n, double m, double gpre, double halo_mass, int idg,
-    double *__restrict__ partial)
+    const S *__restrict__ r, const S *__restrict__ v, const S *__restrict__ ell, const uint32_t *__restrict__ perm,
+    int64_t n, double m, double gpre, double halo_mass, int idg, double *__restrict__ partial)
 {
     __shared__ double sk[kEnergyThreads], sw[kEnergyThreads];
     double k = 0.0, w = 0.0;
@@ -272,51 +290,75 @@ __global__ void k_energy_final(const double *__restrict__ partial, int nblocks, 
     }
 }
 
-template <bool GATHER, bool FAST>
+template <bool GATHER, bool FAST, typename S>
 hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream)
 {
     const dim3 grid((unsigned)((a.n + kThreads - 1) / kThreads));
     if (form == FORM_PLAIN) {
-        hipLaunchKernelGGL((k_integrate_plain<GATHER, FAST>), grid, dim3(kThreads), 0, stream, a);
+        hipLaunchKernelGGL((k_integrate_plain<GATHER, FAST, S>), grid, dim3(kThreads), 0, stream, a);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, 3);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST>), grid, dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST, S>), grid, dim3(kThreads), 0, stream, a);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST>), dim3(kDeepLfBlocks + kDeepLcBlocks), dim3(kThreads), 0,
-                       stream, a);
+    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(kDeepLfBlocks + kDeepLcBlocks), dim3(kThreads),
+                       0, stream, a);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    hipLaunchKernelGGL((k_integrate_lc_wave<GATHER, FAST>), dim3(kLcWaveBlocks), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((k_integrate_lc_wave<GATHER, FAST, S>), dim3(kLcWaveBlocks), dim3(kThreads), 0, stream, a);
     return hipGetLastError();
+}
+
+template <typename S>
+hipError_t launch_typed(const StepArgs &a, bool gather, int form, hipStream_t stream)
+{
+    const bool fast = form == FORM_BINNED_FAST;
+    if (gather)
+        return fast ? launch_form<true, true, S>(a, form, stream) : launch_form<true, false, S>(a, form, stream);
+    return fast ? launch_form<false, true, S>(a, form, stream) : launch_form<false, false, S>(a, form, stream);
 }
 
 } // namespace
 
-hipError_t launch_integrate(const StepArgs &a, bool gather, int form, hipStream_t stream)
+hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream)
 {
     if (a.n <= 0)
         return hipSuccess;
-    const bool fast = form == FORM_BINNED_FAST;
-    if (gather)
-        return fast ? launch_form<true, true>(a, form, stream) : launch_form<true, false>(a, form, stream);
-    return fast ? launch_form<false, true>(a, form, stream) : launch_form<false, false>(a, form, stream);
+    return f32 ? launch_typed<float>(a, gather, form, stream) : launch_typed<double>(a, gather, form, stream);
 }
 
-hipError_t launch_gather_state(const double *r0, const double *v0, const double *l0, const uint32_t *id0,
-                               const uint32_t *perm, double *r1, double *v1, double *l1, uint32_t *id1,
-                               int64_t n, hipStream_t stream)
+template <typename S>
+hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
+                               S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_gather_state, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream,
-                       r0, v0, l0, id0, perm, r1, v1, l1, id1, n);
+    hipLaunchKernelGGL(k_gather_state<S>, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       stream, r0, v0, l0, id0, perm, r1, v1, l1, id1, n);
     return hipGetLastError();
 }
+template hipError_t launch_gather_state<double>(const double *, const double *, const double *, const uint32_t *,
+                                                const uint32_t *, double *, double *, double *, uint32_t *,
+                                                int64_t, hipStream_t);
+template hipError_t launch_gather_state<float>(const float *, const float *, const float *, const uint32_t *,
+                                               const uint32_t *, float *, float *, float *, uint32_t *, int64_t,
+                                               hipStream_t);
+
+template <typename D, typename S>
+hipError_t launch_convert(const S *src, D *dst, int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL((k_convert<D, S>), dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       stream, src, dst, n);
+    return hipGetLastError();
+}
+template hipError_t launch_convert<float, double>(const double *, float *, int64_t, hipStream_t);
+template hipError_t launch_convert<double, float>(const float *, double *, int64_t, hipStream_t);
 
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream)
 {
@@ -335,12 +377,12 @@ int energy_blocks(int64_t n)
     return b < 1 ? 1 : (int)b;
 }
 
-hipError_t launch_energy(const double *r, const double *v, const double *ell, const uint32_t *perm, int64_t n,
-                         double m, double gpre, double halo_mass, int idg, double *partial, double *out,
-                         hipStream_t stream)
+template <typename S>
+hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, int64_t n, double m,
+                         double gpre, double halo_mass, int idg, double *partial, double *out, hipStream_t stream)
 {
     const int blocks = energy_blocks(n);
-    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, n, m,
+    hipLaunchKernelGGL(k_energy_partial<S>, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, n, m,
                        gpre, halo_mass, idg, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -348,5 +390,9 @@ hipError_t launch_energy(const double *r, const double *v, const double *ell, co
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, stream, partial, blocks, out);
     return hipGetLastError();
 }
+template hipError_t launch_energy<double>(const double *, const double *, const double *, const uint32_t *,
+                                          int64_t, double, double, double, int, double *, double *, hipStream_t);
+template hipError_t launch_energy<float>(const float *, const float *, const float *, const uint32_t *, int64_t,
+                                         double, double, double, int, double *, double *, hipStream_t);
 
 } // namespace ns

@@ -7,14 +7,16 @@
 
 namespace ns {
 
-// Everything the fused step kernel needs, passed by value.
+// Everything the fused step kernel needs, passed by value.  State arrays are
+// float64 (keys 64-bit) or, in the mixed-precision form, float32 (keys
+// 32-bit); the kernels are instantiated for each.
 struct StepArgs {
-    const double *r0, *v0, *l0;  // state in the previous rank order
+    const void *r0, *v0, *l0;    // state in the previous rank order
     const uint32_t *id0;
     const uint32_t *perm;        // new rank -> previous position (GATHER form)
-    double *r1, *v1, *l1;        // state in the new rank order
+    void *r1, *v1, *l1;          // state in the new rank order
     uint32_t *id1;
-    uint64_t *keys;              // radius keys of the new r (may be null)
+    void *keys;                  // radius keys of the new r (may be null)
     uint8_t *lvl;                // level records (may be null)
     unsigned long long *budget_hits;
     int64_t n;
@@ -38,14 +40,17 @@ enum IntegratorForm : int {
     FORM_BINNED_FAST = 2   // as BINNED with the fast arithmetic form
 };
 
-hipError_t launch_integrate(const StepArgs &a, bool gather, int form, hipStream_t stream);
-hipError_t launch_gather_state(const double *r0, const double *v0, const double *l0, const uint32_t *id0,
-                               const uint32_t *perm, double *r1, double *v1, double *l1, uint32_t *id1,
-                               int64_t n, hipStream_t stream);
+hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream);
+template <typename S>
+hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
+                               S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream);
 int energy_blocks(int64_t n);
-hipError_t launch_energy(const double *r, const double *v, const double *ell, const uint32_t *perm,
-                         int64_t n, double m, double gpre, double halo_mass, int idg, double *partial,
-                         double *out, hipStream_t stream);
+template <typename S>
+hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, int64_t n, double m,
+                         double gpre, double halo_mass, int idg, double *partial, double *out, hipStream_t stream);
+// Element-wise precision conversion (float64 <-> float32, round to nearest).
+template <typename D, typename S>
+hipError_t launch_convert(const S *src, D *dst, int64_t n, hipStream_t stream);
 
 } // namespace ns

@@ -5,15 +5,16 @@
 // comparator of nsphere.c:8297-8306).  An LSD radix sort is stable, so keys
 // that tie keep their input (previous-step) order -- the order quadsort gives.
 //
-// Design (one HIP stream, wave64):
-//   * one upfront pass reads every key once and builds all eight 8-bit digit
+// Design (one HIP stream, wave64), for 64-bit keys (float64 radii, 8 passes)
+// or 32-bit keys (float32 radii, 4 passes):
+//   * one upfront pass reads every key once and builds all the 8-bit digit
 //     histograms (LDS-privatised, run-length-combined counters);
-//   * eight "onesweep" passes, one kernel each: a workgroup takes a 4096-key
-//     tile by atomic ticket, ranks its keys stably per digit with wave
-//     ballots (a 64-lane match of the 8 digit bits), publishes its digit
-//     counts and resolves its global digit offsets by decoupled lookback over
-//     earlier tiles, stages the tile in LDS in digit order and writes it out
-//     so that each digit run leaves as one coalesced segment;
+//   * one "onesweep" kernel per pass: a workgroup takes a 4096-key tile by
+//     atomic ticket, ranks its keys stably per digit with wave ballots (a
+//     64-lane match of the 8 digit bits), publishes its digit counts and
+//     resolves its global digit offsets by decoupled lookback over earlier
+//     tiles, stages the tile in LDS in digit order and writes it out so that
+//     each digit run leaves as one coalesced segment;
 //   * pass 0 synthesises the payload (input positions) instead of reading
 //     it, and the last pass writes only the payload (the permutation).
 #include <hip/hip_runtime.h>
@@ -79,7 +80,8 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_wave, ui
     return before + incl - x;
 }
 
-__global__ void k_radius_keys(const double *__restrict__ r, uint64_t *__restrict__ keys, int64_t n)
+template <typename S>
+__global__ void k_radius_keys(const S *__restrict__ r, typename KeyOf<S>::type *__restrict__ keys, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
@@ -102,49 +104,52 @@ __global__ void k_sort_zero(uint32_t *__restrict__ hist, int nhist, uint32_t *__
         counters[i] = 0;
 }
 
-// All eight digit histograms in one read of the keys.  Each thread takes
+// All digit histograms in one read of the keys.  Each thread takes
 // kHistItems consecutive keys and adds runs of equal digits with one LDS
 // atomic, which keeps the nearly-constant high digits of nearly-sorted radii
 // from serialising on one bank.
-__global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const uint64_t *__restrict__ keys,
-                                                                  int64_t n,
+template <typename KeyT>
+__global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__restrict__ keys, int64_t n,
                                                                   uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t s_hist[kKeyPasses * kRadix];
-    for (int i = threadIdx.x; i < kKeyPasses * kRadix; i += kHistThreads)
+    constexpr int kPasses = (int)sizeof(KeyT);
+    __shared__ uint32_t s_hist[kPasses * kRadix];
+    for (int i = threadIdx.x; i < kPasses * kRadix; i += kHistThreads)
         s_hist[i] = 0;
     __syncthreads();
 
     const int64_t per_block = (int64_t)kHistThreads * kHistItems;
     for (int64_t base = (int64_t)blockIdx.x * per_block; base < n; base += (int64_t)gridDim.x * per_block) {
         const int64_t first = base + (int64_t)threadIdx.x * kHistItems;
-        uint64_t k[kHistItems];
+        KeyT k[kHistItems];
         int cnt = 0;
         if (first + kHistItems <= n) {
-            const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(keys + first);
+            constexpr int kPerVec = 16 / (int)sizeof(KeyT);
+            const uint4 *src = reinterpret_cast<const uint4 *>(keys + first);
 #pragma unroll
-            for (int j = 0; j < kHistItems / 2; j++) {
-                ulonglong2 t = src[j];
-                k[2 * j] = t.x;
-                k[2 * j + 1] = t.y;
+            for (int j = 0; j < kHistItems / kPerVec; j++) {
+                const uint4 t = src[j];
+                const KeyT *tk = reinterpret_cast<const KeyT *>(&t);
+#pragma unroll
+                for (int e = 0; e < kPerVec; e++)
+                    k[j * kPerVec + e] = tk[e];
             }
             cnt = kHistItems;
         } else {
 #pragma unroll
-            for (int j = 0; j < kHistItems; j++) {
-                k[j] = (first + j < n) ? keys[first + j] : 0;
-            }
+            for (int j = 0; j < kHistItems; j++)
+                k[j] = (first + j < n) ? keys[first + j] : (KeyT)0;
             cnt = first < n ? (int)(n - first) : 0;
         }
         if (cnt > 0) {
 #pragma unroll
-            for (int p = 0; p < kKeyPasses; p++) {
+            for (int p = 0; p < kPasses; p++) {
                 uint32_t cur = (uint32_t)(k[0] >> (kRadixBits * p)) & (kRadix - 1);
                 uint32_t run = 1;
 #pragma unroll
                 for (int j = 1; j < kHistItems; j++) {
                     if (j < cnt) {
-                        uint32_t d = (uint32_t)(k[j] >> (kRadixBits * p)) & (kRadix - 1);
+                        const uint32_t d = (uint32_t)(k[j] >> (kRadixBits * p)) & (kRadix - 1);
                         if (d == cur) {
                             run++;
                         } else {
@@ -159,8 +164,8 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const uint64_t 
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kKeyPasses * kRadix; i += kHistThreads) {
-        uint32_t c = s_hist[i];
+    for (int i = threadIdx.x; i < kPasses * kRadix; i += kHistThreads) {
+        const uint32_t c = s_hist[i];
         if (c)
             atomicAdd(&hist[i], c);
     }
@@ -168,14 +173,13 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const uint64_t 
 
 // One onesweep pass over digit `shift`.  FIRST: payload = input position.
 // LAST: keys are not written.
-template <bool FIRST, bool LAST>
+template <typename KeyT, bool FIRST, bool LAST>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
-    const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
-    uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int64_t n, int shift,
-    const uint32_t *__restrict__ hist_pass, uint64_t *__restrict__ lookback,
-    uint32_t *__restrict__ tile_counter, uint32_t epoch)
+    const KeyT *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, KeyT *__restrict__ keys_out,
+    uint32_t *__restrict__ vals_out, int64_t n, int shift, const uint32_t *__restrict__ hist_pass,
+    uint64_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter, uint32_t epoch)
 {
-    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ KeyT s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
     __shared__ uint32_t s_off[kRadix];    // global base minus tile-local start, per digit
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         (&s_wcnt[0][0])[i] = 0;
     {
         uint32_t tot;
-        uint32_t b = block_exclusive_scan(hist_pass[tid], s_wave, &tot);
+        const uint32_t b = block_exclusive_scan(hist_pass[tid], s_wave, &tot);
         s_dbase[tid] = b;
     }
     __syncthreads();
@@ -202,14 +206,14 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const int64_t tile_base = (int64_t)tile * kSortTile;
     const int64_t wave_base = tile_base + (int64_t)wave * (kSortItems * 64);
 
-    uint64_t key[kSortItems];
+    KeyT key[kSortItems];
     uint32_t val[kSortItems];
     uint32_t rank[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         const bool valid = idx < n;
-        key[j] = valid ? keys_in[idx] : ~0ull;
+        key[j] = valid ? keys_in[idx] : (KeyT)~(KeyT)0;
         if (FIRST)
             val[j] = (uint32_t)idx;
         else
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         uint32_t total = 0;
 #pragma unroll
         for (int w = 0; w < kSortWaves; w++) {
-            uint32_t c = s_wcnt[w][d];
+            const uint32_t c = s_wcnt[w][d];
             s_wcnt[w][d] = total;
             total += c;
         }
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const int count = remain < kSortTile ? (int)remain : kSortTile;
 #pragma unroll 4
     for (int j = tid; j < count; j += kSortThreads) {
-        const uint64_t k = s_keys[j];
+        const KeyT k = s_keys[j];
         const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
         const uint32_t dest = s_off[d] + (uint32_t)j;
         if (!LAST)
@@ -306,17 +310,73 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     }
 }
 
-template <bool FIRST, bool LAST>
-hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass,
-                       hipStream_t stream)
+template <typename KeyT, bool FIRST, bool LAST>
+hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, hipStream_t stream)
 {
     const int64_t tiles = sort_tiles(n);
     const uint32_t epoch = w.epoch++;
-    hipLaunchKernelGGL((k_onesweep<FIRST, LAST>), dim3((unsigned)tiles), dim3(kSortThreads), 0, stream,
-                       w.keys[src], FIRST ? nullptr : w.vals[src], LAST ? nullptr : w.keys[src ^ 1],
-                       vals_out, n, pass * kRadixBits, w.hist + pass * kRadix, w.lookback,
-                       w.counters + pass, epoch);
+    hipLaunchKernelGGL((k_onesweep<KeyT, FIRST, LAST>), dim3((unsigned)tiles), dim3(kSortThreads), 0, stream,
+                       reinterpret_cast<const KeyT *>(w.keys[src]), FIRST ? nullptr : w.vals[src],
+                       LAST ? nullptr : reinterpret_cast<KeyT *>(w.keys[src ^ 1]), vals_out, n,
+                       pass * kRadixBits, w.hist + pass * kRadix, w.lookback, w.counters + pass, epoch);
     return hipGetLastError();
+}
+
+template <typename KeyT>
+hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
+{
+    constexpr int kPasses = (int)sizeof(KeyT);
+    hipError_t e;
+    // The epoch lives in 30 bits of the lookback word; wrap by clearing (a
+    // fill kernel, not a memset node, so that the sort stays capturable).
+    if (w.epoch + kPasses >= (1u << 30) || w.epoch == 0) {
+        const int64_t words = sort_tiles(n) * kRadix;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, w.lookback,
+                           words, (uint64_t)0);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        w.epoch = 1;
+    }
+    if (prof)
+        prof->begin(K_SORT_ZERO);
+    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kPasses * kRadix, w.counters, kPasses);
+    if (prof)
+        prof->end(K_SORT_ZERO);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    {
+        const int64_t per_block = (int64_t)kHistThreads * kHistItems;
+        int64_t blocks = (n + per_block - 1) / per_block;
+        if (blocks > 2048)
+            blocks = 2048;
+        if (prof)
+            prof->begin(K_SORT_HIST);
+        hipLaunchKernelGGL(k_sort_histogram<KeyT>, dim3((unsigned)blocks), dim3(kHistThreads), 0, stream,
+                           reinterpret_cast<const KeyT *>(w.keys[0]), n, w.hist);
+        if (prof)
+            prof->end(K_SORT_HIST);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    int src = 0;
+    for (int p = 0; p < kPasses; p++) {
+        const bool first = p == 0, last = p == kPasses - 1;
+        uint32_t *vout = last ? perm_out : w.vals[src ^ 1];
+        if (prof)
+            prof->begin(K_SORT_PASS);
+        if (first)
+            e = launch_pass<KeyT, true, false>(w, src, vout, n, p, stream);
+        else if (last)
+            e = launch_pass<KeyT, false, true>(w, src, vout, n, p, stream);
+        else
+            e = launch_pass<KeyT, false, false>(w, src, vout, n, p, stream);
+        if (prof)
+            prof->end(K_SORT_PASS);
+        if (e != hipSuccess)
+            return e;
+        src ^= 1;
+    }
+    return hipSuccess;
 }
 
 } // namespace
@@ -325,72 +385,26 @@ hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStr
 {
     if (n <= 0)
         return hipSuccess;
-    const int threads = 256;
-    hipLaunchKernelGGL(k_radius_keys, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads), 0,
-                       stream, r, keys, n);
+    hipLaunchKernelGGL(k_radius_keys<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, r, keys, n);
     return hipGetLastError();
 }
 
-hipError_t radix_sort_perm(SortWork &w, int64_t n, const uint32_t *tie_ids, uint32_t *perm_out,
-                           hipStream_t stream, Profiler *prof)
+hipError_t launch_radius_keys(const float *r, uint32_t *keys, int64_t n, hipStream_t stream)
 {
-    (void)tie_ids;
     if (n <= 0)
         return hipSuccess;
-    hipError_t e;
-    // The epoch lives in 30 bits of the lookback word; wrap by clearing
-    // (a fill kernel, not a memset node, so that the sort stays capturable).
-    if (w.epoch + kKeyPasses >= (1u << 30) || w.epoch == 0) {
-        const int64_t words = sort_tiles(n) * kRadix;
-        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream,
-                           w.lookback, words, (uint64_t)0);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        w.epoch = 1;
-    }
-    if (prof)
-        prof->begin(K_SORT_ZERO);
-    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kKeyPasses * kRadix,
-                       w.counters, kKeyPasses);
-    if (prof)
-        prof->end(K_SORT_ZERO);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
+    hipLaunchKernelGGL(k_radius_keys<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, r, keys, n);
+    return hipGetLastError();
+}
 
-    {
-        const int64_t per_block = (int64_t)kHistThreads * kHistItems;
-        int64_t blocks = (n + per_block - 1) / per_block;
-        if (blocks > 2048)
-            blocks = 2048;
-        if (prof)
-            prof->begin(K_SORT_HIST);
-        hipLaunchKernelGGL(k_sort_histogram, dim3((unsigned)blocks), dim3(kHistThreads), 0, stream,
-                           w.keys[0], n, w.hist);
-        if (prof)
-            prof->end(K_SORT_HIST);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-    }
-
-    int src = 0;
-    for (int p = 0; p < kKeyPasses; p++) {
-        const bool first = p == 0, last = p == kKeyPasses - 1;
-        uint32_t *vout = last ? perm_out : w.vals[src ^ 1];
-        if (prof)
-            prof->begin(K_SORT_PASS);
-        if (first)
-            e = launch_pass<true, false>(w, src, vout, n, p, stream);
-        else if (last)
-            e = launch_pass<false, true>(w, src, vout, n, p, stream);
-        else
-            e = launch_pass<false, false>(w, src, vout, n, p, stream);
-        if (prof)
-            prof->end(K_SORT_PASS);
-        if (e != hipSuccess)
-            return e;
-        src ^= 1;
-    }
-    return hipSuccess;
+hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_out, hipStream_t stream,
+                           Profiler *prof)
+{
+    if (n <= 0)
+        return hipSuccess;
+    if (key_bits == 32)
+        return sort_typed<uint32_t>(w, n, perm_out, stream, prof);
+    return sort_typed<uint64_t>(w, n, perm_out, stream, prof);
 }
 
 } // namespace ns
