@@ -60,12 +60,13 @@ __device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r,
     return r > 1.0e-30 && r < r_crit;
 }
 
+// 1/x for the fast form: v_rcp_f64 (about 24 correct bits on gfx950) and one
+// Newton step, within 11 ulp of the correctly rounded reciprocal (measured
+// over 4M values spanning 2^-40..2^40); a second step would make it exact.
 __device__ inline double fast_rcp(double x)
 {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, y, 1.0);
     return fma(y, e, y);
 }
 
@@ -266,10 +267,22 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
             leap_output(a.out_type, prev_r, prev_v, prev_r, prev_v, r_out, v_out);
             return (k - 1) | NS_LVL_NOCONV;
         }
+        // Iterations [2^k, 2^(k+1)) belong to lanes k..kmax, whose passes all
+        // run through them; the lane mask is set once per level, the last
+        // iteration closes lane k's pass with its half kick.  Level 0 also
+        // carries the one-iteration coarse pass of lane 63.
         const int64_t target = (int64_t)2 << k;
-        for (; t < target; t++) {
-            if (active && t < pairs)
-                drift_kick<FAST>(ph, ms, (t < pairs - 1) ? ms : hk, r, v);
+        if (k == 0) {
+            for (; t < target; t++)
+                if (active && t < pairs)
+                    drift_kick<FAST>(ph, ms, (t < pairs - 1) ? ms : hk, r, v);
+        } else {
+            if (lane >= k && lane <= kmax) {
+                for (int64_t it = t; it < target - 1; it++)
+                    drift_kick<FAST>(ph, ms, ms, r, v);
+                drift_kick<FAST>(ph, ms, lane == k ? hk : ms, r, v);
+            }
+            t = target;
         }
         const int src_c = (k == 0) ? 63 : k - 1;
         const double rc = __shfl(r, src_c, 64), vc = __shfl(v, src_c, 64);
@@ -301,8 +314,8 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
     const double dtau = h_tau / (double)sub;
     double rho = rho_in, v = v_in, t = t_in;
     double f = force_lc<FAST>(ph, rho);
-    for (int64_t ss = 0; ss < sub; ss++) {
-        if constexpr (!FAST) {
+    if constexpr (!FAST) {
+        for (int64_t ss = 0; ss < sub; ss++) {
             const double v_half = v + 0.5 * dtau * f;
             const double rho_next = rho + dtau * (0.5 * rho * v_half);
             const double f2 = force_lc<false>(ph, rho_next);
@@ -313,17 +326,27 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
             rho = rho_next;
             v = v_next;
             f = f2;
-        } else {
-            const double hd = 0.5 * dtau;
-            const double v_half = fma(hd, f, v);
-            const double rho_next = fma(dtau, 0.5 * rho * v_half, rho);
+        }
+    } else if (sub > 0) {
+        // fast form: the closing half-kick of one micro-step and the opening
+        // half-kick of the next merge into one kick of dtau, and hd*rho
+        // leaves the dependency chain
+        const double hd = 0.5 * dtau;
+        double vh = fma(hd, f, v);
+        for (int64_t ss = 0; ss < sub - 1; ss++) {
+            const double rho_next = fma(hd * rho, vh, rho);
             const double f2 = force_lc<true>(ph, rho_next);
             const double rho_mid = 0.5 * (rho + rho_next);
             t = fma(dtau, rho_mid * rho_mid, t);
-            v = fma(hd, f2, v_half);
+            vh = fma(dtau, f2, vh);
             rho = rho_next;
-            f = f2;
         }
+        const double rho_next = fma(hd * rho, vh, rho);
+        const double f2 = force_lc<true>(ph, rho_next);
+        const double rho_mid = 0.5 * (rho + rho_next);
+        t = fma(dtau, rho_mid * rho_mid, t);
+        v = fma(hd, f2, vh);
+        rho = rho_next;
     }
     rho_out = rho;
     v_out = v;
@@ -437,48 +460,76 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
         t_out = t_in;
         return NS_LVL_NOCONV | NS_LVL_MASK;
     }
-    int64_t it = 0;
-    auto advance_to = [&](int64_t target) {
-        for (; it < target; it++) {
-            if (!(active && it < sub))
-                continue;
-            if constexpr (!FAST) {
-                const double v_half = v + 0.5 * dtau * f;
-                const double rho_next = rho + dtau * (0.5 * rho * v_half);
-                const double f2 = force_lc<false>(ph, rho_next);
-                const double v_next = v_half + 0.5 * dtau * f2;
-                const double rho_mid = 0.5 * (rho + rho_next);
-                const double dt_phys = dtau * (rho_mid * rho_mid);
-                t += dt_phys;
-                rho = rho_next;
-                v = v_next;
-                f = f2;
-            } else {
-                const double hd = 0.5 * dtau;
-                const double v_half = fma(hd, f, v);
-                const double rho_next = fma(dtau, 0.5 * rho * v_half, rho);
-                const double f2 = force_lc<true>(ph, rho_next);
-                const double rho_mid = 0.5 * (rho + rho_next);
-                t = fma(dtau, rho_mid * rho_mid, t);
-                v = fma(hd, f2, v_half);
-                rho = rho_next;
-                f = f2;
-            }
+    // Fast form: vh is the half-kicked velocity (the closing half kick of one
+    // micro-step and the opening one of the next merge into one kick of
+    // dtau, as in micro_lc<true>); vh_prev is its value before the last
+    // micro-step, from which the pass's final velocity follows.
+    const double hd = 0.5 * dtau;
+    double vh = FAST ? fma(hd, f, v) : 0.0, vh_prev = 0.0;
+    auto micro_step = [&]() {
+        if constexpr (!FAST) {
+            const double v_half = v + 0.5 * dtau * f;
+            const double rho_next = rho + dtau * (0.5 * rho * v_half);
+            const double f2 = force_lc<false>(ph, rho_next);
+            const double v_next = v_half + 0.5 * dtau * f2;
+            const double rho_mid = 0.5 * (rho + rho_next);
+            const double dt_phys = dtau * (rho_mid * rho_mid);
+            t += dt_phys;
+            rho = rho_next;
+            v = v_next;
+            f = f2;
+        } else {
+            const double rho_next = fma(hd * rho, vh, rho);
+            const double f2 = force_lc<true>(ph, rho_next);
+            const double rho_mid = 0.5 * (rho + rho_next);
+            t = fma(dtau, rho_mid * rho_mid, t);
+            vh_prev = vh;
+            vh = fma(dtau, f2, vh);
+            rho = rho_next;
+            f = f2;
         }
+    };
+    // velocity at the end of the micro-steps this lane has run
+    auto v_now = [&]() -> double {
+        if constexpr (FAST)
+            return fma(hd, f, vh_prev);
+        else
+            return v;
+    };
+    // Iterations [4*2^(k-1)+1, 4*2^k+1) belong to lanes k..kmax+1, whose
+    // passes all run through them, so the lane mask is set once per level.
+    // Level 0 (iterations [0, 5)) also carries lane 63's 3-step coarse pass.
+    int64_t it = 0;
+    auto run_level = [&](int k) {
+        const int64_t end = 4 * ((int64_t)1 << k) + 1;
+        if (k == 0) {
+            if (active)
+                for (int s = 0; s < 3; s++)
+                    micro_step();
+            if (active && !coarse_lane)
+                for (int s = 3; s < 5; s++)
+                    micro_step();
+        } else if (lane >= k && lane <= kmax + 1) {
+            for (int64_t s = it; s < end; s++)
+                micro_step();
+        }
+        it = end;
     };
     for (int k = 0; k <= kmax; k++) {
         const int64_t Nk = (int64_t)1 << k;
         const int src_c = (k == 0) ? 63 : k - 1;
         if (!bud.take(4 * Nk + 2)) {
             // coarse of level k: the previous fine pass (or lane 63)
+            const double vn = v_now();
             rho_out = __shfl(rho, src_c, 64);
-            v_out = __shfl(v, src_c, 64);
+            v_out = __shfl(vn, src_c, 64);
             t_out = __shfl(t, src_c, 64);
             return (k == 0 ? 0 : level_of(Nk >> 1)) | NS_LVL_NOCONV;
         }
-        advance_to(4 * Nk + 1);
-        const double rhoC = __shfl(rho, src_c, 64), vC = __shfl(v, src_c, 64), tC = __shfl(t, src_c, 64);
-        const double rhoF = __shfl(rho, k, 64), vF = __shfl(v, k, 64), tF = __shfl(t, k, 64);
+        run_level(k);
+        const double vn = v_now();
+        const double rhoC = __shfl(rho, src_c, 64), vC = __shfl(vn, src_c, 64), tC = __shfl(t, src_c, 64);
+        const double rhoF = __shfl(rho, k, 64), vF = __shfl(vn, k, 64), tF = __shfl(t, k, 64);
         const double rF = rhoF * rhoF;
         const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
         const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
@@ -490,14 +541,16 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     // not converged: the fine pass at the overflowed N = 2^(kmax+1)
     const int64_t Nf = (int64_t)2 << kmax;
     if (!bud.take(4 * Nf + 2)) {
+        const double vn = v_now();
         rho_out = __shfl(rho, kmax, 64);
-        v_out = __shfl(v, kmax, 64);
+        v_out = __shfl(vn, kmax, 64);
         t_out = __shfl(t, kmax, 64);
         return level_of(Nf >> 1) | NS_LVL_NOCONV;
     }
-    advance_to(4 * Nf + 1);
+    run_level(kmax + 1);
+    const double vn = v_now();
     rho_out = __shfl(rho, kmax + 1, 64);
-    v_out = __shfl(v, kmax + 1, 64);
+    v_out = __shfl(vn, kmax + 1, 64);
     t_out = __shfl(t, kmax + 1, 64);
     return level_of(Nf) | NS_LVL_NOCONV;
 }
