@@ -83,6 +83,8 @@ enum {
                                  bit-identical to 0;
                               2 binned with the fast arithmetic form (one
                                  reciprocal per force, fused multiply-adds)   */
+    NS_OPT_SORT_TILE = 6,  /* onesweep tile: (threads << 16) | keys per thread,
+                              one of 256x16 (default), 256x24, 512x12, 512x16 */
     NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
                               float32 r, v_r, ell in HBM and 32-bit radius
                               keys (4 sort passes), float64 arithmetic in the

@@ -487,6 +487,16 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
             return set_error(NS_ERR_ARG, "NS_OPT_BUDGET must be >= 2 force evaluations");
         c->budget = value;
         return NS_OK;
+    case NS_OPT_SORT_TILE: {
+        ns::SortConfig cfg;
+        cfg.threads = (int)(value >> 16);
+        cfg.items = (int)(value & 0xFFFF);
+        if (!ns::sort_config_supported(cfg))
+            return set_error(NS_ERR_ARG, "NS_OPT_SORT_TILE: (threads << 16 | items) with threads x items in "
+                                         "{256x16, 256x24, 512x12, 512x16}");
+        c->sw.config = cfg;
+        return NS_OK;
+    }
     case NS_OPT_PRECISION: {
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_PRECISION takes 0 (float64 state) or 1 (float32 state)");

@@ -66,9 +66,18 @@ struct KeyOf<float> {
     using type = uint32_t;
 };
 
+// Onesweep tile shape: workgroup size and keys per thread (tile = product,
+// at least kSortTile so that the lookback buffer's tile count bounds it).
+struct SortConfig {
+    int threads = kSortThreads;
+    int items = kSortItems;
+};
+bool sort_config_supported(SortConfig c);
+
 // Device workspace of one radix sort.  Key buffers hold 64-bit keys, or
 // 32-bit keys in their first half.
 struct SortWork {
+    SortConfig config;
     uint64_t *keys[2];   // ping-pong key buffers (keys[0] holds the input)
     uint32_t *vals[2];   // ping-pong payload buffers
     uint32_t *hist;      // kKeyPasses * kRadix digit counts

@@ -9,14 +9,16 @@
 // or 32-bit keys (float32 radii, 4 passes):
 //   * one upfront pass reads every key once and builds all the 8-bit digit
 //     histograms (LDS-privatised, run-length-combined counters);
-//   * one "onesweep" kernel per pass: a workgroup takes a 4096-key tile by
-//     atomic ticket, ranks its keys stably per digit with wave ballots (a
-//     64-lane match of the 8 digit bits), publishes its digit counts and
-//     resolves its global digit offsets by decoupled lookback over earlier
-//     tiles, stages the tile in LDS in digit order and writes it out so that
-//     each digit run leaves as one coalesced segment;
+//   * one "onesweep" kernel per pass: a workgroup takes a tile of
+//     THREADS * ITEMS keys by atomic ticket, ranks its keys stably per digit
+//     with wave ballots (a 64-lane match of the 8 digit bits), publishes its
+//     digit counts and resolves its global digit offsets by decoupled
+//     lookback over earlier tiles, then stages keys and payload -- one after
+//     the other, through one LDS buffer -- in digit order and writes them out
+//     so that each digit run leaves as one coalesced segment;
 //   * pass 0 synthesises the payload (input positions) instead of reading
 //     it, and the last pass writes only the payload (the permutation).
+// Block size and keys per thread are a SortConfig (the autotuner's choice).
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
@@ -54,29 +56,27 @@ __device__ inline uint64_t lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Exclusive scan of one value per thread over a 256-thread workgroup.
-__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_wave, uint32_t *total)
+// Exclusive scan of x over the first 256 threads (one per digit; the first
+// four waves).  Every thread of the workgroup must call it (it synchronises);
+// threads past 256 get an unspecified value.
+__device__ inline uint32_t digit_exclusive_scan(uint32_t x, uint32_t *s_wave)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = x;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(incl, off, 64);
+        const uint32_t y = __shfl_up(incl, off, 64);
         if (lane >= off)
             incl += y;
     }
-    if (lane == 63)
+    if (lane == 63 && wave < 4)
         s_wave[wave] = incl;
     __syncthreads();
-    uint32_t before = 0, all = 0;
+    uint32_t before = 0;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; w++) {
-        uint32_t c = s_wave[w];
-        before += (w < wave) ? c : 0u;
-        all += c;
-    }
+    for (int w = 0; w < 4; w++)
+        before += (w < wave) ? s_wave[w] : 0u;
     __syncthreads();
-    *total = all;
     return before + incl - x;
 }
 
@@ -173,19 +173,22 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__r
 
 // One onesweep pass over digit `shift`.  FIRST: payload = input position.
 // LAST: keys are not written.
-template <typename KeyT, bool FIRST, bool LAST>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
+__global__ __launch_bounds__(THREADS) void k_onesweep(
     const KeyT *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, KeyT *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, const uint32_t *__restrict__ hist_pass,
     uint64_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter, uint32_t epoch)
 {
-    __shared__ KeyT s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
-    __shared__ uint32_t s_off[kRadix];    // global base minus tile-local start, per digit
-    __shared__ uint32_t s_lstart[kRadix]; // tile-local digit start
-    __shared__ uint32_t s_dbase[kRadix];  // global exclusive digit base
-    __shared__ uint32_t s_wave[kSortWaves];
+    constexpr int WAVES = THREADS / 64;
+    constexpr int TILE = THREADS * ITEMS;
+    static_assert(THREADS >= kRadix, "one thread per digit");
+    __shared__ KeyT s_stage[TILE];          // keys in digit order, then the payload
+    __shared__ uint8_t s_dig[TILE];         // digit of each staged slot
+    __shared__ uint32_t s_wcnt[WAVES][kRadix];
+    __shared__ uint32_t s_off[kRadix];      // global base minus tile-local start, per digit
+    __shared__ uint32_t s_lstart[kRadix];   // tile-local digit start
+    __shared__ uint32_t s_dbase[kRadix];    // global exclusive digit base
+    __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x;
@@ -193,24 +196,24 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 
     if (tid == 0)
         s_tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads)
+    for (int i = tid; i < WAVES * kRadix; i += THREADS)
         (&s_wcnt[0][0])[i] = 0;
     {
-        uint32_t tot;
-        const uint32_t b = block_exclusive_scan(hist_pass[tid], s_wave, &tot);
-        s_dbase[tid] = b;
+        const uint32_t b = digit_exclusive_scan(tid < kRadix ? hist_pass[tid] : 0u, s_wave);
+        if (tid < kRadix)
+            s_dbase[tid] = b;
     }
     __syncthreads();
 
     const uint32_t tile = s_tile;
-    const int64_t tile_base = (int64_t)tile * kSortTile;
-    const int64_t wave_base = tile_base + (int64_t)wave * (kSortItems * 64);
+    const int64_t tile_base = (int64_t)tile * TILE;
+    const int64_t wave_base = tile_base + (int64_t)wave * (ITEMS * 64);
 
-    KeyT key[kSortItems];
-    uint32_t val[kSortItems];
-    uint32_t rank[kSortItems];
+    KeyT key[ITEMS];
+    uint32_t val[ITEMS];
+    uint32_t pos[ITEMS];
 #pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         const bool valid = idx < n;
         key[j] = valid ? keys_in[idx] : (KeyT)~(KeyT)0;
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     // (round j covers indices wave_base + 64 j + lane).
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
+    for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         const bool valid = idx < n;
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
@@ -240,23 +243,22 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         if (valid && lane == leader)
             old = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
         old = __shfl(old, leader, 64);
-        rank[j] = old + (uint32_t)__popcll(peers & lt);
+        pos[j] = old + (uint32_t)__popcll(peers & lt);  // rank within the wave, for now
     }
     __syncthreads();
 
-    // Per digit (thread tid owns digit tid): wave-exclusive offsets, tile
-    // total, decoupled lookback for the tile's global offset.
-    {
+    // Per digit (thread tid < 256 owns digit tid): wave-exclusive offsets,
+    // tile total, decoupled lookback for the tile's global offset.
+    uint32_t total = 0, excl = 0;
+    if (tid < kRadix) {
         const int d = tid;
-        uint32_t total = 0;
 #pragma unroll
-        for (int w = 0; w < kSortWaves; w++) {
+        for (int w = 0; w < WAVES; w++) {
             const uint32_t c = s_wcnt[w][d];
             s_wcnt[w][d] = total;
             total += c;
         }
         uint64_t *mine = lookback + (size_t)tile * kRadix + d;
-        uint32_t excl = 0;
         if (tile == 0) {
             lb_store(mine, lb_pack(epoch, kStatusPrefix, total));
         } else {
@@ -277,49 +279,90 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
             }
             lb_store(mine, lb_pack(epoch, kStatusPrefix, excl + total));
         }
-        uint32_t tot;
-        const uint32_t lstart = block_exclusive_scan(total, s_wave, &tot);
-        s_lstart[d] = lstart;
-        s_off[d] = s_dbase[d] + excl - lstart;
     }
-    __syncthreads();
-
-    // Stage the tile in digit order.
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        if (idx < n) {
-            const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
-            const uint32_t pos = s_lstart[d] + s_wcnt[wave][d] + rank[j];
-            s_keys[pos] = key[j];
-            s_vals[pos] = val[j];
+    {
+        const uint32_t lstart = digit_exclusive_scan(total, s_wave);
+        if (tid < kRadix) {
+            s_lstart[tid] = lstart;
+            s_off[tid] = s_dbase[tid] + excl - lstart;
         }
     }
     __syncthreads();
 
-    const int64_t remain = n - tile_base;
-    const int count = remain < kSortTile ? (int)remain : kSortTile;
-#pragma unroll 4
-    for (int j = tid; j < count; j += kSortThreads) {
-        const KeyT k = s_keys[j];
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
-        const uint32_t dest = s_off[d] + (uint32_t)j;
-        if (!LAST)
-            keys_out[dest] = k;
-        vals_out[dest] = s_vals[j];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+        pos[j] += s_lstart[d] + s_wcnt[wave][d];  // slot in the tile, digit order
     }
+    const int64_t remain = n - tile_base;
+    const int count = remain < TILE ? (int)remain : TILE;
+
+    if (!LAST) {
+        // keys through LDS in digit order, then out in coalesced digit runs
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+            if (idx < n) {
+                s_stage[pos[j]] = key[j];
+                s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < count; j += THREADS)
+            keys_out[s_off[s_dig[j]] + (uint32_t)j] = s_stage[j];
+        __syncthreads();
+    }
+    uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_stage);
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+        if (idx < n) {
+            s_pay[pos[j]] = val[j];
+            if (LAST)
+                s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < count; j += THREADS)
+        vals_out[s_off[s_dig[j]] + (uint32_t)j] = s_pay[j];
 }
 
-template <typename KeyT, bool FIRST, bool LAST>
+template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
 hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, hipStream_t stream)
 {
-    const int64_t tiles = sort_tiles(n);
+    constexpr int TILE = THREADS * ITEMS;
+    const int64_t tiles = (n + TILE - 1) / TILE;
     const uint32_t epoch = w.epoch++;
-    hipLaunchKernelGGL((k_onesweep<KeyT, FIRST, LAST>), dim3((unsigned)tiles), dim3(kSortThreads), 0, stream,
-                       reinterpret_cast<const KeyT *>(w.keys[src]), FIRST ? nullptr : w.vals[src],
+    hipLaunchKernelGGL((k_onesweep<KeyT, THREADS, ITEMS, FIRST, LAST>), dim3((unsigned)tiles), dim3(THREADS), 0,
+                       stream, reinterpret_cast<const KeyT *>(w.keys[src]), FIRST ? nullptr : w.vals[src],
                        LAST ? nullptr : reinterpret_cast<KeyT *>(w.keys[src ^ 1]), vals_out, n,
                        pass * kRadixBits, w.hist + pass * kRadix, w.lookback, w.counters + pass, epoch);
     return hipGetLastError();
+}
+
+template <typename KeyT, int THREADS, int ITEMS>
+hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
+                           hipStream_t stream)
+{
+    if (pass == 0)
+        return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream);
+    if (pass == passes - 1)
+        return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream);
+    return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream);
+}
+
+template <typename KeyT>
+hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
+                           hipStream_t stream)
+{
+    const SortConfig c = w.config;
+    if (c.threads == 512 && c.items == 16)
+        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream);
+    if (c.threads == 256 && c.items == 24)
+        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream);
+    if (c.threads == 512 && c.items == 12)
+        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream);
+    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream);
 }
 
 template <typename KeyT>
@@ -360,16 +403,10 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
     }
     int src = 0;
     for (int p = 0; p < kPasses; p++) {
-        const bool first = p == 0, last = p == kPasses - 1;
-        uint32_t *vout = last ? perm_out : w.vals[src ^ 1];
+        uint32_t *vout = (p == kPasses - 1) ? perm_out : w.vals[src ^ 1];
         if (prof)
             prof->begin(K_SORT_PASS);
-        if (first)
-            e = launch_pass<KeyT, true, false>(w, src, vout, n, p, stream);
-        else if (last)
-            e = launch_pass<KeyT, false, true>(w, src, vout, n, p, stream);
-        else
-            e = launch_pass<KeyT, false, false>(w, src, vout, n, p, stream);
+        e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream);
         if (prof)
             prof->end(K_SORT_PASS);
         if (e != hipSuccess)
@@ -380,6 +417,12 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
 }
 
 } // namespace
+
+bool sort_config_supported(SortConfig c)
+{
+    return (c.threads == 256 && (c.items == 16 || c.items == 24)) ||
+           (c.threads == 512 && (c.items == 12 || c.items == 16));
+}
 
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream)
 {
