@@ -30,6 +30,9 @@ namespace {
 // Lookback word: high 32 bits = epoch << 2 | status, low 32 bits = count.
 constexpr uint32_t kStatusAggregate = 1;
 constexpr uint32_t kStatusPrefix = 2;
+// Predecessor tiles read per lookback round (each read is a ~0.4 us coherent
+// load; walks of 30-80 tiles are typical with ~1300 tiles in flight).
+constexpr int kLookbackWindow = 16;
 
 __device__ inline uint64_t lb_pack(uint32_t epoch, uint32_t status, uint32_t count)
 {
@@ -224,8 +227,14 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     }
 
     // Stable per-wave ranking: items of one wave are visited in index order
-    // (round j covers indices wave_base + 64 j + lane).
+    // (round j covers indices wave_base + 64 j + lane).  Three batched phases
+    // -- all ballots, then all LDS counter updates (one per digit group and
+    // round, issued back to back; LDS applies one wave's operations in
+    // order), then all broadcasts -- so the rounds' LDS round trips overlap.
     const uint64_t lt = lanemask_lt();
+    uint32_t below[ITEMS];  // same-digit lanes below this one in the round
+    uint32_t group[ITEMS];  // same-digit lanes in the round
+    uint32_t leader[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
@@ -238,13 +247,20 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             const uint64_t bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
         }
-        const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
-        uint32_t old = 0;
-        if (valid && lane == leader)
-            old = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
-        old = __shfl(old, leader, 64);
-        pos[j] = old + (uint32_t)__popcll(peers & lt);  // rank within the wave, for now
+        below[j] = (uint32_t)__popcll(peers & lt);
+        group[j] = valid ? (uint32_t)__popcll(peers) : 0u;
+        leader[j] = valid ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : (uint32_t)lane;
     }
+    uint32_t old[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        old[j] = 0;
+        if (group[j] != 0 && (uint32_t)lane == leader[j])
+            old[j] = atomicAdd(&s_wcnt[wave][(uint32_t)(key[j] >> shift) & (kRadix - 1)], group[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++)
+        pos[j] = __shfl(old[j], (int)leader[j], 64) + below[j];  // rank within the wave, for now
     __syncthreads();
 
     // Per digit (thread tid < 256 owns digit tid): wave-exclusive offsets,
@@ -263,19 +279,35 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             lb_store(mine, lb_pack(epoch, kStatusPrefix, total));
         } else {
             lb_store(mine, lb_pack(epoch, kStatusAggregate, total));
+            // Windowed lookback: the words of the kLookbackWindow previous
+            // tiles are loaded together and consumed in order -- aggregates
+            // summed until an inclusive prefix is met; a tile that has not
+            // published yet ends the window and is read again.
             int64_t t = (int64_t)tile - 1;
             while (true) {
-                const uint64_t wv = lb_load(lookback + (size_t)t * kRadix + d);
-                const uint32_t hi = (uint32_t)(wv >> 32);
-                const uint32_t status = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
-                if (status == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+                uint64_t wv[kLookbackWindow];
+#pragma unroll
+                for (int k = 0; k < kLookbackWindow; k++)
+                    wv[k] = (t - k >= 0) ? lb_load(lookback + (size_t)(t - k) * kRadix + d) : 0ull;
+                int used = 0;
+                bool found = false;
+#pragma unroll
+                for (int k = 0; k < kLookbackWindow; k++) {
+                    if (found || used < k)
+                        continue;
+                    const uint32_t hi = (uint32_t)(wv[k] >> 32);
+                    const uint32_t status = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
+                    if (status == 0)
+                        continue;
+                    excl += (uint32_t)wv[k];
+                    used = k + 1;
+                    found = status == kStatusPrefix;
                 }
-                excl += (uint32_t)wv;
-                if (status == kStatusPrefix)
+                if (found)
                     break;
-                t--;
+                if (used == 0)
+                    __builtin_amdgcn_s_sleep(1);
+                t -= used;
             }
             lb_store(mine, lb_pack(epoch, kStatusPrefix, excl + total));
         }
