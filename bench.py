@@ -41,7 +41,7 @@ def bytes_per_particle(state_bytes: int) -> dict:
     last = (s + 4) + 4               # read key + payload; write the permutation
     return {
         "sort_histogram": s,                                     # read the key
-        "sort_onesweep": (first + (passes - 2) * middle + last) / passes,
+        "sort_scatter": (first + (passes - 2) * middle + last) / passes,
         # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
         "integrate": (4 + 3 * s + 4) + (3 * s + 4 + s + 1),
         "radius_keys": 2 * s,
@@ -225,7 +225,7 @@ def main() -> int:
         "config": {"workload": "plummer_n1.2e7_lowl5pct_dt1e-3", "n": n, "dt": p.dt, "form": args.form,
                    "state": args.precision, "lowl_frac": 0.05, "lowl_scale": 1e-3, "seed": args.seed},
         "roofline": roofline_for(chosen, n, dom),
-        "roofline_sort": roofline_for(chosen, n, "sort_onesweep"),
+        "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
         "dominant_kernel": dom,
         "kernels": chosen["kernels"],
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),

@@ -83,8 +83,12 @@ enum {
                                  bit-identical to 0;
                               2 binned with the fast arithmetic form (one
                                  reciprocal per force, fused multiply-adds)   */
-    NS_OPT_SORT_TILE = 6,  /* onesweep tile: (threads << 16) | keys per thread,
-                              one of 256x16 (default), 256x24, 512x12, 512x16 */
+    NS_OPT_SORT_TILE = 6,  /* radix tile: (threads << 16) | keys per thread,
+                              one of 256x16, 256x24, 512x12 (default), 512x16 */
+    NS_OPT_SORT_ALG = 7,   /* radix pass: 0 reduce-then-scan (per pass: tile
+                              digit counts, their scan, scatter; default),
+                              1 onesweep (one scatter kernel per pass with
+                              decoupled lookback).  Same result.             */
     NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
                               float32 r, v_r, ell in HBM and 32-bit radius
                               keys (4 sort passes), float64 arithmetic in the

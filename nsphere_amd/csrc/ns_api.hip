@@ -64,9 +64,9 @@ void clear_error() { g_err[0] = 0; }
 
 const char *kernel_name(int id)
 {
-    static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_onesweep",
+    static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
-                                         "sort_tiefix"};
+                                         "sort_tiefix", "sort_upsweep", "sort_scan"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -488,12 +488,20 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         c->budget = value;
         return NS_OK;
     case NS_OPT_SORT_TILE: {
-        ns::SortConfig cfg;
+        ns::SortConfig cfg = c->sw.config;
         cfg.threads = (int)(value >> 16);
         cfg.items = (int)(value & 0xFFFF);
         if (!ns::sort_config_supported(cfg))
             return set_error(NS_ERR_ARG, "NS_OPT_SORT_TILE: (threads << 16 | items) with threads x items in "
                                          "{256x16, 256x24, 512x12, 512x16}");
+        c->sw.config = cfg;
+        return NS_OK;
+    }
+    case NS_OPT_SORT_ALG: {
+        ns::SortConfig cfg = c->sw.config;
+        cfg.alg = (int)value;
+        if (value < 0 || value > 1 || !ns::sort_config_supported(cfg))
+            return set_error(NS_ERR_ARG, "NS_OPT_SORT_ALG takes 0 (reduce-then-scan) or 1 (onesweep)");
         c->sw.config = cfg;
         return NS_OK;
     }

@@ -66,11 +66,19 @@ struct KeyOf<float> {
     using type = uint32_t;
 };
 
-// Onesweep tile shape: workgroup size and keys per thread (tile = product,
-// at least kSortTile so that the lookback buffer's tile count bounds it).
+// Radix-pass algorithm: reduce-then-scan (per pass a count kernel, a scan of
+// the tile counts and a scatter kernel; no inter-workgroup waiting) or
+// onesweep (one scatter kernel per pass with decoupled lookback).
+constexpr int kSortReduceScan = 0;
+constexpr int kSortOnesweep = 1;
+
+// Sort configuration (the autotuner's choice): the pass algorithm and the
+// tile shape, workgroup size and keys per thread (tile = product, at least
+// kSortTile so that the lookback buffer's tile count bounds it).
 struct SortConfig {
-    int threads = kSortThreads;
-    int items = kSortItems;
+    int alg = kSortReduceScan;
+    int threads = 512;  // 512 x 12: fastest measured shape for both key widths
+    int items = 12;     // at N = 1.2e7 on MI355X, with either pass algorithm
 };
 bool sort_config_supported(SortConfig c);
 
@@ -80,8 +88,10 @@ struct SortWork {
     SortConfig config;
     uint64_t *keys[2];   // ping-pong key buffers (keys[0] holds the input)
     uint32_t *vals[2];   // ping-pong payload buffers
-    uint32_t *hist;      // kKeyPasses * kRadix digit counts
-    uint64_t *lookback;  // sort_tiles(n) * kRadix decoupled-lookback words
+    uint32_t *hist;      // kKeyPasses * kRadix digit counts (onesweep) or the
+                         // pass's digit totals (reduce-then-scan)
+    uint64_t *lookback;  // sort_tiles(n) * kRadix decoupled-lookback words, or
+                         // the reduce-then-scan tile counts (32-bit)
     uint32_t *counters;  // kKeyPasses tile tickets (+ spare)
     uint32_t epoch;      // lookback epoch of the next pass (host-owned)
 };
@@ -106,6 +116,8 @@ enum KernelId : int {
     K_INVMAP,
     K_ENERGY,
     K_TIEFIX,
+    K_SORT_UPSWEEP,
+    K_SORT_SCAN,
     K_COUNT
 };
 const char *kernel_name(int id);

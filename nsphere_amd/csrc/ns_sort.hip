@@ -6,19 +6,25 @@
 // that tie keep their input (previous-step) order -- the order quadsort gives.
 //
 // Design (one HIP stream, wave64), for 64-bit keys (float64 radii, 8 passes)
-// or 32-bit keys (float32 radii, 4 passes):
-//   * one upfront pass reads every key once and builds all the 8-bit digit
-//     histograms (LDS-privatised, run-length-combined counters);
-//   * one "onesweep" kernel per pass: a workgroup takes a tile of
-//     THREADS * ITEMS keys by atomic ticket, ranks its keys stably per digit
-//     with wave ballots (a 64-lane match of the 8 digit bits), publishes its
-//     digit counts and resolves its global digit offsets by decoupled
-//     lookback over earlier tiles, then stages keys and payload -- one after
-//     the other, through one LDS buffer -- in digit order and writes them out
-//     so that each digit run leaves as one coalesced segment;
-//   * pass 0 synthesises the payload (input positions) instead of reading
-//     it, and the last pass writes only the payload (the permutation).
-// Block size and keys per thread are a SortConfig (the autotuner's choice).
+// or 32-bit keys (float32 radii, 4 passes).  Every pass scatters tiles of
+// THREADS * ITEMS keys: a workgroup ranks its keys stably per digit with wave
+// ballots (a 64-lane match of the 8 digit bits), finds the tile's global
+// offset of every digit, then stages keys and payload -- one after the
+// other, through one LDS buffer -- in digit order and writes them out so that
+// each digit run leaves as one coalesced segment.  Pass 0 synthesises the
+// payload (input positions) instead of reading it; the last pass writes only
+// the payload (the permutation).  The tile offsets come from one of:
+//   * reduce-then-scan (default): per pass, a count kernel writes every
+//     tile's digit counts, one workgroup per digit scans them over the
+//     tiles, and the scatter kernel reads its offsets -- one extra read of
+//     the keys per pass, no workgroup ever waits for another;
+//   * onesweep: one upfront kernel builds all the digit histograms, and
+//     per pass a single scatter kernel takes tiles by atomic ticket,
+//     publishes its digit counts and resolves its offsets by decoupled
+//     lookback over earlier tiles.  At N ~ 1e7 about 1300 tiles are in
+//     flight at once and the lookback walks dominate (measured ~95 us per
+//     32-bit pass against ~60 us for the scatter of reduce-then-scan).
+// Algorithm, block size and keys per thread are a SortConfig.
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
@@ -30,9 +36,6 @@ namespace {
 // Lookback word: high 32 bits = epoch << 2 | status, low 32 bits = count.
 constexpr uint32_t kStatusAggregate = 1;
 constexpr uint32_t kStatusPrefix = 2;
-// Predecessor tiles read per lookback round (each read is a ~0.4 us coherent
-// load; walks of 30-80 tiles are typical with ~1300 tiles in flight).
-constexpr int kLookbackWindow = 16;
 
 __device__ inline uint64_t lb_pack(uint32_t epoch, uint32_t status, uint32_t count)
 {
@@ -57,6 +60,23 @@ __device__ inline uint64_t lanemask_lt()
 {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Lanes (within `valid`) whose 8-bit digit equals this lane's: one ballot per
+// digit bit, each folded in as peers &= (ballot ^ ~bit-mask) on both 32-bit
+// halves -- a single v_bitop3 per half on gfx950.
+__device__ inline uint64_t match_digit(uint32_t d, uint64_t valid)
+{
+    uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
+    const uint32_t nd = ~d;
+#pragma unroll
+    for (int b = 0; b < kRadixBits; b++) {
+        const uint32_t nm = (uint32_t)__builtin_amdgcn_sbfe((int)nd, b, 1);  // ~0 where bit b of d is 0
+        const uint64_t bb = __ballot(nm == 0);                                 // lanes whose bit b is 1
+        lo &= (uint32_t)bb ^ nm;
+        hi &= (uint32_t)(bb >> 32) ^ nm;
+    }
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Exclusive scan of x over the first 256 threads (one per digit; the first
@@ -174,13 +194,119 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__r
     }
 }
 
-// One onesweep pass over digit `shift`.  FIRST: payload = input position.
-// LAST: keys are not written.
-template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
-__global__ __launch_bounds__(THREADS) void k_onesweep(
+// Reduce-then-scan, step 1: the pass digit's count in every tile, written
+// digit-major (counts[d * tiles + tile]) so that step 2 scans rows.  A wave
+// whose 64 keys share one digit (common for the near-constant high bytes of
+// nearly sorted radii) adds them with one LDS atomic instead of 64 colliding
+// ones; other waves add per lane.
+template <typename KeyT, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict__ keys, int64_t n, int shift,
+                                                         uint32_t *__restrict__ counts, int64_t tiles)
+{
+    constexpr int WAVES = THREADS / 64;
+    __shared__ uint32_t s_cnt[WAVES][kRadix];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < WAVES * kRadix; i += THREADS)
+        (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+
+    const int64_t tile = blockIdx.x;
+    const int64_t wave_base = tile * (THREADS * ITEMS) + (int64_t)wave * (ITEMS * 64);
+    KeyT key[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+        key[j] = idx < n ? keys[idx] : (KeyT)0;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        // valid lanes are a prefix of the wave (lane 0 first)
+        const bool valid = wave_base + (int64_t)j * 64 + lane < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // whole wave: no divergence yet
+        if (__ballot(valid && d != d0) == 0) {
+            if (lane == 0 && valid)
+                atomicAdd(&s_cnt[wave][d0], nvalid);
+        } else if (valid) {
+            atomicAdd(&s_cnt[wave][d], 1u);
+        }
+    }
+    __syncthreads();
+    if (tid < kRadix) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++)
+            c += s_cnt[w][tid];
+        counts[(int64_t)tid * tiles + tile] = c;
+    }
+}
+
+// Reduce-then-scan, step 2: one workgroup per digit turns its row of tile
+// counts into exclusive tile offsets (in place) and writes the digit total.
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+
+__global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict__ counts, int64_t tiles,
+                                                           uint32_t *__restrict__ totals)
+{
+    constexpr int WAVES = kScanThreads / 64;
+    __shared__ uint32_t s_wave[WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t *row = counts + (int64_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < tiles; base += kScanThreads * kScanItems) {
+        const int64_t first = base + (int64_t)tid * kScanItems;
+        uint32_t x[kScanItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            x[k] = first + k < tiles ? row[first + k] : 0u;
+            sum += x[k];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off)
+                incl += y;
+        }
+        if (lane == 63)
+            s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+            const uint32_t t = s_wave[w];
+            before += w < wave ? t : 0u;
+            chunk += t;
+        }
+        __syncthreads();
+        uint32_t run = before + incl - sum;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            if (first + k < tiles)
+                row[first + k] = run;
+            run += x[k];
+        }
+        carry += chunk;
+    }
+    if (tid == 0)
+        totals[blockIdx.x] = carry;
+}
+
+// One scatter pass over digit `shift`.  FIRST: payload = input position.
+// LAST: keys are not written.  LOOKBACK (onesweep): tiles are taken by
+// ticket and each tile's global digit offsets come from decoupled lookback
+// over earlier tiles, with hist_pass the pass's digit histogram.  Otherwise
+// (reduce-then-scan) tile = blockIdx.x, the offsets are read from
+// tile_offsets[d * tiles + tile] and hist_pass holds the digit totals.
+template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST, bool LOOKBACK>
+__global__ __launch_bounds__(THREADS) void k_radix_scatter(
     const KeyT *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, KeyT *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, const uint32_t *__restrict__ hist_pass,
-    uint64_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter, uint32_t epoch)
+    uint64_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter, uint32_t epoch,
+    const uint32_t *__restrict__ tile_offsets, int64_t tiles)
 {
     constexpr int WAVES = THREADS / 64;
     constexpr int TILE = THREADS * ITEMS;
@@ -198,7 +324,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     const int lane = tid & 63, wave = tid >> 6;
 
     if (tid == 0)
-        s_tile = atomicAdd(tile_counter, 1u);
+        s_tile = LOOKBACK ? atomicAdd(tile_counter, 1u) : (uint32_t)blockIdx.x;
     for (int i = tid; i < WAVES * kRadix; i += THREADS)
         (&s_wcnt[0][0])[i] = 0;
     {
@@ -227,40 +353,21 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     }
 
     // Stable per-wave ranking: items of one wave are visited in index order
-    // (round j covers indices wave_base + 64 j + lane).  Three batched phases
-    // -- all ballots, then all LDS counter updates (one per digit group and
-    // round, issued back to back; LDS applies one wave's operations in
-    // order), then all broadcasts -- so the rounds' LDS round trips overlap.
+    // (round j covers indices wave_base + 64 j + lane).
     const uint64_t lt = lanemask_lt();
-    uint32_t below[ITEMS];  // same-digit lanes below this one in the round
-    uint32_t group[ITEMS];  // same-digit lanes in the round
-    uint32_t leader[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         const bool valid = idx < n;
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
-        below[j] = (uint32_t)__popcll(peers & lt);
-        group[j] = valid ? (uint32_t)__popcll(peers) : 0u;
-        leader[j] = valid ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : (uint32_t)lane;
+        const uint64_t peers = match_digit(d, __ballot(valid));
+        const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+        uint32_t old = 0;
+        if (valid && lane == leader)
+            old = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
+        old = __shfl(old, leader, 64);
+        pos[j] = old + (uint32_t)__popcll(peers & lt);  // rank within the wave, for now
     }
-    uint32_t old[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        old[j] = 0;
-        if (group[j] != 0 && (uint32_t)lane == leader[j])
-            old[j] = atomicAdd(&s_wcnt[wave][(uint32_t)(key[j] >> shift) & (kRadix - 1)], group[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++)
-        pos[j] = __shfl(old[j], (int)leader[j], 64) + below[j];  // rank within the wave, for now
     __syncthreads();
 
     // Per digit (thread tid < 256 owns digit tid): wave-exclusive offsets,
@@ -274,40 +381,26 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             s_wcnt[w][d] = total;
             total += c;
         }
-        uint64_t *mine = lookback + (size_t)tile * kRadix + d;
-        if (tile == 0) {
-            lb_store(mine, lb_pack(epoch, kStatusPrefix, total));
+        if (!LOOKBACK) {
+            excl = tile_offsets[(int64_t)d * tiles + tile];
+        } else if (tile == 0) {
+            lb_store(lookback + d, lb_pack(epoch, kStatusPrefix, total));
         } else {
+            uint64_t *mine = lookback + (size_t)tile * kRadix + d;
             lb_store(mine, lb_pack(epoch, kStatusAggregate, total));
-            // Windowed lookback: the words of the kLookbackWindow previous
-            // tiles are loaded together and consumed in order -- aggregates
-            // summed until an inclusive prefix is met; a tile that has not
-            // published yet ends the window and is read again.
             int64_t t = (int64_t)tile - 1;
             while (true) {
-                uint64_t wv[kLookbackWindow];
-#pragma unroll
-                for (int k = 0; k < kLookbackWindow; k++)
-                    wv[k] = (t - k >= 0) ? lb_load(lookback + (size_t)(t - k) * kRadix + d) : 0ull;
-                int used = 0;
-                bool found = false;
-#pragma unroll
-                for (int k = 0; k < kLookbackWindow; k++) {
-                    if (found || used < k)
-                        continue;
-                    const uint32_t hi = (uint32_t)(wv[k] >> 32);
-                    const uint32_t status = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
-                    if (status == 0)
-                        continue;
-                    excl += (uint32_t)wv[k];
-                    used = k + 1;
-                    found = status == kStatusPrefix;
-                }
-                if (found)
-                    break;
-                if (used == 0)
+                const uint64_t wv = lb_load(lookback + (size_t)t * kRadix + d);
+                const uint32_t hi = (uint32_t)(wv >> 32);
+                const uint32_t status = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
+                if (status == 0) {
                     __builtin_amdgcn_s_sleep(1);
-                t -= used;
+                    continue;
+                }
+                excl += (uint32_t)wv;
+                if (status == kStatusPrefix)
+                    break;
+                t--;
             }
             lb_store(mine, lb_pack(epoch, kStatusPrefix, excl + total));
         }
@@ -360,41 +453,79 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
 }
 
 template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
-hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, hipStream_t stream)
+hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, hipStream_t stream,
+                       Profiler *prof)
 {
     constexpr int TILE = THREADS * ITEMS;
     const int64_t tiles = (n + TILE - 1) / TILE;
-    const uint32_t epoch = w.epoch++;
-    hipLaunchKernelGGL((k_onesweep<KeyT, THREADS, ITEMS, FIRST, LAST>), dim3((unsigned)tiles), dim3(THREADS), 0,
-                       stream, reinterpret_cast<const KeyT *>(w.keys[src]), FIRST ? nullptr : w.vals[src],
-                       LAST ? nullptr : reinterpret_cast<KeyT *>(w.keys[src ^ 1]), vals_out, n,
-                       pass * kRadixBits, w.hist + pass * kRadix, w.lookback, w.counters + pass, epoch);
+    const KeyT *kin = reinterpret_cast<const KeyT *>(w.keys[src]);
+    KeyT *kout = LAST ? nullptr : reinterpret_cast<KeyT *>(w.keys[src ^ 1]);
+    const uint32_t *vin = FIRST ? nullptr : w.vals[src];
+    const int shift = pass * kRadixBits;
+    hipError_t e;
+    if (w.config.alg == kSortOnesweep) {
+        const uint32_t epoch = w.epoch++;
+        if (prof)
+            prof->begin(K_SORT_PASS);
+        hipLaunchKernelGGL((k_radix_scatter<KeyT, THREADS, ITEMS, FIRST, LAST, true>), dim3((unsigned)tiles),
+                           dim3(THREADS), 0, stream, kin, vin, kout, vals_out, n, shift, w.hist + pass * kRadix,
+                           w.lookback, w.counters + pass, epoch, nullptr, tiles);
+        if (prof)
+            prof->end(K_SORT_PASS);
+        return hipGetLastError();
+    }
+    // reduce-then-scan: the tile counts live in the lookback buffer (which
+    // holds at least sort_tiles(n) * kRadix 64-bit words), the digit totals
+    // in the first kRadix words of hist.
+    uint32_t *counts = reinterpret_cast<uint32_t *>(w.lookback);
+    if (prof)
+        prof->begin(K_SORT_UPSWEEP);
+    hipLaunchKernelGGL((k_rts_upsweep<KeyT, THREADS, ITEMS>), dim3((unsigned)tiles), dim3(THREADS), 0, stream, kin,
+                       n, shift, counts, tiles);
+    if (prof)
+        prof->end(K_SORT_UPSWEEP);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if (prof)
+        prof->begin(K_SORT_SCAN);
+    hipLaunchKernelGGL(k_rts_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, tiles, w.hist);
+    if (prof)
+        prof->end(K_SORT_SCAN);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if (prof)
+        prof->begin(K_SORT_PASS);
+    hipLaunchKernelGGL((k_radix_scatter<KeyT, THREADS, ITEMS, FIRST, LAST, false>), dim3((unsigned)tiles),
+                       dim3(THREADS), 0, stream, kin, vin, kout, vals_out, n, shift, w.hist, nullptr, nullptr, 0u,
+                       counts, tiles);
+    if (prof)
+        prof->end(K_SORT_PASS);
     return hipGetLastError();
 }
 
 template <typename KeyT, int THREADS, int ITEMS>
 hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream)
+                           hipStream_t stream, Profiler *prof)
 {
     if (pass == 0)
-        return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream);
+        return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream, prof);
     if (pass == passes - 1)
-        return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream);
-    return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream);
+        return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream, prof);
+    return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream, prof);
 }
 
 template <typename KeyT>
 hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream)
+                           hipStream_t stream, Profiler *prof)
 {
     const SortConfig c = w.config;
     if (c.threads == 512 && c.items == 16)
-        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream);
+        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream, prof);
     if (c.threads == 256 && c.items == 24)
-        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream);
+        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof);
     if (c.threads == 512 && c.items == 12)
-        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream);
-    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream);
+        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof);
+    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof);
 }
 
 template <typename KeyT>
@@ -402,24 +533,26 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
 {
     constexpr int kPasses = (int)sizeof(KeyT);
     hipError_t e;
-    // The epoch lives in 30 bits of the lookback word; wrap by clearing (a
-    // fill kernel, not a memset node, so that the sort stays capturable).
-    if (w.epoch + kPasses >= (1u << 30) || w.epoch == 0) {
-        const int64_t words = sort_tiles(n) * kRadix;
-        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, w.lookback,
-                           words, (uint64_t)0);
+    if (w.config.alg == kSortOnesweep) {
+        // The epoch lives in 30 bits of the lookback word; wrap by clearing
+        // (a fill kernel, not a memset node, so that the sort stays
+        // capturable).  Epoch 0 also marks words left by reduce-then-scan.
+        if (w.epoch + kPasses >= (1u << 30) || w.epoch == 0) {
+            const int64_t words = sort_tiles(n) * kRadix;
+            hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream,
+                               w.lookback, words, (uint64_t)0);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            w.epoch = 1;
+        }
+        if (prof)
+            prof->begin(K_SORT_ZERO);
+        hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kPasses * kRadix, w.counters,
+                           kPasses);
+        if (prof)
+            prof->end(K_SORT_ZERO);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
-        w.epoch = 1;
-    }
-    if (prof)
-        prof->begin(K_SORT_ZERO);
-    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kPasses * kRadix, w.counters, kPasses);
-    if (prof)
-        prof->end(K_SORT_ZERO);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
-    {
         const int64_t per_block = (int64_t)kHistThreads * kHistItems;
         int64_t blocks = (n + per_block - 1) / per_block;
         if (blocks > 2048)
@@ -432,16 +565,13 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
             prof->end(K_SORT_HIST);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
+    } else {
+        w.epoch = 0;  // the tile counts overwrite lookback words
     }
     int src = 0;
     for (int p = 0; p < kPasses; p++) {
         uint32_t *vout = (p == kPasses - 1) ? perm_out : w.vals[src ^ 1];
-        if (prof)
-            prof->begin(K_SORT_PASS);
-        e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream);
-        if (prof)
-            prof->end(K_SORT_PASS);
-        if (e != hipSuccess)
+        if ((e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream, prof)) != hipSuccess)
             return e;
         src ^= 1;
     }
@@ -452,6 +582,8 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
 
 bool sort_config_supported(SortConfig c)
 {
+    if (c.alg != kSortReduceScan && c.alg != kSortOnesweep)
+        return false;
     return (c.threads == 256 && (c.items == 16 || c.items == 24)) ||
            (c.threads == 512 && (c.items == 12 || c.items == 16));
 }
