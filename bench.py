@@ -29,6 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 (vector and matrix) peak
+FLOPS_PER_FORCE_EVAL = 16     # FP64 flops per force evaluation + kick/drift, a divide counted as one
+# canonical HBM bytes per particle-step of the whole step (8-bit digits, uint32
+# payload, keys from the integrate epilogue): float64 state / float32 state
+STEP_BYTES = {"f64": 304, "f32": 124}
+
 
 def bytes_per_particle(state_bytes: int) -> dict:
     """Algorithmic HBM bytes per particle of each kernel family of one step,
@@ -39,13 +45,20 @@ def bytes_per_particle(state_bytes: int) -> dict:
     first = s + (s + 4)              # read key; write key + payload
     middle = 2 * (s + 4)             # read and write key + payload
     last = (s + 4) + 4               # read key + payload; write the permutation
+    # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
+    integrate = (4 + 3 * s + 4) + (3 * s + 4 + s + 1)
     return {
-        "sort_histogram": s,                                     # read the key
+        "sort_histogram": s,                                     # read the key (onesweep)
+        "sort_upsweep": s,                                       # read the key (reduce-then-scan)
         "sort_scatter": (first + (passes - 2) * middle + last) / passes,
-        # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
-        "integrate": (4 + 3 * s + 4) + (3 * s + 4 + s + 1),
+        "integrate": integrate,                                  # plain form
+        "integrate_bulk": integrate,                             # binned forms: every particle's gather + write
         "radius_keys": 2 * s,
     }
+
+
+WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
+            "integrate_lc_wave": "lc_wave"}
 
 
 
@@ -122,6 +135,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
                       file=sys.stderr, flush=True)
         el = time.perf_counter() - t0
         stats = ctx.kernel_stats()
+        evals = ctx.force_evals()
         ctx.set_option(ns.NS_OPT_PROFILE, 0)
         hits = ctx.cap_hits()
         K1, W1 = ctx.energy(p)
@@ -134,6 +148,10 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         d = {"launches": launches, "total_ms": round(ms, 4), "avg_us": round(avg_ms * 1e3, 3)}
         if name in bpp:
             d["algorithmic_GBps"] = round(bpp[name] * n / (avg_ms * 1e-3) / 1e9, 1)
+        if name in WORK_ROW and ms > 0:
+            ev = evals[WORK_ROW[name]]
+            d["force_evals_per_launch"] = ev / max(launches, 1)
+            d["fp64_TFLOPs"] = round(FLOPS_PER_FORCE_EVAL * ev / (ms * 1e-3) / 1e12, 4)
         kernels[name] = d
     lev = lv & ns.NS_LVL_MASK
     return {
@@ -142,6 +160,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         "value": n * steps / el,
         "ms_per_step": el / steps * 1e3,
         "stats": stats,
+        "evals": evals,
         "kernels": kernels,
         "E0": K0 + W0,
         "E1": K1 + W1,
@@ -152,16 +171,29 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
 
 
 def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
+    """HBM roofline of a streaming kernel, or, for the deep and Levi-Civita
+    wave kernels (a few thousand particles, latency-bound chains of dependent
+    FP64 operations), its FP64 rate against the FP64 peak."""
     stats = run["stats"]
-    if kernel not in stats or kernel not in run["bpp"]:
+    if kernel not in stats:
         return None
     launches, ms = stats[kernel]
     avg_s = ms / launches * 1e-3
-    bpp = run["bpp"][kernel]
-    achieved = bpp * n / avg_s / 1e9
-    return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-            "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
+    if kernel in run["bpp"]:
+        bpp = run["bpp"][kernel]
+        achieved = bpp * n / avg_s / 1e9
+        return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
+    if kernel in WORK_ROW:
+        ev = run["evals"][WORK_ROW[kernel]] / launches
+        flops = FLOPS_PER_FORCE_EVAL * ev
+        achieved = flops / avg_s / 1e12
+        return {"kernel": kernel, "bound": "fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None,
+                "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
+                "avg_us": round(avg_s * 1e6, 2)}
+    return None
 
 
 def main() -> int:
@@ -226,6 +258,10 @@ def main() -> int:
                    "state": args.precision, "lowl_frac": 0.05, "lowl_scale": 1e-3, "seed": args.seed},
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
+        "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
+                          "achieved": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9 / PEAK_HBM_GBS, 4)},
         "dominant_kernel": dom,
         "kernels": chosen["kernels"],
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),

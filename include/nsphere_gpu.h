@@ -162,6 +162,12 @@ int ns_gpu_energy(ns_gpu_ctx *ctx, const ns_step_params *p, double *K, double *W
 int ns_gpu_kernel_stats(ns_gpu_ctx *ctx, int *count, const char **names,
                         int64_t *launches, double *total_ms);
 int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);
+/* Force evaluations (the work units of NS_OPT_BUDGET: the reference's
+ * sequential count) spent while NS_OPT_PROFILE was on, since the last
+ * ns_gpu_kernel_stats_reset: out[0] by the plain kernel or the binned forms'
+ * bulk kernel, out[1] by the deep kernel, out[2] by the Levi-Civita wave
+ * kernel.  Synchronises the context's stream. */
+int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[3]);
 
 /* Seeded initial conditions in G = M = a = 1 units (host, deterministic per
  * particle through a Philox-4x32-10 counter stream keyed by seed):

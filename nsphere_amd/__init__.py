@@ -192,6 +192,12 @@ class GpuContext:
     def reset_kernel_stats(self) -> None:
         check(lib.ns_gpu_kernel_stats_reset(self._h))
 
+    def force_evals(self) -> dict:
+        """Force evaluations spent while profiling, per integrate kernel."""
+        out = (ctypes.c_uint64 * 3)()
+        check(lib.ns_gpu_force_evals(self._h, out))
+        return {"bulk_or_plain": out[0], "deep": out[1], "lc_wave": out[2]}
+
 
 # -- reference-interface mirror -------------------------------------------
 

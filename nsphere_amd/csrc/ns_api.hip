@@ -35,6 +35,8 @@ struct ns_gpu_ctx {
     double *energy_partial = nullptr;
     double *energy_out = nullptr;  // device, 2 doubles
     unsigned long long *budget_hits = nullptr;
+    unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
+    int64_t work_stride = 0;
     uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_lcw = nullptr, *q_count = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
@@ -66,7 +68,8 @@ const char *kernel_name(int id)
 {
     static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
-                                         "sort_tiefix", "sort_upsweep", "sort_scan"};
+                                         "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
+                                         "integrate_deep", "integrate_lc_wave"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -207,7 +210,7 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits, c->q_lf, c->q_lc, c->q_lcw, c->q_count,
+                    c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_lcw, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1]};
     for (void *b : bufs)
         if (b)
@@ -287,6 +290,8 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.q_count = c->q_count;
     a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
     a.cap_lc = 8;  // Levi-Civita tau-step levels 0..3 lane-per-particle
+    a.work = c->prof.on ? c->work : nullptr;
+    a.work_stride = c->work_stride;
     return a;
 }
 
@@ -333,9 +338,7 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
     a.id1 = gather ? c->id[d] : nullptr;
     a.keys = c->sw.keys[0];
     a.lvl = c->lvl;
-    c->prof.begin(ns::K_INTEGRATE);
-    HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream));
-    c->prof.end(ns::K_INTEGRATE);
+    HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
     c->cur = d;
     c->keys_valid = true;
     c->levels_valid = true;
@@ -424,13 +427,16 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->energy_partial, 2 * ns::energy_blocks(n));
     if (e == hipSuccess) e = dalloc(&c->energy_out, 2);
     if (e == hipSuccess) e = dalloc(&c->budget_hits, 1);
+    c->work_stride = ns::work_stride_for(npts);
+    if (e == hipSuccess) e = dalloc(&c->work, ns::kWorkKernels * c->work_stride);
     if (e == hipSuccess) e = dalloc(&c->q_lf, n);
     if (e == hipSuccess) e = dalloc(&c->q_lc, n);
     if (e == hipSuccess) e = dalloc(&c->q_lcw, n);
-    if (e == hipSuccess) e = dalloc(&c->q_count, 4);
+    if (e == hipSuccess) e = dalloc(&c->q_count, ns::kQueueCounters);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         int code = (e == hipErrorOutOfMemory) ? NS_ERR_NOMEM : NS_ERR_HIP;
@@ -810,6 +816,36 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
         return rc;
     DeviceGuard g(c->device);
     c->prof.reset();
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long)));
+    return NS_OK;
+}
+
+int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[3])
+{
+    if (int rc = check_ctx(c, "ns_gpu_force_evals"))
+        return rc;
+    if (!out)
+        return set_error(NS_ERR_ARG, "ns_gpu_force_evals: null output");
+    DeviceGuard g(c->device);
+    const size_t count = (size_t)ns::kWorkKernels * (size_t)c->work_stride;
+    unsigned long long *h = (unsigned long long *)malloc(count * sizeof(unsigned long long));
+    if (!h)
+        return set_error(NS_ERR_NOMEM, "ns_gpu_force_evals: host buffer");
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(h, c->work, count * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        free(h);
+        return set_error(NS_ERR_HIP, "ns_gpu_force_evals: %s", hipGetErrorString(e));
+    }
+    for (int k = 0; k < ns::kWorkKernels; k++) {
+        uint64_t s = 0;
+        for (int64_t j = 0; j < c->work_stride; j++)
+            s += h[k * c->work_stride + j];
+        out[k] = s;
+    }
+    free(h);
     return NS_OK;
 }
 

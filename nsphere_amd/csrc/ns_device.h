@@ -118,6 +118,9 @@ enum KernelId : int {
     K_TIEFIX,
     K_SORT_UPSWEEP,
     K_SORT_SCAN,
+    K_INT_BULK,
+    K_INT_DEEP,
+    K_INT_LCWAVE,
     K_COUNT
 };
 const char *kernel_name(int id);

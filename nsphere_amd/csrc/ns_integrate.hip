@@ -59,6 +59,32 @@ __device__ inline void wave_push(uint32_t *q, uint32_t *count, bool pred, uint32
         q[base + (uint32_t)__popcll(m & lanemask_lt())] = value;
 }
 
+// Profiling: adds the wave's force evaluations (every lane's `units`) to the
+// wave's own slot of the kernel's row.  Every lane of the wave must call it,
+// once, at the end of the kernel.
+__device__ inline void add_work(const StepArgs &a, int kernel, unsigned long long units)
+{
+    if (!a.work)
+        return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        units += __shfl_xor(units, off, 64);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0 && units != 0)
+        a.work[kernel * a.work_stride + wave] += units;
+}
+
+// Next queue position for the calling wave: one atomic per wave on a ticket,
+// so that waves which drew cheap particles take the next ones (a static
+// stride could stack two long chains on one wave).  Wave-uniform result.
+__device__ inline uint32_t wave_ticket(uint32_t *ticket)
+{
+    uint32_t q = 0;
+    if ((threadIdx.x & 63) == 0)
+        q = atomicAdd(ticket, 1u);
+    return __shfl(q, 0, 64);
+}
+
 template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit)
 {
@@ -95,20 +121,23 @@ template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n)
-        return;
-    double r, v, ell;
-    load_particle<GATHER, S>(a, i, r, v, ell);
-    carry_ell_id<GATHER, S>(a, i, ell);
-    const Phys ph = make_phys(a, i, ell);
-    Budget bud{a.budget, false};
-    double r_new, v_new;
-    int rec;
-    if (takes_levi_civita(a, i, r, ell))
-        rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
-    else
-        rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
-    write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+    unsigned long long spent = 0;
+    if (i < a.n) {
+        double r, v, ell;
+        load_particle<GATHER, S>(a, i, r, v, ell);
+        carry_ell_id<GATHER, S>(a, i, ell);
+        const Phys ph = make_phys(a, i, ell);
+        Budget bud{a.budget, false};
+        double r_new, v_new;
+        int rec;
+        if (takes_levi_civita(a, i, r, ell))
+            rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
+        else
+            rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+        spent = (unsigned long long)(a.budget - bud.left);
+    }
+    add_work(a, 0, spent);
 }
 
 template <bool GATHER, bool FAST, typename S>
@@ -117,6 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     bool to_lc = false, to_lf = false;
+    unsigned long long spent = 0;
     if (valid) {
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
@@ -132,10 +162,12 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
                 to_lf = true;
             else
                 write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+            spent = (unsigned long long)(a.budget - bud.left);
         }
     }
     wave_push(a.q_lc, a.q_count + 1, to_lc, (uint32_t)i);
     wave_push(a.q_lf, a.q_count + 0, to_lf, (uint32_t)i);
+    add_work(a, 0, spent);
 }
 
 // Blocks [0, kDeepLfBlocks): one wave per queued leapfrog particle.
@@ -143,11 +175,11 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 {
+    unsigned long long spent = 0;
     if (blockIdx.x < kDeepLfBlocks) {
         const int lane = threadIdx.x & 63;
         const uint32_t count = a.q_count[0];
-        const uint32_t waves = kDeepLfBlocks * (kThreads / 64);
-        for (uint32_t q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); q < count; q += waves) {
+        for (uint32_t q = wave_ticket(a.q_count + 3); q < count; q = wave_ticket(a.q_count + 3)) {
             const int64_t i = a.q_lf[q];
             double r, v, ell;
             load_particle<GATHER, S>(a, i, r, v, ell);
@@ -155,8 +187,10 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
             Budget bud{a.budget, false};
             double r_new, v_new;
             const int rec = adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
-            if (lane == 0)
+            if (lane == 0) {
                 write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+                spent += (unsigned long long)(a.budget - bud.left);  // the wave's particle, once
+            }
         }
     } else {
         const uint32_t count = a.q_count[1];
@@ -173,8 +207,10 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
             if (!escalate)
                 write_result<S>(a, i, r_new, v_new, rec, bud.hit);
             wave_push(a.q_lcw, a.q_count + 2, escalate, (uint32_t)i);
+            spent += (unsigned long long)(a.budget - bud.left);
         }
     }
+    add_work(a, 1, spent);
 }
 
 // One wave per Levi-Civita particle whose tau steps need more than cap_lc:
@@ -185,8 +221,8 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t count = a.q_count[2];
-    const uint32_t waves = gridDim.x * (kThreads / 64);
-    for (uint32_t q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); q < count; q += waves) {
+    unsigned long long spent = 0;
+    for (uint32_t q = wave_ticket(a.q_count + 4); q < count; q = wave_ticket(a.q_count + 4)) {
         const int64_t i = a.q_lcw[q];
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
@@ -194,9 +230,12 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
         Budget bud{a.budget, false};
         double r_new, v_new;
         const int rec = adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud);
-        if (lane == 0)
+        if (lane == 0) {
             write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+            spent += (unsigned long long)(a.budget - bud.left);
+        }
     }
+    add_work(a, 2, spent);
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -291,44 +330,72 @@ __global__ void k_energy_final(const double *__restrict__ partial, int nblocks, 
 }
 
 template <bool GATHER, bool FAST, typename S>
-hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream)
+hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler *prof)
 {
     const dim3 grid((unsigned)((a.n + kThreads - 1) / kThreads));
+    auto begin = [&](int kid) {
+        if (prof)
+            prof->begin(kid);
+    };
+    auto end = [&](int kid) {
+        if (prof)
+            prof->end(kid);
+    };
     if (form == FORM_PLAIN) {
+        begin(K_INTEGRATE);
         hipLaunchKernelGGL((k_integrate_plain<GATHER, FAST, S>), grid, dim3(kThreads), 0, stream, a);
+        end(K_INTEGRATE);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, 3);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, kQueueCounters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
+    begin(K_INT_BULK);
     hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST, S>), grid, dim3(kThreads), 0, stream, a);
+    end(K_INT_BULK);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
+    begin(K_INT_DEEP);
     hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(kDeepLfBlocks + kDeepLcBlocks), dim3(kThreads),
                        0, stream, a);
+    end(K_INT_DEEP);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
+    begin(K_INT_LCWAVE);
     hipLaunchKernelGGL((k_integrate_lc_wave<GATHER, FAST, S>), dim3(kLcWaveBlocks), dim3(kThreads), 0, stream, a);
+    end(K_INT_LCWAVE);
     return hipGetLastError();
 }
 
 template <typename S>
-hipError_t launch_typed(const StepArgs &a, bool gather, int form, hipStream_t stream)
+hipError_t launch_typed(const StepArgs &a, bool gather, int form, hipStream_t stream, Profiler *prof)
 {
     const bool fast = form == FORM_BINNED_FAST;
     if (gather)
-        return fast ? launch_form<true, true, S>(a, form, stream) : launch_form<true, false, S>(a, form, stream);
-    return fast ? launch_form<false, true, S>(a, form, stream) : launch_form<false, false, S>(a, form, stream);
+        return fast ? launch_form<true, true, S>(a, form, stream, prof)
+                    : launch_form<true, false, S>(a, form, stream, prof);
+    return fast ? launch_form<false, true, S>(a, form, stream, prof)
+                : launch_form<false, false, S>(a, form, stream, prof);
 }
 
 } // namespace
 
-hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream)
+int64_t work_stride_for(int64_t n)
+{
+    const int64_t bulk = (n + kThreads - 1) / kThreads * (kThreads / 64);
+    const int64_t deep = (int64_t)(kDeepLfBlocks + kDeepLcBlocks) * (kThreads / 64);
+    const int64_t lcw = (int64_t)kLcWaveBlocks * (kThreads / 64);
+    int64_t m = bulk > deep ? bulk : deep;
+    return m > lcw ? m : lcw;
+}
+
+hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream, Profiler *prof)
 {
     if (a.n <= 0)
         return hipSuccess;
-    return f32 ? launch_typed<float>(a, gather, form, stream) : launch_typed<double>(a, gather, form, stream);
+    return f32 ? launch_typed<float>(a, gather, form, stream, prof)
+               : launch_typed<double>(a, gather, form, stream, prof);
 }
 
 template <typename S>

@@ -7,9 +7,13 @@
 //   STRICT (FAST = false): the reference's operation order with IEEE division
 //     and no contraction (the library is compiled with -ffp-contract=off), so
 //     results equal the strict CPU restatement bit for bit;
-//   FAST: one reciprocal (v_rcp_f64 refined by two Newton steps) per force
-//     evaluation and fused multiply-adds; same algorithm and control flow,
-//     results within a few ulp per operation of the strict form.
+//   FAST: one v_rcp_f64 per force evaluation refined by one Newton step, the
+//     kick coefficient folded into the force constants, fused multiply-adds
+//     and (Levi-Civita) rescaled state, so that a micro-step is seven (leap-
+//     frog) or nine (Levi-Civita) FP64 operations -- a lone wave's chain runs
+//     at about 3 ns per operation on MI355X, so the count sets the tail of
+//     the step; same algorithm and control flow, results within a few tens
+//     of ulp per force of the strict form.
 // Exact rewrites used by both forms (no result changes):
 //   * fine pass at level k (4N+1 micro-steps) == coarse pass at level k+1
 //     (2(2N)+1 micro-steps): same step sizes and operations;
@@ -60,64 +64,64 @@ __device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r,
     return r > 1.0e-30 && r < r_crit;
 }
 
-// 1/x for the fast form: v_rcp_f64 (about 24 correct bits on gfx950) and one
-// Newton step, within 11 ulp of the correctly rounded reciprocal (measured
-// over 4M values spanning 2^-40..2^40); a second step would make it exact.
+// Fast-form reciprocal: v_rcp_f64 (y = (1 + d)/x, |d| < 2^-24 on gfx950)
+// and one Newton step, y + y (1 - x y): error d^2, within 11 ulp of the
+// correctly rounded 1/x over 4M values spanning 2^-40..2^40.
 __device__ inline double fast_rcp(double x)
 {
     const double y = __builtin_amdgcn_rcp(x);
-    const double e = fma(-x, y, 1.0);
-    return fma(y, e, y);
+    return fma(y, fma(-x, y, 1.0), y);
 }
 
 // gravitational_force + effective_angular_force (nsphere.c:502-535).
-template <bool FAST>
-__device__ inline double accel(const Phys &ph, double r)
+__device__ inline double accel_strict(const Phys &ph, double r)
 {
-    if constexpr (!FAST) {
-        const double r2 = r * r;
-        const double f = ph.idg ? 0.0 : ph.gm / r2;
-        return f + ph.l2 / (r2 * r);
-    } else {
-        const double inv = fast_rcp(r);
-        return fma(ph.l2, inv, ph.idg ? 0.0 : ph.gm) * (inv * inv);
-    }
+    const double r2 = r * r;
+    const double f = ph.idg ? 0.0 : ph.gm / r2;
+    return f + ph.l2 / (r2 * r);
 }
 
 // forceLCfun (nsphere.c:1777-1787 with 563-599): -GM/rho^2 + ell^2/rho^4.
+__device__ inline double force_lc_strict(const Phys &ph, double rho)
+{
+    const double rr = rho * rho;
+    const double g = ph.idg ? 0.0 : ph.gm / rr;
+    return g + ph.l2 / (rr * rho * rho);
+}
+
+// A kick of coefficient kk.  The fast form folds kk into the force
+// constants: kk accel(r) = (lk / r + gk) / r^2, lk = kk ell^2, gk = kk gm
+// (and kk forceLC(rho) = (lk / rho^2 + gk) / rho^2 with kk = dtau).
+struct Kick {
+    double kk, lk, gk;
+};
+
+__device__ inline Kick make_kick(const Phys &ph, double kk)
+{
+    return {kk, ph.l2 * kk, (ph.idg ? 0.0 : ph.gm) * kk};
+}
+
 template <bool FAST>
-__device__ inline double force_lc(const Phys &ph, double rho)
+__device__ inline void kick(const Phys &ph, const Kick &c, double r, double &v)
 {
     if constexpr (!FAST) {
-        const double rr = rho * rho;
-        const double g = ph.idg ? 0.0 : ph.gm / rr;
-        return g + ph.l2 / (rr * rho * rho);
+        v += c.kk * accel_strict(ph, r);
     } else {
-        const double inv = fast_rcp(rho);
-        const double inv2 = inv * inv;
-        return fma(ph.l2, inv2, ph.idg ? 0.0 : ph.gm) * inv2;
+        const double inv = fast_rcp(r);
+        v = fma(fma(c.lk, inv, c.gk), inv * inv, v);
     }
 }
 
 template <bool FAST>
-__device__ inline void drift_kick(const Phys &ph, double ms, double kk, double &r, double &v)
+__device__ inline void drift_kick(const Phys &ph, double ms, const Kick &c, double &r, double &v)
 {
     if constexpr (!FAST) {
         r += ms * v;
-        v += kk * accel<false>(ph, r);
+        v += c.kk * accel_strict(ph, r);
     } else {
         r = fma(ms, v, r);
-        v = fma(kk, accel<true>(ph, r), v);
+        kick<true>(ph, c, r, v);
     }
-}
-
-template <bool FAST>
-__device__ inline void kick(const Phys &ph, double kk, double r, double &v)
-{
-    if constexpr (!FAST)
-        v += kk * accel<false>(ph, r);
-    else
-        v = fma(kk, accel<true>(ph, r), v);
 }
 
 // Force-evaluation budget per particle and step: a guard against inputs on
@@ -146,11 +150,12 @@ __device__ inline void micro_leap(const Phys &ph, double r_in, double v_in, doub
     const double hk = fine ? h / (4.0 * Nd) : h / (2.0 * Nd);
     const double ms = fine ? h / (2.0 * Nd) : h / (1.0 * Nd);
     const int64_t pairs = fine ? 2 * N : N;
+    const Kick kh = make_kick(ph, hk), km = make_kick(ph, ms);
     double r = r_in, v = v_in;
-    kick<FAST>(ph, hk, r, v);
+    kick<FAST>(ph, kh, r, v);
     for (int64_t pp = 1; pp <= pairs - 1; pp++)
-        drift_kick<FAST>(ph, ms, ms, r, v);
-    drift_kick<FAST>(ph, ms, hk, r, v);
+        drift_kick<FAST>(ph, ms, km, r, v);
+    drift_kick<FAST>(ph, ms, kh, r, v);
     r_out = r;
     v_out = v;
 }
@@ -246,6 +251,7 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
     const double hk = coarse_lane ? h / (2.0 * Nd) : h / (4.0 * Nd);
     const double ms = coarse_lane ? h / (1.0 * Nd) : h / (2.0 * Nd);
     const int64_t pairs = coarse_lane ? N : 2 * N;
+    const Kick kh = make_kick(ph, hk), km = make_kick(ph, ms);
     double r = r_in, v = v_in;
     if (!bud.take(2)) {
         r_out = r_in;
@@ -253,7 +259,7 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
         return NS_LVL_NOCONV | NS_LVL_MASK;
     }
     if (active)
-        kick<FAST>(ph, hk, r, v);
+        kick<FAST>(ph, kh, r, v);
     int64_t t = 0;
     double prev_r = 0.0, prev_v = 0.0;  // fine of the previous level (shifted coarse)
     for (int k = 0; k <= kmax; k++) {
@@ -275,12 +281,12 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
         if (k == 0) {
             for (; t < target; t++)
                 if (active && t < pairs)
-                    drift_kick<FAST>(ph, ms, (t < pairs - 1) ? ms : hk, r, v);
+                    drift_kick<FAST>(ph, ms, (t < pairs - 1) ? km : kh, r, v);
         } else {
             if (lane >= k && lane <= kmax) {
                 for (int64_t it = t; it < target - 1; it++)
-                    drift_kick<FAST>(ph, ms, ms, r, v);
-                drift_kick<FAST>(ph, ms, lane == k ? hk : ms, r, v);
+                    drift_kick<FAST>(ph, ms, km, r, v);
+                drift_kick<FAST>(ph, ms, lane == k ? kh : km, r, v);
             }
             t = target;
         }
@@ -306,6 +312,48 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
     return NS_LVL_NOCONV;
 }
 
+// Fast-form state of one Levi-Civita pass (doMicroLeviCivita's loop).  The
+// half-kicked velocity vh is carried as u = hd vh (hd = dtau/2), so that the
+// drift is rho' = rho + rho u; the kick constants are pre-scaled so that
+// g inv2 = hd dtau forceLC(rho); the physical time is accumulated as
+// T = sum (rho + rho')^2, t = t_in + dtau T / 4 at the end.  The closing
+// half-kick of one micro-step and the opening one of the next are one kick.
+struct LcFast {
+    double hd, lh, gh;  // dtau/2, hd dtau ell^2, hd dtau gm
+    double rho, u, u_prev, T;
+    double g, inv2;     // force terms at the current rho
+
+    __device__ inline void init(const Phys &ph, double dtau, double rho_in, double v_in)
+    {
+        hd = 0.5 * dtau;
+        const double sc = hd * dtau;
+        lh = ph.l2 * sc;
+        gh = (ph.idg ? 0.0 : ph.gm) * sc;
+        rho = rho_in;
+        T = 0.0;
+        const double q = rho * rho;
+        inv2 = fast_rcp(q);
+        g = fma(lh, inv2, gh);
+        u = fma(0.5 * g, inv2, hd * v_in);  // hd (v + hd forceLC(rho))
+        u_prev = u;
+    }
+    __device__ inline void step()
+    {
+        const double rn = fma(rho, u, rho);
+        inv2 = fast_rcp(rn * rn);
+        g = fma(lh, inv2, gh);
+        const double s = rho + rn;
+        T = fma(s, s, T);
+        u_prev = u;
+        u = fma(g, inv2, u);
+        rho = rn;
+    }
+    // velocity after the steps taken: the closing half kick on the u held
+    // before the last full kick
+    __device__ inline double v_end() const { return fma(0.5 * g, inv2, u_prev) / hd; }
+    __device__ inline double t_end(double t_in) const { return fma(0.5 * hd, T, t_in); }
+};
+
 // doMicroLeviCivita, nsphere.c:1990-2036 (force at rho_next reused).
 template <bool FAST>
 __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, double t_in, int64_t sub,
@@ -313,12 +361,12 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
 {
     const double dtau = h_tau / (double)sub;
     double rho = rho_in, v = v_in, t = t_in;
-    double f = force_lc<FAST>(ph, rho);
     if constexpr (!FAST) {
+        double f = force_lc_strict(ph, rho);
         for (int64_t ss = 0; ss < sub; ss++) {
             const double v_half = v + 0.5 * dtau * f;
             const double rho_next = rho + dtau * (0.5 * rho * v_half);
-            const double f2 = force_lc<false>(ph, rho_next);
+            const double f2 = force_lc_strict(ph, rho_next);
             const double v_next = v_half + 0.5 * dtau * f2;
             const double rho_mid = 0.5 * (rho + rho_next);
             const double dt_phys = dtau * (rho_mid * rho_mid);
@@ -328,25 +376,13 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
             f = f2;
         }
     } else if (sub > 0) {
-        // fast form: the closing half-kick of one micro-step and the opening
-        // half-kick of the next merge into one kick of dtau, and hd*rho
-        // leaves the dependency chain
-        const double hd = 0.5 * dtau;
-        double vh = fma(hd, f, v);
-        for (int64_t ss = 0; ss < sub - 1; ss++) {
-            const double rho_next = fma(hd * rho, vh, rho);
-            const double f2 = force_lc<true>(ph, rho_next);
-            const double rho_mid = 0.5 * (rho + rho_next);
-            t = fma(dtau, rho_mid * rho_mid, t);
-            vh = fma(dtau, f2, vh);
-            rho = rho_next;
-        }
-        const double rho_next = fma(hd * rho, vh, rho);
-        const double f2 = force_lc<true>(ph, rho_next);
-        const double rho_mid = 0.5 * (rho + rho_next);
-        t = fma(dtau, rho_mid * rho_mid, t);
-        v = fma(hd, f2, vh);
-        rho = rho_next;
+        LcFast lf;
+        lf.init(ph, dtau, rho_in, v_in);
+        for (int64_t ss = 0; ss < sub; ss++)
+            lf.step();
+        rho = lf.rho;
+        v = lf.v_end();
+        t = lf.t_end(t_in);
     }
     rho_out = rho;
     v_out = v;
@@ -452,25 +488,24 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     const bool active = coarse_lane || lane <= kmax + 1;
     const int64_t sub = coarse_lane ? 3 : (active ? 4 * ((int64_t)1 << lane) + 1 : 1);
     const double dtau = h / (double)sub;
-    double rho = rho_in, v = v_in, t = t_in;
-    double f = force_lc<FAST>(ph, rho);
+    // strict: the reference's variables; fast: the pass state of micro_lc<true>
+    double rho = rho_in, v = v_in, t = t_in, f = 0.0;
+    LcFast lf;
+    if constexpr (!FAST)
+        f = force_lc_strict(ph, rho);
+    else
+        lf.init(ph, dtau, rho_in, v_in);
     if (!bud.take(4)) {
         rho_out = rho_in;
         v_out = v_in;
         t_out = t_in;
         return NS_LVL_NOCONV | NS_LVL_MASK;
     }
-    // Fast form: vh is the half-kicked velocity (the closing half kick of one
-    // micro-step and the opening one of the next merge into one kick of
-    // dtau, as in micro_lc<true>); vh_prev is its value before the last
-    // micro-step, from which the pass's final velocity follows.
-    const double hd = 0.5 * dtau;
-    double vh = FAST ? fma(hd, f, v) : 0.0, vh_prev = 0.0;
     auto micro_step = [&]() {
         if constexpr (!FAST) {
             const double v_half = v + 0.5 * dtau * f;
             const double rho_next = rho + dtau * (0.5 * rho * v_half);
-            const double f2 = force_lc<false>(ph, rho_next);
+            const double f2 = force_lc_strict(ph, rho_next);
             const double v_next = v_half + 0.5 * dtau * f2;
             const double rho_mid = 0.5 * (rho + rho_next);
             const double dt_phys = dtau * (rho_mid * rho_mid);
@@ -479,22 +514,22 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
             v = v_next;
             f = f2;
         } else {
-            const double rho_next = fma(hd * rho, vh, rho);
-            const double f2 = force_lc<true>(ph, rho_next);
-            const double rho_mid = 0.5 * (rho + rho_next);
-            t = fma(dtau, rho_mid * rho_mid, t);
-            vh_prev = vh;
-            vh = fma(dtau, f2, vh);
-            rho = rho_next;
-            f = f2;
+            lf.step();
         }
     };
-    // velocity at the end of the micro-steps this lane has run
+    // state at the end of the micro-steps this lane has run
+    auto rho_now = [&]() -> double { return FAST ? lf.rho : rho; };
     auto v_now = [&]() -> double {
         if constexpr (FAST)
-            return fma(hd, f, vh_prev);
+            return lf.v_end();
         else
             return v;
+    };
+    auto t_now = [&]() -> double {
+        if constexpr (FAST)
+            return lf.t_end(t_in);
+        else
+            return t;
     };
     // Iterations [4*2^(k-1)+1, 4*2^k+1) belong to lanes k..kmax+1, whose
     // passes all run through them, so the lane mask is set once per level.
@@ -520,16 +555,16 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
         const int src_c = (k == 0) ? 63 : k - 1;
         if (!bud.take(4 * Nk + 2)) {
             // coarse of level k: the previous fine pass (or lane 63)
-            const double vn = v_now();
-            rho_out = __shfl(rho, src_c, 64);
+            const double rn = rho_now(), vn = v_now(), tn = t_now();
+            rho_out = __shfl(rn, src_c, 64);
             v_out = __shfl(vn, src_c, 64);
-            t_out = __shfl(t, src_c, 64);
+            t_out = __shfl(tn, src_c, 64);
             return (k == 0 ? 0 : level_of(Nk >> 1)) | NS_LVL_NOCONV;
         }
         run_level(k);
-        const double vn = v_now();
-        const double rhoC = __shfl(rho, src_c, 64), vC = __shfl(vn, src_c, 64), tC = __shfl(t, src_c, 64);
-        const double rhoF = __shfl(rho, k, 64), vF = __shfl(vn, k, 64), tF = __shfl(t, k, 64);
+        const double rn = rho_now(), vn = v_now(), tn = t_now();
+        const double rhoC = __shfl(rn, src_c, 64), vC = __shfl(vn, src_c, 64), tC = __shfl(tn, src_c, 64);
+        const double rhoF = __shfl(rn, k, 64), vF = __shfl(vn, k, 64), tF = __shfl(tn, k, 64);
         const double rF = rhoF * rhoF;
         const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
         const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
@@ -541,17 +576,17 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     // not converged: the fine pass at the overflowed N = 2^(kmax+1)
     const int64_t Nf = (int64_t)2 << kmax;
     if (!bud.take(4 * Nf + 2)) {
-        const double vn = v_now();
-        rho_out = __shfl(rho, kmax, 64);
+        const double rn = rho_now(), vn = v_now(), tn = t_now();
+        rho_out = __shfl(rn, kmax, 64);
         v_out = __shfl(vn, kmax, 64);
-        t_out = __shfl(t, kmax, 64);
+        t_out = __shfl(tn, kmax, 64);
         return level_of(Nf >> 1) | NS_LVL_NOCONV;
     }
     run_level(kmax + 1);
-    const double vn = v_now();
-    rho_out = __shfl(rho, kmax + 1, 64);
+    const double rn = rho_now(), vn = v_now(), tn = t_now();
+    rho_out = __shfl(rn, kmax + 1, 64);
     v_out = __shfl(vn, kmax + 1, 64);
-    t_out = __shfl(t, kmax + 1, 64);
+    t_out = __shfl(tn, kmax + 1, 64);
     return level_of(Nf) | NS_LVL_NOCONV;
 }
 
