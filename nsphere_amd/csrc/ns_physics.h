@@ -124,6 +124,29 @@ __device__ inline void drift_kick(const Phys &ph, double ms, const Kick &c, doub
     }
 }
 
+// n (the same on every active lane) calls of f, four per loop trip, with a
+// scalar trip counter: a lone wave's chain pays for every loop instruction.
+__device__ inline int64_t uniform_i64(int64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <typename F>
+__device__ inline void repeat_uniform(int64_t n, F &&f)
+{
+    int64_t m = uniform_i64(n);
+    for (; m >= 4; m -= 4) {
+        f();
+        f();
+        f();
+        f();
+    }
+    for (; m > 0; m--)
+        f();
+}
+
 // Force-evaluation budget per particle and step: a guard against inputs on
 // which the reference would loop for hours (NaN state, r -> 0 in the LC
 // branch); the default is far above any converging particle's cost.
@@ -284,8 +307,7 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
                     drift_kick<FAST>(ph, ms, (t < pairs - 1) ? km : kh, r, v);
         } else {
             if (lane >= k && lane <= kmax) {
-                for (int64_t it = t; it < target - 1; it++)
-                    drift_kick<FAST>(ph, ms, km, r, v);
+                repeat_uniform(target - 1 - t, [&]() { drift_kick<FAST>(ph, ms, km, r, v); });
                 drift_kick<FAST>(ph, ms, lane == k ? kh : km, r, v);
             }
             t = target;
@@ -378,6 +400,7 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
     } else if (sub > 0) {
         LcFast lf;
         lf.init(ph, dtau, rho_in, v_in);
+#pragma unroll 4
         for (int64_t ss = 0; ss < sub; ss++)
             lf.step();
         rho = lf.rho;
@@ -545,8 +568,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
                 for (int s = 3; s < 5; s++)
                     micro_step();
         } else if (lane >= k && lane <= kmax + 1) {
-            for (int64_t s = it; s < end; s++)
-                micro_step();
+            repeat_uniform(end - it, micro_step);  // 2^(k+1) steps
         }
         it = end;
     };
