@@ -57,8 +57,7 @@ def bytes_per_particle(state_bytes: int) -> dict:
     }
 
 
-WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
-            "integrate_lc_wave": "lc_wave"}
+WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep"}
 
 
 

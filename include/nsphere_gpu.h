@@ -78,8 +78,9 @@ enum {
     NS_OPT_PROFILE = 2,    /* 1: time every kernel with HIP events             */
     NS_OPT_INTEGRATOR = 3, /* 0 plain: one lane per particle, levels in turn,
                                  strict IEEE order of the reference (default);
-                              1 binned: levels 0-3 per lane, deeper particles
-                                 on a whole wave with one level per lane;
+                              1 binned: leapfrog levels 0-3 per lane; deeper
+                                 leapfrog particles and every Levi-Civita
+                                 particle on a whole wave, one level per lane;
                                  bit-identical to 0;
                               2 binned with the fast arithmetic form (one
                                  reciprocal per force, fused multiply-adds)   */
@@ -165,9 +166,9 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);
 /* Force evaluations (the work units of NS_OPT_BUDGET: the reference's
  * sequential count) spent while NS_OPT_PROFILE was on, since the last
  * ns_gpu_kernel_stats_reset: out[0] by the plain kernel or the binned forms'
- * bulk kernel, out[1] by the deep kernel, out[2] by the Levi-Civita wave
- * kernel.  Synchronises the context's stream. */
-int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[3]);
+ * bulk kernel, out[1] by their wave-per-particle kernel.  Synchronises the
+ * context's stream. */
+int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[2]);
 
 /* Seeded initial conditions in G = M = a = 1 units (host, deterministic per
  * particle through a Philox-4x32-10 counter stream keyed by seed):

@@ -194,9 +194,9 @@ class GpuContext:
 
     def force_evals(self) -> dict:
         """Force evaluations spent while profiling, per integrate kernel."""
-        out = (ctypes.c_uint64 * 3)()
+        out = (ctypes.c_uint64 * 2)()
         check(lib.ns_gpu_force_evals(self._h, out))
-        return {"bulk_or_plain": out[0], "deep": out[1], "lc_wave": out[2]}
+        return {"bulk_or_plain": out[0], "deep": out[1]}
 
 
 # -- reference-interface mirror -------------------------------------------

@@ -37,7 +37,7 @@ struct ns_gpu_ctx {
     unsigned long long *budget_hits = nullptr;
     unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
     int64_t work_stride = 0;
-    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_lcw = nullptr, *q_count = nullptr;
+    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool levels_valid = false;
@@ -69,7 +69,7 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
-                                         "integrate_deep", "integrate_lc_wave"};
+                                         "integrate_deep"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -210,7 +210,7 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_lcw, c->q_count,
+                    c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1]};
     for (void *b : bufs)
         if (b)
@@ -286,10 +286,8 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.budget_hits = c->budget_hits;
     a.q_lf = c->q_lf;
     a.q_lc = c->q_lc;
-    a.q_lcw = c->q_lcw;
     a.q_count = c->q_count;
     a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
-    a.cap_lc = 8;  // Levi-Civita tau-step levels 0..3 lane-per-particle
     a.work = c->prof.on ? c->work : nullptr;
     a.work_stride = c->work_stride;
     return a;
@@ -431,7 +429,6 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->work, ns::kWorkKernels * c->work_stride);
     if (e == hipSuccess) e = dalloc(&c->q_lf, n);
     if (e == hipSuccess) e = dalloc(&c->q_lc, n);
-    if (e == hipSuccess) e = dalloc(&c->q_lcw, n);
     if (e == hipSuccess) e = dalloc(&c->q_count, ns::kQueueCounters);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
@@ -821,7 +818,7 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
     return NS_OK;
 }
 
-int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[3])
+int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[2])
 {
     if (int rc = check_ctx(c, "ns_gpu_force_evals"))
         return rc;

@@ -120,7 +120,6 @@ enum KernelId : int {
     K_SORT_SCAN,
     K_INT_BULK,
     K_INT_DEEP,
-    K_INT_LCWAVE,
     K_COUNT
 };
 const char *kernel_name(int id);

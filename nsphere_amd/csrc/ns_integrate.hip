@@ -32,9 +32,7 @@ namespace ns {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kDeepLfBlocks = 64;     // 256 waves for the wave-per-particle queue
-constexpr int kDeepLcBlocks = 1024;   // 262144 lanes for the Levi-Civita queue
-constexpr int kLcWaveBlocks = 128;    // 512 waves for escalated Levi-Civita particles
+constexpr int kDeepBlocks = 1024;     // 4096 waves for the wave-per-particle queues
 
 __device__ inline uint64_t lanemask_lt()
 {
@@ -170,72 +168,33 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
     add_work(a, 0, spent);
 }
 
-// Blocks [0, kDeepLfBlocks): one wave per queued leapfrog particle.
-// Blocks [kDeepLfBlocks, ...): one lane per queued Levi-Civita particle.
+// One wave per queued particle: every queued Levi-Civita particle (its tau
+// steps' levels one per lane), then every leapfrog particle past cap_N (its
+// levels one per lane).  Waves take particles by ticket, the Levi-Civita
+// queue first: it holds the particles with the longest chains, which then
+// start at once instead of after a lane-per-particle attempt.
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 {
-    unsigned long long spent = 0;
-    if (blockIdx.x < kDeepLfBlocks) {
-        const int lane = threadIdx.x & 63;
-        const uint32_t count = a.q_count[0];
-        for (uint32_t q = wave_ticket(a.q_count + 3); q < count; q = wave_ticket(a.q_count + 3)) {
-            const int64_t i = a.q_lf[q];
-            double r, v, ell;
-            load_particle<GATHER, S>(a, i, r, v, ell);
-            const Phys ph = make_phys(a, i, ell);
-            Budget bud{a.budget, false};
-            double r_new, v_new;
-            const int rec = adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
-            if (lane == 0) {
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
-                spent += (unsigned long long)(a.budget - bud.left);  // the wave's particle, once
-            }
-        }
-    } else {
-        const uint32_t count = a.q_count[1];
-        const uint32_t lanes = (gridDim.x - kDeepLfBlocks) * kThreads;
-        for (uint32_t q = (blockIdx.x - kDeepLfBlocks) * kThreads + threadIdx.x; q < count; q += lanes) {
-            const int64_t i = a.q_lc[q];
-            double r, v, ell;
-            load_particle<GATHER, S>(a, i, r, v, ell);
-            const Phys ph = make_phys(a, i, ell);
-            Budget bud{a.budget, false};
-            double r_new, v_new;
-            const int rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_lc);
-            const bool escalate = rec == kEscalate;
-            if (!escalate)
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
-            wave_push(a.q_lcw, a.q_count + 2, escalate, (uint32_t)i);
-            spent += (unsigned long long)(a.budget - bud.left);
-        }
-    }
-    add_work(a, 1, spent);
-}
-
-// One wave per Levi-Civita particle whose tau steps need more than cap_lc:
-// every tau step's levels run one per lane (restarting the particle from its
-// step-start state, so the result is that of the lane form).
-template <bool GATHER, bool FAST, typename S>
-__global__ __launch_bounds__(kThreads) void k_integrate_lc_wave(StepArgs a)
-{
     const int lane = threadIdx.x & 63;
-    const uint32_t count = a.q_count[2];
+    const uint32_t n_lc = a.q_count[1], n_lf = a.q_count[0];
     unsigned long long spent = 0;
-    for (uint32_t q = wave_ticket(a.q_count + 4); q < count; q = wave_ticket(a.q_count + 4)) {
-        const int64_t i = a.q_lcw[q];
+    for (uint32_t q = wave_ticket(a.q_count + 2); q < n_lc + n_lf; q = wave_ticket(a.q_count + 2)) {
+        const bool lc = q < n_lc;  // wave-uniform
+        const int64_t i = lc ? a.q_lc[q] : a.q_lf[q - n_lc];
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
         const Phys ph = make_phys(a, i, ell);
         Budget bud{a.budget, false};
         double r_new, v_new;
-        const int rec = adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud);
+        const int rec = lc ? adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud)
+                           : adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
         if (lane == 0) {
             write_result<S>(a, i, r_new, v_new, rec, bud.hit);
-            spent += (unsigned long long)(a.budget - bud.left);
+            spent += (unsigned long long)(a.budget - bud.left);  // the wave's particle, once
         }
     }
-    add_work(a, 2, spent);
+    add_work(a, 1, spent);
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -357,14 +316,8 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     begin(K_INT_DEEP);
-    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(kDeepLfBlocks + kDeepLcBlocks), dim3(kThreads),
-                       0, stream, a);
+    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(kDeepBlocks), dim3(kThreads), 0, stream, a);
     end(K_INT_DEEP);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
-    begin(K_INT_LCWAVE);
-    hipLaunchKernelGGL((k_integrate_lc_wave<GATHER, FAST, S>), dim3(kLcWaveBlocks), dim3(kThreads), 0, stream, a);
-    end(K_INT_LCWAVE);
     return hipGetLastError();
 }
 
@@ -384,10 +337,8 @@ hipError_t launch_typed(const StepArgs &a, bool gather, int form, hipStream_t st
 int64_t work_stride_for(int64_t n)
 {
     const int64_t bulk = (n + kThreads - 1) / kThreads * (kThreads / 64);
-    const int64_t deep = (int64_t)(kDeepLfBlocks + kDeepLcBlocks) * (kThreads / 64);
-    const int64_t lcw = (int64_t)kLcWaveBlocks * (kThreads / 64);
-    int64_t m = bulk > deep ? bulk : deep;
-    return m > lcw ? m : lcw;
+    const int64_t deep = (int64_t)kDeepBlocks * (kThreads / 64);
+    return bulk > deep ? bulk : deep;
 }
 
 hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream, Profiler *prof)

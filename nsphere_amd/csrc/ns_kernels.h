@@ -26,22 +26,20 @@ struct StepArgs {
     int32_t out_type, N_taumin, identity_gravity;
     // binned integrator: queues of ranks handed between its kernels
     uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
-    uint32_t *q_lc;              // Levi-Civita particles (lane per particle)
-    uint32_t *q_lcw;             // Levi-Civita particles past cap_lc in some tau step (wave)
-    uint32_t *q_count;           // lengths of q_lf, q_lc, q_lcw; wave tickets of
-                                 // the q_lf and q_lcw consumers (kQueueCounters)
+    uint32_t *q_lc;              // Levi-Civita particles (wave per particle)
+    uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave
+                                 // ticket (kQueueCounters)
     int64_t cap_N;               // deepest leapfrog N tried lane-per-particle
-    int64_t cap_lc;              // deepest Levi-Civita N tried lane-per-particle
     // profiling only (else null): force evaluations spent, one slot per
     // wave of a launch (slot = global wave index; each wave adds its own with
     // a plain read-modify-write, launches being stream-ordered), in rows of
-    // work_stride slots: row 0 plain or bulk, 1 deep, 2 lc_wave
+    // work_stride slots: row 0 plain or bulk, 1 deep
     unsigned long long *work;
     int64_t work_stride;
 };
 
-constexpr int kWorkKernels = 3;
-constexpr int kQueueCounters = 5;
+constexpr int kWorkKernels = 2;
+constexpr int kQueueCounters = 3;
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
 
@@ -54,7 +52,7 @@ enum IntegratorForm : int {
 
 struct Profiler;
 // Times the plain form as K_INTEGRATE, the binned forms' three kernels as
-// K_INT_BULK, K_INT_DEEP and K_INT_LCWAVE (prof may be null).
+// K_INT_BULK and K_INT_DEEP (prof may be null).
 hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream,
                             Profiler *prof);
 template <typename S>

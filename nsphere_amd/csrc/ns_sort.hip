@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__r
 }
 
 // Reduce-then-scan, step 1: the pass digit's count in every tile, written
-// digit-major (counts[d * tiles + tile]) so that step 2 scans rows.  A wave
+// tile-major (counts[tile * kRadix + d], one 1 KB row per tile).  A wave
 // whose 64 keys share one digit (common for the near-constant high bytes of
 // nearly sorted radii) adds them with one LDS atomic instead of 64 colliding
 // ones; other waves add per lane.
@@ -238,12 +238,15 @@ __global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict_
 #pragma unroll
         for (int w = 0; w < WAVES; w++)
             c += s_cnt[w][tid];
-        counts[(int64_t)tid * tiles + tile] = c;
+        counts[tile * kRadix + tid] = c;
     }
 }
 
-// Reduce-then-scan, step 2: one workgroup per digit turns its row of tile
+// Reduce-then-scan, step 2: one workgroup per digit turns its column of tile
 // counts into exclusive tile offsets (in place) and writes the digit total.
+// Workgroups are dealt to the eight XCDs round-robin; workgroup b takes digit
+// (b % 8) * 32 + b / 8, so that the 16 digits of a 64-byte line of a tile row
+// are all read and written through one XCD's L2.
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 
@@ -253,7 +256,8 @@ __global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict_
     constexpr int WAVES = kScanThreads / 64;
     __shared__ uint32_t s_wave[WAVES];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t *row = counts + (int64_t)blockIdx.x * tiles;
+    const int d = (blockIdx.x & 7) * (kRadix / 8) + (blockIdx.x >> 3);
+    uint32_t *col = counts + d;  // entry t at col[t * kRadix]
     uint32_t carry = 0;
     for (int64_t base = 0; base < tiles; base += kScanThreads * kScanItems) {
         const int64_t first = base + (int64_t)tid * kScanItems;
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict_
         uint32_t sum = 0;
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
-            x[k] = first + k < tiles ? row[first + k] : 0u;
+            x[k] = first + k < tiles ? col[(first + k) * kRadix] : 0u;
             sum += x[k];
         }
         uint32_t incl = sum;
@@ -286,13 +290,13 @@ __global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict_
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
             if (first + k < tiles)
-                row[first + k] = run;
+                col[(first + k) * kRadix] = run;
             run += x[k];
         }
         carry += chunk;
     }
     if (tid == 0)
-        totals[blockIdx.x] = carry;
+        totals[d] = carry;
 }
 
 // One scatter pass over digit `shift`.  FIRST: payload = input position.
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict_
 // ticket and each tile's global digit offsets come from decoupled lookback
 // over earlier tiles, with hist_pass the pass's digit histogram.  Otherwise
 // (reduce-then-scan) tile = blockIdx.x, the offsets are read from
-// tile_offsets[d * tiles + tile] and hist_pass holds the digit totals.
+// tile_offsets[tile * kRadix + d] and hist_pass holds the digit totals.
 template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST, bool LOOKBACK>
 __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     const KeyT *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, KeyT *__restrict__ keys_out,
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
             total += c;
         }
         if (!LOOKBACK) {
-            excl = tile_offsets[(int64_t)d * tiles + tile];
+            excl = tile_offsets[(int64_t)tile * kRadix + d];
         } else if (tile == 0) {
             lb_store(lookback + d, lb_pack(epoch, kStatusPrefix, total));
         } else {
