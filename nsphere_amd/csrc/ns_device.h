@@ -23,6 +23,18 @@ constexpr int kHistItems = 16;
 
 inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
+// MI355X deals a launch's workgroups to its eight XCDs round-robin (block b
+// to XCD b % 8), and each XCD has its own L2.  This bijection of [0, nblocks)
+// gives XCD x one contiguous range of block indices instead, so that
+// neighbouring blocks -- which gather from overlapping windows, or write the
+// two halves of one cache line -- meet in the same L2.
+__device__ inline uint32_t xcd_contiguous_block(uint32_t b, uint32_t nblocks)
+{
+    const uint32_t q = nblocks / 8, r = nblocks % 8;
+    const uint32_t x = b % 8, j = b / 8;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // Order-preserving 64-bit radius key: unsigned order of the key equals the
 // order of compare_particles (nsphere.c:8297-8306) on non-NaN radii; -0 and
 // +0 map to one key (they compare equal there), every NaN maps to one key

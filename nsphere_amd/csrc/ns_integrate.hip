@@ -118,7 +118,7 @@ __device__ inline void carry_ell_id(const StepArgs &a, int64_t i, double ell)
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     unsigned long long spent = 0;
     if (i < a.n) {
         double r, v, ell;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
     bool to_lc = false, to_lf = false;
     unsigned long long spent = 0;
@@ -209,7 +209,7 @@ __global__ void k_gather_state(const S *__restrict__ r0, const S *__restrict__ v
                                S *__restrict__ r1, S *__restrict__ v1, S *__restrict__ l1,
                                uint32_t *__restrict__ id1, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i < n) {
         const uint32_t p = perm[i];
         r1[i] = r0[p];

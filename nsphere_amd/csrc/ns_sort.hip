@@ -328,7 +328,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     const int lane = tid & 63, wave = tid >> 6;
 
     if (tid == 0)
-        s_tile = LOOKBACK ? atomicAdd(tile_counter, 1u) : (uint32_t)blockIdx.x;
+        s_tile = LOOKBACK ? atomicAdd(tile_counter, 1u) : xcd_contiguous_block(blockIdx.x, gridDim.x);
     for (int i = tid; i < WAVES * kRadix; i += THREADS)
         (&s_wcnt[0][0])[i] = 0;
     {
