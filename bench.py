@@ -57,6 +57,12 @@ def bytes_per_particle(state_bytes: int) -> dict:
     }
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+# (N = 1.2e7, float32 state), scaled by the counters' reading for kernels of known traffic
+# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md); sort_scatter is the mean over its 4 passes.
+MEASURED_TRAFFIC = {("f32", 12_000_000): {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6,
+                                          "integrate_bulk": 499.3e6, "integrate_deep": 14.0e6}}
+
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep"}
 
 
@@ -154,6 +160,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         kernels[name] = d
     lev = lv & ns.NS_LVL_MASK
     return {
+        "precision": precision,
         "bpp": bpp,
         "seconds": el,
         "value": n * steps / el,
@@ -169,6 +176,10 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
     }
 
 
+def traffic_for(run: dict, n: int, kernel: str):
+    return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
+
+
 def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
     """HBM roofline of a streaming kernel, or, for the deep and Levi-Civita
     wave kernels (a few thousand particles, latency-bound chains of dependent
@@ -182,14 +193,15 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
         bpp = run["bpp"][kernel]
         achieved = bpp * n / avg_s / 1e9
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(run, n, kernel),
                 "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
     if kernel in WORK_ROW:
         ev = run["evals"][WORK_ROW[kernel]] / launches
         flops = FLOPS_PER_FORCE_EVAL * ev
         achieved = flops / avg_s / 1e12
         return {"kernel": kernel, "bound": "fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+                "traffic": traffic_for(run, n, kernel),
                 "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
                 "avg_us": round(avg_s * 1e6, 2)}
     return None
