@@ -145,6 +145,21 @@ int ns_gpu_sync(ns_gpu_ctx *ctx);
  * was created or last reset; synchronises the context's stream. */
 int ns_gpu_cap_hits(ns_gpu_ctx *ctx, uint64_t *count, int reset);
 
+/* Per-particle masses (no reference counterpart: the reference's particles
+ * all weigh halo_mass / npts and M(r) is taken from the rank, nsphere.c:513).
+ * mass[k] belongs to the particle held at position k of the current state
+ * (the order of the last upload, or of the last download); the masses then
+ * move with the particles.  From the next integrate on, M_enc at rank i is
+ * the exclusive prefix sum of the masses in rank order (a decoupled-lookback
+ * scan on the device, float64 for float64 state, float32 two-sum pairs for
+ * float32 state; reproducible bit for bit), the force uses it in place of
+ * (i/npts) M, r_crit of rank 0 uses a tenth of its own mass, and the energy
+ * uses the masses.  NULL returns to equal masses. */
+int ns_gpu_set_masses(ns_gpu_ctx *ctx, const double *mass);
+/* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
+ * [npts] the total, [npts + 1] the rank-0 particle's mass. */
+int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);
+
 /* Level records of the last integrate (npts bytes, rank order). */
 int ns_gpu_levels(ns_gpu_ctx *ctx, uint8_t *host_out);
 /* Shell-model energy of the current state with the rank-based mass of

@@ -192,6 +192,22 @@ class GpuContext:
     def reset_kernel_stats(self) -> None:
         check(lib.ns_gpu_kernel_stats_reset(self._h))
 
+    def set_masses(self, mass) -> None:
+        """Per-particle masses in the current state order (None: equal masses)."""
+        if mass is None:
+            check(lib.ns_gpu_set_masses(self._h, None))
+            return
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        if m.shape != (self.npts,):
+            raise ValueError(f"mass must have shape ({self.npts},)")
+        check(lib.ns_gpu_set_masses(self._h, m.ctypes.data))
+
+    def enclosed_mass(self) -> np.ndarray:
+        """M_enc of the last scan: [i] below rank i, [n] total, [n + 1] rank-0 mass."""
+        out = np.empty(self.npts + 2, dtype=np.float64)
+        check(lib.ns_gpu_enclosed_mass(self._h, out.ctypes.data))
+        return out
+
     def force_evals(self) -> dict:
         """Force evaluations spent while profiling, per integrate kernel."""
         out = (ctypes.c_uint64 * 2)()

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -35,6 +36,13 @@ struct ns_gpu_ctx {
     double *energy_partial = nullptr;
     double *energy_out = nullptr;  // device, 2 doubles
     unsigned long long *budget_hits = nullptr;
+    // per-particle masses (optional): float64 / float32 per buffer, like the
+    // state; menc = enclosed mass per rank of the last scan (n + 2 doubles)
+    double *md[2] = {};
+    float *mf[2] = {};
+    double *menc = nullptr;
+    bool has_mass = false;
+    ns::MassScan ms = {};
     unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
     int64_t work_stride = 0;
     uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
@@ -69,7 +77,7 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
-                                         "integrate_deep"};
+                                         "integrate_deep", "mass_scan"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -211,7 +219,8 @@ void free_all(ns_gpu_ctx *c)
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
-                    c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1]};
+                    c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
+                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ticket};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -289,8 +298,50 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.q_count = c->q_count;
     a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
     a.work = c->prof.on ? c->work : nullptr;
+    a.menc = c->has_mass ? c->menc : nullptr;
     a.work_stride = c->work_stride;
     return a;
+}
+
+// Per-particle mass buffers, allocated on first use.
+int ensure_mass(ns_gpu_ctx *c)
+{
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; b++)
+        if (!c->md[b])
+            e = dalloc(&c->md[b], c->n);
+    if (e == hipSuccess && c->f32)
+        for (int b = 0; b < 2 && e == hipSuccess; b++)
+            if (!c->mf[b])
+                e = dalloc(&c->mf[b], c->n);
+    if (e == hipSuccess && !c->menc)
+        e = dalloc(&c->menc, c->n + 2);
+    if (e == hipSuccess && !c->ms.words)
+        e = dalloc(&c->ms.words, ns::mass_scan_words(c->n));
+    if (e == hipSuccess && !c->ms.ticket)
+        e = dalloc(&c->ms.ticket, 1);
+    if (e != hipSuccess)
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "mass buffers (npts=%lld): %s",
+                         (long long)c->n, hipGetErrorString(e));
+    return NS_OK;
+}
+
+// Enclosed mass per rank for the masses of buffer `cur` taken through perm
+// (null: in place); with gather the rank-ordered masses go to buffer cur^1.
+int mass_scan(ns_gpu_ctx *c, const uint32_t *perm, bool gather)
+{
+    if (!c->has_mass)
+        return NS_OK;
+    const int s = c->cur, d = c->cur ^ 1;
+    c->prof.begin(ns::K_MASS_SCAN);
+    if (c->f32)
+        HIP_TRY(ns::launch_mass_scan<float>(c->ms, c->mf[s], perm, gather ? c->mf[d] : nullptr, c->menc, c->n,
+                                            c->stream));
+    else
+        HIP_TRY(ns::launch_mass_scan<double>(c->ms, c->md[s], perm, gather ? c->md[d] : nullptr, c->menc, c->n,
+                                             c->stream));
+    c->prof.end(ns::K_MASS_SCAN);
+    return NS_OK;
 }
 
 // Sort r[cur] into c->perm (keys from the integrate epilogue when valid).
@@ -312,6 +363,8 @@ int sort_perm(ns_gpu_ctx *c)
 
 int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
 {
+    if (int rc = mass_scan(c, gather ? c->perm : nullptr, gather))
+        return rc;
     ns::StepArgs a = make_args(c, p);
     // gather: previous order in buffer cur -> new rank order in cur^1;
     // otherwise in place (each thread reads and writes only its own index).
@@ -524,6 +577,15 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
             HIP_TRY(hipStreamSynchronize(c->stream));
         }
         c->f32 = want;
+        if (c->has_mass && c->n > 0) {  // and the masses
+            if (int rc = ensure_mass(c))
+                return rc;
+            if (want)
+                HIP_TRY((ns::launch_convert<float, double>(c->md[c->cur], c->mf[c->cur], c->n, c->stream)));
+            else
+                HIP_TRY((ns::launch_convert<double, float>(c->mf[c->cur], c->md[c->cur], c->n, c->stream)));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
         c->keys_valid = false;
         return NS_OK;
     }
@@ -639,6 +701,8 @@ int ns_gpu_sort_particles(ns_gpu_ctx *c)
         return set_error(NS_ERR_STATE, "ns_gpu_sort_particles: no state uploaded");
     DeviceGuard g(c->device);
     if (int rc = sort_perm(c))
+        return rc;
+    if (int rc = mass_scan(c, c->perm, true))  // masses move with the particles
         return rc;
     const int s = c->cur, d = c->cur ^ 1;
     c->prof.begin(ns::K_GATHER);
@@ -770,17 +834,20 @@ int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
     // that step's sort, not of the new radii).
     if (int rc = sort_perm(c))
         return rc;
+    if (int rc = mass_scan(c, c->perm, false))
+        return rc;
     const double m = p->halo_mass / (double)c->n;
+    const double *menc = c->has_mass ? c->menc : nullptr;
     c->prof.begin(ns::K_ENERGY);
     const int s = c->cur;
     if (c->f32)
-        HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], c->perm, c->n, m, p->vel_conv_sq * p->G,
-                                         p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
-                                         c->stream));
+        HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], c->perm, c->has_mass ? c->mf[s] : nullptr,
+                                         menc, c->n, m, p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity,
+                                         c->energy_partial, c->energy_out, c->stream));
     else
-        HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], c->perm, c->n, m, p->vel_conv_sq * p->G,
-                                          p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
-                                          c->stream));
+        HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], c->perm, c->has_mass ? c->md[s] : nullptr,
+                                          menc, c->n, m, p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity,
+                                          c->energy_partial, c->energy_out, c->stream));
     c->prof.end(ns::K_ENERGY);
     double out[2];
     HIP_TRY(hipMemcpyAsync(out, c->energy_out, sizeof out, hipMemcpyDeviceToHost, c->stream));
@@ -815,6 +882,56 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
     c->prof.reset();
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long)));
+    return NS_OK;
+}
+
+int ns_gpu_set_masses(ns_gpu_ctx *c, const double *mass)
+{
+    if (int rc = check_ctx(c, "ns_gpu_set_masses"))
+        return rc;
+    if (!mass) {
+        c->has_mass = false;
+        return NS_OK;
+    }
+    if (c->n == 0)
+        return NS_OK;
+    long double total = 0.0L;
+    for (int64_t i = 0; i < c->n; i++) {
+        if (!(mass[i] >= 0.0) || !std::isfinite(mass[i]))
+            return set_error(NS_ERR_ARG, "ns_gpu_set_masses: mass[%lld] = %g is not a finite non-negative number",
+                             (long long)i, mass[i]);
+        total += mass[i];
+    }
+    if (!(total > 0.0L))
+        return set_error(NS_ERR_ARG, "ns_gpu_set_masses: total mass is zero");
+    DeviceGuard g(c->device);
+    if (int rc = ensure_mass(c))
+        return rc;
+    // fixed-point scale of the scan's tile prefixes: a power of two with
+    // total * scale < 2^62 (the float32 pairs' truncations stay inside)
+    int ex = 0;
+    (void)std::frexp((double)total, &ex);  // total < 2^ex
+    c->ms.scale = std::ldexp(1.0, 61 - ex);
+    HIP_TRY(hipMemcpy(c->md[c->cur], mass, (size_t)c->n * sizeof(double), hipMemcpyHostToDevice));
+    if (c->f32) {
+        HIP_TRY((ns::launch_convert<float, double>(c->md[c->cur], c->mf[c->cur], c->n, c->stream)));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    c->has_mass = true;
+    return NS_OK;
+}
+
+int ns_gpu_enclosed_mass(ns_gpu_ctx *c, double *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_enclosed_mass"))
+        return rc;
+    if (!host_out)
+        return set_error(NS_ERR_ARG, "ns_gpu_enclosed_mass: null output");
+    if (!c->has_mass)
+        return set_error(NS_ERR_STATE, "ns_gpu_enclosed_mass: no per-particle masses set");
+    DeviceGuard g(c->device);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(host_out, c->menc, (size_t)(c->n + 2) * sizeof(double), hipMemcpyDeviceToHost));
     return NS_OK;
 }
 

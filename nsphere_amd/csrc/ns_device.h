@@ -132,6 +132,7 @@ enum KernelId : int {
     K_SORT_SCAN,
     K_INT_BULK,
     K_INT_DEEP,
+    K_MASS_SCAN,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -245,7 +245,8 @@ constexpr int kEnergyThreads = 256;
 template <typename S>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
     const S *__restrict__ r, const S *__restrict__ v, const S *__restrict__ ell, const uint32_t *__restrict__ perm,
-    int64_t n, double m, double gpre, double halo_mass, int idg, double *__restrict__ partial)
+    const S *__restrict__ mass, const double *__restrict__ menc, int64_t n, double m, double gpre, double halo_mass,
+    int idg, double *__restrict__ partial)
 {
     __shared__ double sk[kEnergyThreads], sw[kEnergyThreads];
     double k = 0.0, w = 0.0;
@@ -253,10 +254,11 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
     for (int64_t i = (int64_t)blockIdx.x * kEnergyThreads + threadIdx.x; i < n; i += stride) {
         const int64_t p = perm ? (int64_t)perm[i] : i;
         const double rr = r[p], vv = v[p], ll = ell[p];
-        k += m * 0.5 * (vv * vv + (ll * ll) / (rr * rr));
+        const double mi = mass ? (double)mass[p] : m;
+        k += mi * 0.5 * (vv * vv + (ll * ll) / (rr * rr));
         if (!idg) {
-            const double menc = ((double)i / (double)n) * halo_mass;
-            w -= gpre * menc * m / rr;
+            const double me = menc ? menc[i] : ((double)i / (double)n) * halo_mass;
+            w -= gpre * me * mi / rr;
         }
     }
     sk[threadIdx.x] = k;
@@ -396,12 +398,13 @@ int energy_blocks(int64_t n)
 }
 
 template <typename S>
-hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, int64_t n, double m,
-                         double gpre, double halo_mass, int idg, double *partial, double *out, hipStream_t stream)
+hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, const S *mass,
+                         const double *menc, int64_t n, double m, double gpre, double halo_mass, int idg,
+                         double *partial, double *out, hipStream_t stream)
 {
     const int blocks = energy_blocks(n);
-    hipLaunchKernelGGL(k_energy_partial<S>, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, n, m,
-                       gpre, halo_mass, idg, partial);
+    hipLaunchKernelGGL(k_energy_partial<S>, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, mass,
+                       menc, n, m, gpre, halo_mass, idg, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
@@ -409,8 +412,10 @@ hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *p
     return hipGetLastError();
 }
 template hipError_t launch_energy<double>(const double *, const double *, const double *, const uint32_t *,
-                                          int64_t, double, double, double, int, double *, double *, hipStream_t);
-template hipError_t launch_energy<float>(const float *, const float *, const float *, const uint32_t *, int64_t,
-                                         double, double, double, int, double *, double *, hipStream_t);
+                                          const double *, const double *, int64_t, double, double, double, int,
+                                          double *, double *, hipStream_t);
+template hipError_t launch_energy<float>(const float *, const float *, const float *, const uint32_t *,
+                                         const float *, const double *, int64_t, double, double, double, int,
+                                         double *, double *, hipStream_t);
 
 } // namespace ns

@@ -25,6 +25,9 @@ struct StepArgs {
     int64_t budget;              // force evaluations allowed per particle per step
     int32_t out_type, N_taumin, identity_gravity;
     // binned integrator: queues of ranks handed between its kernels
+    // per-particle masses (else null: M_enc from the rank, as the reference):
+    // menc[i] = mass enclosed below rank i, menc[n + 1] = mass of rank 0
+    const double *menc;
     uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
     uint32_t *q_lc;              // Levi-Civita particles (wave per particle)
     uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave
@@ -50,6 +53,21 @@ enum IntegratorForm : int {
     FORM_BINNED_FAST = 2   // as BINNED with the fast arithmetic form
 };
 
+// Workspace of the enclosed-mass scan (ns_scan.hip).
+struct MassScan {
+    uint64_t *words;   // mass_scan_words(n) lookback words
+    uint32_t *ticket;  // tile ticket
+    uint32_t epoch;    // lookback epoch of the next launch (0: clear first)
+    double scale;      // fixed-point scale, a power of two with total mass * scale < 2^62
+};
+int64_t mass_scan_words(int64_t n);
+// menc[i] (i < n) = sum of the masses of ranks < i, menc[n] = total,
+// menc[n + 1] = mass of rank 0; masses are mass_in[perm[i]] (perm may be
+// null), copied to mass_out (may be null) in rank order.
+template <typename M>
+hipError_t launch_mass_scan(MassScan &w, const M *mass_in, const uint32_t *perm, M *mass_out, double *menc,
+                            int64_t n, hipStream_t stream);
+
 struct Profiler;
 // Times the plain form as K_INTEGRATE, the binned forms' three kernels as
 // K_INT_BULK and K_INT_DEEP (prof may be null).
@@ -60,9 +78,12 @@ hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint
                                S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream);
 int energy_blocks(int64_t n);
+// mass (rank order via perm, may be null: equal masses m) and menc (may be
+// null: M_enc from the rank) select the per-particle-mass energy.
 template <typename S>
-hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, int64_t n, double m,
-                         double gpre, double halo_mass, int idg, double *partial, double *out, hipStream_t stream);
+hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, const S *mass,
+                         const double *menc, int64_t n, double m, double gpre, double halo_mass, int idg,
+                         double *partial, double *out, hipStream_t stream);
 // Element-wise precision conversion (float64 <-> float32, round to nearest).
 template <typename D, typename S>
 hipError_t launch_convert(const S *src, D *dst, int64_t n, hipStream_t stream);

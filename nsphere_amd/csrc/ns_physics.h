@@ -39,11 +39,20 @@ struct Phys {
     bool idg;   // identity gravity (nsphere.c:476)
 };
 
+// Enclosed mass at rank i: the reference's (i / npts) M (nsphere.c:513), or
+// with per-particle masses the scanned sum of the masses below rank i.
+__device__ inline double enclosed_mass(const StepArgs &a, int64_t i)
+{
+    return a.menc ? a.menc[i] : ((double)i / (double)a.n) * a.halo_mass;
+}
+
 __device__ inline Phys make_phys(const StepArgs &a, int64_t i, double ell)
 {
     Phys ph;
     ph.idg = a.identity_gravity != 0;
-    ph.gm = -(a.vel_conv_sq * a.G) * ((double)i / (double)a.n) * a.halo_mass;
+    // the reference's association, ((-(vel_conv_sq G)) (i / npts)) M
+    ph.gm = a.menc ? -(a.vel_conv_sq * a.G) * a.menc[i]
+                   : -(a.vel_conv_sq * a.G) * ((double)i / (double)a.n) * a.halo_mass;
     ph.l2 = ell * ell;
     return ph;
 }
@@ -53,11 +62,12 @@ __device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r,
 {
     double r_crit = 0.0;
     if (fabs(ell) > 1.0e-30) {
+        // rank 0: a tenth of one particle's mass (with masses: of its own)
         double M_enc;
         if (i == 0)
-            M_enc = 0.1 * (1.0 / (double)a.n) * a.halo_mass;
+            M_enc = a.menc ? 0.1 * a.menc[a.n + 1] : 0.1 * (1.0 / (double)a.n) * a.halo_mass;
         else
-            M_enc = ((double)i / (double)a.n) * a.halo_mass;
+            M_enc = enclosed_mass(a, i);
         const double grav_part = (a.vel_conv_sq * a.G) * M_enc;
         r_crit = (ell * ell) * a.alpha / grav_part;
     }

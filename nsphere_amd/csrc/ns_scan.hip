@@ -1,0 +1,277 @@
+// ns_scan.hip -- enclosed mass M(r) for per-particle masses, MI355X (gfx950).
+//
+// The reference has equal masses and takes M(r) from the rank,
+// ((double)i / npts) * HALO_MASS (nsphere.c:513, 5815), and so does this
+// library unless the caller sets per-particle masses (no reference function
+// has them).  Then the enclosed mass of the particle at rank i is the
+// exclusive prefix sum of the masses in rank order, computed in one pass:
+//   * a workgroup takes a tile of kScanTile masses by ticket, gathered
+//     through the sort's permutation when one is given; each thread sums 16
+//     consecutive masses in registers, then a wave-shuffle scan and a
+//     four-wave combine finish the tile -- in float64 for float64 masses, or
+//     for float32 masses in float32 (hi, lo) pairs with two-sum compensation
+//     (Knuth's TwoSum, renormalised after every addition);
+//   * the tile's aggregate, as a 64-bit fixed-point integer, is published
+//     and the tile's exclusive prefix is found by decoupled lookback, one
+//     wave reading 64 predecessor tiles per round.  Integer partial sums do
+//     not depend on how far a walk had to go, so M(r) is the same bit for bit
+//     on every run.
+// menc[i] = (double)(tile prefix) / scale + (within-tile exclusive prefix);
+// menc[n] = the total, menc[n + 1] = the mass of the particle at rank 0.
+#include <hip/hip_runtime.h>
+
+#include "ns_device.h"
+#include "ns_kernels.h"
+
+namespace ns {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScanWaves = kScanThreads / 64;
+constexpr uint32_t kFlagAggregate = 1;
+constexpr uint32_t kFlagPrefix = 2;
+
+// float32 pair hi + lo with |lo| <= ulp(hi) / 2 (exact TwoSum needs
+// -ffp-contract=off, which the library is built with)
+struct F2 {
+    float hi, lo;
+};
+
+__device__ inline F2 two_sum(float a, float b)
+{
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+__device__ inline F2 f2_add(F2 a, F2 b)
+{
+    const F2 s = two_sum(a.hi, b.hi);
+    const float lo = s.lo + (a.lo + b.lo);
+    const float hi = s.hi + lo;  // fast renormalisation: |lo| small against s.hi
+    return {hi, lo - (hi - s.hi)};
+}
+
+// Accumulator per mass type.
+template <typename M>
+struct Acc;
+
+template <>
+struct Acc<double> {
+    using T = double;
+    __device__ static T zero() { return 0.0; }
+    __device__ static T of(double m) { return m; }
+    __device__ static T add(T a, T b) { return a + b; }
+    __device__ static T shfl_up(T x, int d) { return __shfl_up(x, d, 64); }
+    __device__ static double value(T x) { return x; }
+    __device__ static int64_t fixed(T x, double scale) { return (int64_t)(x * scale); }
+};
+
+template <>
+struct Acc<float> {
+    using T = F2;
+    __device__ static T zero() { return {0.0f, 0.0f}; }
+    __device__ static T of(float m) { return {m, 0.0f}; }
+    __device__ static T add(T a, T b) { return f2_add(a, b); }
+    __device__ static T shfl_up(T x, int d) { return {__shfl_up(x.hi, d, 64), __shfl_up(x.lo, d, 64)}; }
+    __device__ static double value(T x) { return (double)x.hi + (double)x.lo; }
+    __device__ static int64_t fixed(T x, double scale)
+    {
+        return (int64_t)((double)x.hi * scale) + (int64_t)((double)x.lo * scale);
+    }
+};
+
+__device__ inline void word_store(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t word_load(const uint64_t *p)
+{
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A tile's 64-bit value goes out as two self-describing words, each
+// (tag << 32 | one half); a reader accepts a pair only when both tags carry
+// this launch's epoch and the same status, i.e. come from one publication.
+__device__ inline void publish(uint64_t *words, int64_t tile, uint32_t epoch, uint32_t status, uint64_t x)
+{
+    const uint64_t tag = (uint64_t)((epoch << 2) | status) << 32;
+    word_store(words + 2 * tile, tag | (x >> 32));
+    word_store(words + 2 * tile + 1, tag | (x & 0xFFFFFFFFull));
+}
+
+template <typename M>
+__global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict__ mass_in,
+                                                             const uint32_t *__restrict__ perm,
+                                                             M *__restrict__ mass_out, double *__restrict__ menc,
+                                                             int64_t n, double scale, uint64_t *__restrict__ words,
+                                                             uint32_t *__restrict__ ticket, uint32_t epoch)
+{
+    using A = Acc<M>;
+    using T = typename A::T;
+    __shared__ T s_wave[kScanWaves];
+    __shared__ uint32_t s_tile;
+    __shared__ int64_t s_prefix;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0)
+        s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t first = tile * kScanTile + (int64_t)tid * kScanItems;
+
+    M m[kScanItems];
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+        const int64_t idx = first + j;
+        M x = (M)0;
+        if (idx < n)
+            x = perm ? mass_in[perm[idx]] : mass_in[idx];
+        m[j] = x;
+        if (mass_out && idx < n)
+            mass_out[idx] = x;
+    }
+    T sum = A::zero();
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++)
+        sum = A::add(sum, A::of(m[j]));
+
+    // wave-inclusive scan of the thread sums, then the four-wave combine
+    T incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T y = A::shfl_up(incl, off);
+        if (lane >= off)
+            incl = A::add(y, incl);
+    }
+    if (lane == 63)
+        s_wave[wave] = incl;
+    __syncthreads();
+    T before = A::zero(), total = A::zero();
+#pragma unroll
+    for (int w = 0; w < kScanWaves; w++) {
+        if (w < wave)
+            before = A::add(before, s_wave[w]);
+        total = A::add(total, s_wave[w]);
+    }
+    // exclusive prefix of this thread's first item within the tile
+    T excl = A::shfl_up(incl, 1);
+    excl = lane == 0 ? before : A::add(before, excl);
+
+    // decoupled lookback on fixed-point tile aggregates (wave 0)
+    if (wave == 0) {
+        const uint64_t agg = (uint64_t)A::fixed(total, scale);
+        uint64_t acc = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                publish(words, 0, epoch, kFlagPrefix, agg);
+        } else {
+            if (lane == 0)
+                publish(words, tile, epoch, kFlagAggregate, agg);
+            int64_t base = tile - 1;
+            while (true) {
+                const int64_t t = base - lane;
+                uint32_t st = kFlagPrefix;  // before tile 0: an empty prefix
+                uint64_t val = 0;
+                if (t >= 0) {
+                    const uint64_t w0 = word_load(words + 2 * t), w1 = word_load(words + 2 * t + 1);
+                    const uint32_t g0 = (uint32_t)(w0 >> 32), g1 = (uint32_t)(w1 >> 32);
+                    st = (g0 == g1 && (g0 >> 2) == epoch) ? (g0 & 3u) : 0u;
+                    val = ((w0 & 0xFFFFFFFFull) << 32) | (w1 & 0xFFFFFFFFull);
+                }
+                const uint64_t not_ready = __ballot(st == 0);
+                const uint64_t prefix = __ballot(st == kFlagPrefix);
+                const int ready = not_ready ? __ffsll((unsigned long long)not_ready) - 1 : 64;
+                const uint64_t run = ready >= 64 ? ~0ull : ((1ull << ready) - 1);
+                const uint64_t pre = prefix & run;
+                // lanes taken this round: up to the nearest prefix, else the
+                // whole published run
+                const int take = pre ? __ffsll((unsigned long long)pre) : ready;
+                uint64_t part = lane < take ? val : 0;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1)
+                    part += __shfl_xor(part, off, 64);
+                acc += part;
+                if (pre)
+                    break;
+                if (take == 0)
+                    __builtin_amdgcn_s_sleep(1);
+                base -= take;
+            }
+            if (lane == 0)
+                publish(words, tile, epoch, kFlagPrefix, acc + agg);
+        }
+        if (lane == 0)
+            s_prefix = (int64_t)acc;
+    }
+    __syncthreads();
+    const double tile_base = (double)s_prefix / scale;
+
+    T run = excl;
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+        const int64_t idx = first + j;
+        if (idx < n)
+            menc[idx] = tile_base + A::value(run);
+        run = A::add(run, A::of(m[j]));
+    }
+    if (tile == 0 && tid == 0)
+        menc[n + 1] = (double)m[0];
+    if (first < n && first + kScanItems >= n) {  // the thread holding the last item
+        menc[n] = tile_base + A::value(run);
+    }
+}
+
+__global__ void k_zero_ticket(uint32_t *t)
+{
+    if (threadIdx.x == 0)
+        *t = 0;
+}
+
+__global__ void k_clear_words(uint64_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+} // namespace
+
+int64_t mass_scan_words(int64_t n) { return 2 * ((n + kScanTile - 1) / kScanTile) + 2; }
+
+template <typename M>
+hipError_t launch_mass_scan(MassScan &w, const M *mass_in, const uint32_t *perm, M *mass_out, double *menc,
+                            int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipError_t e;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (w.epoch == 0 || w.epoch >= (1u << 30) - 1) {  // 30-bit epochs: clear and restart
+        const int64_t words = mass_scan_words(n);
+        hipLaunchKernelGGL(k_clear_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, w.words,
+                           words);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        w.epoch = 1;
+    }
+    hipLaunchKernelGGL(k_zero_ticket, dim3(1), dim3(64), 0, stream, w.ticket);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((k_mass_scan<M>), dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, mass_in, perm,
+                       mass_out, menc, n, w.scale, w.words, w.ticket, w.epoch);
+    w.epoch++;
+    return hipGetLastError();
+}
+
+template hipError_t launch_mass_scan<double>(MassScan &, const double *, const uint32_t *, double *, double *,
+                                             int64_t, hipStream_t);
+template hipError_t launch_mass_scan<float>(MassScan &, const float *, const uint32_t *, float *, double *, int64_t,
+                                            hipStream_t);
+
+} // namespace ns

@@ -32,6 +32,7 @@ class OrcParams(ctypes.Structure):
         ("N_taumin", ctypes.c_int32),
         ("identity_gravity", ctypes.c_int32),
         ("budget", ctypes.c_int64),
+        ("menc", ctypes.c_void_p),   # NULL: the reference's (rank/npts) M
     ]
 
 
@@ -54,7 +55,7 @@ def reference_units(**kw) -> OrcParams:
 def from_step_params(sp, budget: int = 0) -> OrcParams:
     """Copy a nsphere_amd.StepParams into OrcParams (budget: the GPU context's
     NS_OPT_BUDGET when it is to be mirrored, else 0 = unlimited)."""
-    vals = [getattr(sp, f) for f, _ in OrcParams._fields_ if f != "budget"]
+    vals = [getattr(sp, f) for f, _ in OrcParams._fields_ if f not in ("budget", "menc")]
     return OrcParams(*vals, budget)
 
 
@@ -110,6 +111,10 @@ def lib(fast: bool = False):
         L.orc_integrate.argtypes = [vp, vp, vp, i64, P, vp]
         L.orc_energy.restype = None
         L.orc_energy.argtypes = [vp, vp, vp, i64, P, pd, pd]
+        L.orc_energy_masses.restype = None
+        L.orc_energy_masses.argtypes = [vp, vp, vp, vp, i64, P, pd, pd]
+        L.orc_mass_prefix.restype = None
+        L.orc_mass_prefix.argtypes = [vp, i64, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
         _libs[key] = L
@@ -180,6 +185,36 @@ def integrate(r, v, ell, p: OrcParams, fast=False) -> np.ndarray:
     lv = np.empty(n, np.uint8)
     lib(fast).orc_integrate(_p(r), _p(v), _p(ell), n, ctypes.byref(p), _p(lv))
     return lv
+
+
+def mass_prefix(mass) -> np.ndarray:
+    """Enclosed mass of rank-ordered masses: [i] below rank i (long double sum,
+    one rounding), [n] total, [n + 1] the rank-0 mass."""
+    m = np.ascontiguousarray(mass, np.float64)
+    out = np.empty(m.size + 2, np.float64)
+    lib().orc_mass_prefix(_p(m), m.size, _p(out))
+    return out
+
+
+def with_menc(p: OrcParams, menc) -> OrcParams:
+    """A copy of p whose force uses the enclosed-mass array menc (layout of
+    mass_prefix); the array is kept alive on the returned object."""
+    q = OrcParams()
+    ctypes.pointer(q)[0] = p
+    arr = np.ascontiguousarray(menc, np.float64)
+    q.menc = arr.ctypes.data
+    q._menc_ref = arr
+    return q
+
+
+def energy_masses(r, v, ell, mass, p: OrcParams):
+    k = ctypes.c_double()
+    w = ctypes.c_double()
+    lib().orc_energy_masses(_p(np.ascontiguousarray(r, np.float64)), _p(np.ascontiguousarray(v, np.float64)),
+                            _p(np.ascontiguousarray(ell, np.float64)),
+                            _p(np.ascontiguousarray(mass, np.float64)), r.size, ctypes.byref(p),
+                            ctypes.byref(k), ctypes.byref(w))
+    return k.value, w.value
 
 
 def energy(r, v, ell, p: OrcParams):

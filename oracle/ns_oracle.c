@@ -25,6 +25,8 @@ double orc_gravitational_force(double r, int rank, int npts, const orc_params *p
 {
     if (p->identity_gravity)
         return 0.0;
+    if (p->menc)
+        return -(p->vel_conv_sq * p->G) * p->menc[rank] / (r * r);
     return -(p->vel_conv_sq * p->G) * ((double)rank / (double)npts) * p->halo_mass / (r * r);
 }
 
@@ -39,6 +41,8 @@ double orc_gravitational_force_rho_v(double rho, int rank, int npts, const orc_p
 {
     if (p->identity_gravity)
         return 0.0;
+    if (p->menc)
+        return -(p->vel_conv_sq * p->G) * p->menc[rank] / (rho * rho);
     return -(p->vel_conv_sq * p->G) * ((double)rank / (double)npts) * p->halo_mass / (rho * rho);
 }
 
@@ -580,9 +584,9 @@ static inline uint8_t integrate_one(int64_t i, int64_t npts, double *r_io, doubl
     if (fabs(ell) > 1.0e-30) {
         double M_enc;
         if (i == 0)
-            M_enc = 0.1 * (1.0 / (double)npts) * p->halo_mass;
+            M_enc = p->menc ? 0.1 * p->menc[npts + 1] : 0.1 * (1.0 / (double)npts) * p->halo_mass;
         else
-            M_enc = ((double)i / (double)npts) * p->halo_mass;
+            M_enc = p->menc ? p->menc[i] : ((double)i / (double)npts) * p->halo_mass;
         double grav_part = (p->vel_conv_sq * p->G) * M_enc;
         r_crit = (ell * ell) * p->alpha / grav_part;
     }
@@ -649,6 +653,45 @@ void orc_step(double *r, double *v, double *ell, uint32_t *id, int64_t npts,
     orc_integrate(r, v, ell, npts, p, level);
     free(perm);
     free(tmp);
+}
+
+/* Neumaier-compensated long double sum (error ~2^-64 relative, far below the
+ * one final rounding to double); needs the strict build (no reassociation). */
+void orc_mass_prefix(const double *mass, int64_t npts, double *out)
+{
+    long double s = 0.0L, c = 0.0L;
+    for (int64_t i = 0; i < npts; i++) {
+        out[i] = (double)(s + c);
+        const long double x = (long double)mass[i];
+        const long double t = s + x;
+        if (fabsl(s) >= fabsl(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    out[npts] = (double)(s + c);
+    out[npts + 1] = npts > 0 ? mass[0] : 0.0;
+}
+
+void orc_energy_masses(const double *r, const double *v, const double *ell, const double *mass,
+                       int64_t npts, const orc_params *p, double *K, double *W)
+{
+    if (!mass) {
+        orc_energy(r, v, ell, npts, p, K, W);
+        return;
+    }
+    const long double gpre = (long double)p->vel_conv_sq * (long double)p->G;
+    long double ks = 0.0L, ws = 0.0L, menc = 0.0L;
+    for (int64_t i = 0; i < npts; i++) {
+        long double rr = r[i], vv = v[i], ll = ell[i], mi = mass[i];
+        ks += mi * 0.5L * (vv * vv + (ll * ll) / (rr * rr));
+        if (!p->identity_gravity)
+            ws -= gpre * menc * mi / rr;
+        menc += mi;
+    }
+    *K = (double)ks;
+    *W = (double)ws;
 }
 
 void orc_energy(const double *r, const double *v, const double *ell, int64_t npts,

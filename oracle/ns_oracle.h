@@ -42,6 +42,12 @@ typedef struct {
      * 2N+2 / 4N+2 for the Levi-Civita coarse / fine passes -- and checked at
      * the same points, so capped results stay comparable bit for bit. */
     int64_t budget;
+    /* Not in the reference (all its particles weigh M/npts): per-particle
+     * masses' enclosed mass by rank, menc[i] below rank i, menc[npts + 1] the
+     * rank-0 particle's mass; NULL = the reference's (rank/npts) M.  Used by
+     * the force (M_enc in place of (rank/npts) M) and by the r_crit switch
+     * (rank 0: a tenth of its own mass). */
+    const double *menc;
 } orc_params;
 
 /* Per-particle refinement record written by the step (one byte):
@@ -108,6 +114,14 @@ void orc_integrate(double *r, double *v, const double *ell, int64_t npts,
 /* Shell-model energy with the rank-based enclosed mass of nsphere.c:513:
  * K = sum m (v^2 + ell^2/r^2)/2,  W = -sum (vel_conv_sq G) (i/npts) M m / r,
  * m = M/npts, i = rank.  Summed in long double. */
+/* Enclosed mass for per-particle masses given in rank order: out[i] = the
+ * sum of mass[0..i) to about 2^-64 relative (compensated long double),
+ * rounded once to double; out[npts] = the total, out[npts + 1] = mass[0] --
+ * the layout of orc_params.menc. */
+void orc_mass_prefix(const double *mass, int64_t npts, double *out);
+/* orc_energy with per-particle masses (rank order; NULL = equal masses). */
+void orc_energy_masses(const double *r, const double *v, const double *ell, const double *mass,
+                       int64_t npts, const orc_params *p, double *K, double *W);
 void orc_energy(const double *r, const double *v, const double *ell, int64_t npts,
                 const orc_params *p, double *K, double *W);
 
