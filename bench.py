@@ -54,6 +54,8 @@ def bytes_per_particle(state_bytes: int) -> dict:
         "integrate": integrate,                                  # plain form
         "integrate_bulk": integrate,                             # binned forms: every particle's gather + write
         "radius_keys": 2 * s,
+        # read perm 4 and the mass by it; write the mass in rank order and M_enc (float64)
+        "mass_scan": 4 + 2 * s + 8,
     }
 
 
@@ -66,6 +68,14 @@ MEASURED_TRAFFIC = {("f32", 12_000_000): {"sort_scatter": 171.2e6, "sort_upsweep
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep"}
 
 
+
+# IC recipes (kind, default N, truncation radius, low-ell fraction; ell x 1e-3, dt = 1e-3): the
+# benchmark's Plummer workload, and the Hernquist and Plummer+Hernquist workloads also run by the tests
+WORKLOADS = {
+    "plummer_n1.2e7_lowl5pct_dt1e-3": (0, 12_000_000, 100.0, 0.05),
+    "hernquist_n5e7_lowl5pct_dt1e-3": (1, 50_000_000, 200.0, 0.05),
+    "mix_n1.2e7_lowl10pct_dt1e-3": (2, 12_000_000, 100.0, 0.10),
+}
 
 FORMS = {"plain": 0, "binned": 1, "binned-fast": 2}
 PRECISIONS = {"f64": 0, "f32": 1}
@@ -111,7 +121,8 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int) -> dict:
     }
 
 
-def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int) -> dict:
+def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
+            mass=None) -> dict:
     n = state[0].size
     ctx = ns.GpuContext(n)
     chunk = 10
@@ -121,6 +132,8 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
         ctx.upload(*state)
+        if mass is not None:
+            ctx.set_masses(mass)
         K0, W0 = ctx.energy(p)
         ctx.run(p, warmup)
         ctx.sync()
@@ -212,7 +225,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=12_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="plummer_n1.2e7_lowl5pct_dt1e-3")
+    ap.add_argument("--n", type=int, default=None, help="particles (default: the workload's)")
+    ap.add_argument("--masses", choices=["none", "equal", "random"], default="none",
+                    help="per-particle masses, so that M(r) comes from the enclosed-mass scan instead of "
+                         "the rank (equal: M/N each; random: uniform in [0.5, 1.5] M/N, normalised)")
     ap.add_argument("--seed", type=int, default=20250401)
     ap.add_argument("--form", choices=sorted(FORMS), default="binned-fast")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="f32",
@@ -235,9 +252,10 @@ def main() -> int:
 
     import nsphere_amd as ns
 
-    n = args.n
+    kind, n_default, rmax, lowl = WORKLOADS[args.workload]
+    n = args.n or n_default
     t_ic = time.perf_counter()
-    r, v, ell, ids = ns.generate_ic(ns.IC_PLUMMER, n, args.seed, 100.0, 0.05, 1e-3)
+    r, v, ell, ids = ns.generate_ic(kind, n, args.seed, rmax, lowl, 1e-3)
     p = ns.unit_params()
     # the reference's exact initial sort (nsphere.c:5339-5342), done once
     with ns.GpuContext(n) as ctx:
@@ -246,10 +264,17 @@ def main() -> int:
         state = ctx.download()
     print(f"[bench] N={n} initial state in {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
 
-    chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget)
+    mass = None
+    if args.masses != "none":
+        by_id = np.full(n, 1.0 / n) if args.masses == "equal" else \
+            np.random.default_rng(args.seed + 1).uniform(0.5, 1.5, n)
+        by_id /= by_id.sum()
+        mass = by_id[state[3]]  # in the order of the sorted initial state
+    chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass)
     plain = None
     if not args.no_plain and (args.form, args.precision) != ("plain", "f64"):
-        plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget)
+        plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget,
+                        mass)
 
     dom = max(chosen["stats"].items(), key=lambda kv: kv[1][1])[0]
     out = {
@@ -265,8 +290,9 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plummer_n1.2e7_lowl5pct_dt1e-3", "n": n, "dt": p.dt, "form": args.form,
-                   "state": args.precision, "lowl_frac": 0.05, "lowl_scale": 1e-3, "seed": args.seed},
+        "config": {"workload": args.workload, "n": n, "dt": p.dt, "form": args.form,
+                   "state": args.precision, "lowl_frac": lowl, "lowl_scale": 1e-3, "seed": args.seed,
+                   "masses": args.masses},
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
         "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
@@ -290,6 +316,8 @@ def main() -> int:
     if not args.no_cpu:
         print("[bench] timing CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds, args.budget)
+        if mass is not None:
+            out["cpu_baseline"]["sample"] += "; equal masses from the rank (the oracle step has no mass scan)"
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

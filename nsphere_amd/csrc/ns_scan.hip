@@ -6,11 +6,12 @@
 // has them).  Then the enclosed mass of the particle at rank i is the
 // exclusive prefix sum of the masses in rank order, computed in one pass:
 //   * a workgroup takes a tile of kScanTile masses by ticket, gathered
-//     through the sort's permutation when one is given; each thread sums 16
-//     consecutive masses in registers, then a wave-shuffle scan and a
-//     four-wave combine finish the tile -- in float64 for float64 masses, or
-//     for float32 masses in float32 (hi, lo) pairs with two-sum compensation
-//     (Knuth's TwoSum, renormalised after every addition);
+//     through the sort's permutation when one is given; each wave owns 1024
+//     consecutive masses as 16 rows of 64 (every load and store coalesced),
+//     scans each row with wave shuffles and carries the row totals, then a
+//     four-wave combine finishes the tile -- in float64 for float64 masses,
+//     or for float32 masses in float32 (hi, lo) pairs with two-sum
+//     compensation (Knuth's TwoSum, renormalised after every addition);
 //   * the tile's aggregate, as a 64-bit fixed-point integer, is published
 //     and the tile's exclusive prefix is found by decoupled lookback, one
 //     wave reading 64 predecessor tiles per round.  Integer partial sums do
@@ -67,6 +68,7 @@ struct Acc<double> {
     __device__ static T of(double m) { return m; }
     __device__ static T add(T a, T b) { return a + b; }
     __device__ static T shfl_up(T x, int d) { return __shfl_up(x, d, 64); }
+    __device__ static T shfl(T x, int l) { return __shfl(x, l, 64); }
     __device__ static double value(T x) { return x; }
     __device__ static int64_t fixed(T x, double scale) { return (int64_t)(x * scale); }
 };
@@ -78,6 +80,7 @@ struct Acc<float> {
     __device__ static T of(float m) { return {m, 0.0f}; }
     __device__ static T add(T a, T b) { return f2_add(a, b); }
     __device__ static T shfl_up(T x, int d) { return {__shfl_up(x.hi, d, 64), __shfl_up(x.lo, d, 64)}; }
+    __device__ static T shfl(T x, int l) { return {__shfl(x.hi, l, 64), __shfl(x.lo, l, 64)}; }
     __device__ static double value(T x) { return (double)x.hi + (double)x.lo; }
     __device__ static int64_t fixed(T x, double scale)
     {
@@ -123,12 +126,13 @@ __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict_
         s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t first = tile * kScanTile + (int64_t)tid * kScanItems;
+    const int64_t wave_base = tile * kScanTile + (int64_t)wave * (kScanItems * 64);
 
+    // row j of this wave: items wave_base + 64 j + lane
     M m[kScanItems];
 #pragma unroll
     for (int j = 0; j < kScanItems; j++) {
-        const int64_t idx = first + j;
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         M x = (M)0;
         if (idx < n)
             x = perm ? mass_in[perm[idx]] : mass_in[idx];
@@ -136,21 +140,27 @@ __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict_
         if (mass_out && idx < n)
             mass_out[idx] = x;
     }
-    T sum = A::zero();
+    // exclusive prefix of every item within the wave: per row a shuffle
+    // scan, plus the carried totals of the rows before
+    T loc[kScanItems];
+    T carry = A::zero();
 #pragma unroll
-    for (int j = 0; j < kScanItems; j++)
-        sum = A::add(sum, A::of(m[j]));
-
-    // wave-inclusive scan of the thread sums, then the four-wave combine
-    T incl = sum;
+    for (int j = 0; j < kScanItems; j++) {
+        const T x = A::of(m[j]);
+        T incl = x;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T y = A::shfl_up(incl, off);
-        if (lane >= off)
-            incl = A::add(y, incl);
+        for (int off = 1; off < 64; off <<= 1) {
+            const T y = A::shfl_up(incl, off);
+            if (lane >= off)
+                incl = A::add(y, incl);
+        }
+        const T ex = A::shfl_up(incl, 1);
+        loc[j] = lane == 0 ? carry : A::add(carry, ex);
+        const T row = A::shfl(incl, 63);
+        carry = A::add(carry, row);
     }
-    if (lane == 63)
-        s_wave[wave] = incl;
+    if (lane == 0)
+        s_wave[wave] = carry;
     __syncthreads();
     T before = A::zero(), total = A::zero();
 #pragma unroll
@@ -159,9 +169,6 @@ __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict_
             before = A::add(before, s_wave[w]);
         total = A::add(total, s_wave[w]);
     }
-    // exclusive prefix of this thread's first item within the tile
-    T excl = A::shfl_up(incl, 1);
-    excl = lane == 0 ? before : A::add(before, excl);
 
     // decoupled lookback on fixed-point tile aggregates (wave 0)
     if (wave == 0) {
@@ -212,19 +219,16 @@ __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict_
     __syncthreads();
     const double tile_base = (double)s_prefix / scale;
 
-    T run = excl;
 #pragma unroll
     for (int j = 0; j < kScanItems; j++) {
-        const int64_t idx = first + j;
+        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
         if (idx < n)
-            menc[idx] = tile_base + A::value(run);
-        run = A::add(run, A::of(m[j]));
+            menc[idx] = tile_base + A::value(A::add(before, loc[j]));
+        if (idx == n - 1)  // the total, after the last item
+            menc[n] = tile_base + A::value(A::add(A::add(before, loc[j]), A::of(m[j])));
     }
     if (tile == 0 && tid == 0)
         menc[n + 1] = (double)m[0];
-    if (first < n && first + kScanItems >= n) {  // the thread holding the last item
-        menc[n] = tile_base + A::value(run);
-    }
 }
 
 __global__ void k_zero_ticket(uint32_t *t)
