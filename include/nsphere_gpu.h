@@ -156,6 +156,17 @@ int ns_gpu_cap_hits(ns_gpu_ctx *ctx, uint64_t *count, int reset);
  * (i/npts) M, r_crit of rank 0 uses a tenth of its own mass, and the energy
  * uses the masses.  NULL returns to equal masses. */
 int ns_gpu_set_masses(ns_gpu_ctx *ctx, const double *mass);
+/* An ensemble of independent halos in one context (no reference
+ * counterpart: the reference evolves one halo per run).  Particles
+ * [o_h, o_h + counts[h]) of the uploaded state, o_h = counts[0] + ... +
+ * counts[h-1], form halo h of total mass halo_mass[h] (the step parameters'
+ * halo_mass is then not used); every step sorts each halo by radius on its
+ * own (one segmented radix sort over all halos) and integrates each particle
+ * with its rank and count within its halo, so that halo h evolves exactly as
+ * a context holding only that halo would.  nhalos = 0 or counts = NULL ends
+ * the ensemble.  Needs NS_OPT_SORT_ALG 0; excludes per-particle masses. */
+int ns_gpu_set_halos(ns_gpu_ctx *ctx, int nhalos, const int64_t *counts, const double *halo_mass);
+
 /* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
  * [npts] the total, [npts + 1] the rank-0 particle's mass. */
 int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);

@@ -202,6 +202,18 @@ class GpuContext:
             raise ValueError(f"mass must have shape ({self.npts},)")
         check(lib.ns_gpu_set_masses(self._h, m.ctypes.data))
 
+    def set_halos(self, counts, halo_mass=None) -> None:
+        """Ensemble: consecutive runs of counts[h] particles are independent
+        halos of mass halo_mass[h] (default 1 each); None ends the ensemble."""
+        if counts is None:
+            check(lib.ns_gpu_set_halos(self._h, 0, None, None))
+            return
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        m = np.ones(c.size) if halo_mass is None else np.ascontiguousarray(halo_mass, dtype=np.float64)
+        if m.shape != c.shape:
+            raise ValueError("halo_mass must match counts")
+        check(lib.ns_gpu_set_halos(self._h, int(c.size), c.ctypes.data, m.ctypes.data))
+
     def enclosed_mass(self) -> np.ndarray:
         """M_enc of the last scan: [i] below rank i, [n] total, [n + 1] rank-0 mass."""
         out = np.empty(self.npts + 2, dtype=np.float64)

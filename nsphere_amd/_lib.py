@@ -94,6 +94,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_kernel_stats_reset": (i32, [vp]),
         "ns_gpu_force_evals": (i32, [vp, c.POINTER(c.c_uint64)]),
         "ns_gpu_set_masses": (i32, [vp, vp]),
+        "ns_gpu_set_halos": (i32, [vp, i32, vp, vp]),
         "ns_gpu_enclosed_mass": (i32, [vp, vp]),
         "ns_ic_generate": (i32, [i32, i64, c.c_uint64, dbl, dbl, dbl, vp, vp, vp, vp]),
         "ns_philox4x32_10": (None, [c.POINTER(c.c_uint32), c.POINTER(c.c_uint32), c.POINTER(c.c_uint32)]),

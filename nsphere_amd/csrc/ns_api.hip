@@ -43,6 +43,10 @@ struct ns_gpu_ctx {
     double *menc = nullptr;
     bool has_mass = false;
     ns::MassScan ms = {};
+    // ensemble of halos (nseg > 0): halo h holds ranks [seg_off[h], seg_off[h+1])
+    int nseg = 0;
+    int64_t *seg_off = nullptr;
+    double *seg_mass = nullptr;
     unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
     int64_t work_stride = 0;
     uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
@@ -215,12 +219,13 @@ hipError_t dalloc(T **p, int64_t count)
 
 void free_all(ns_gpu_ctx *c)
 {
+    ns::seg_table_release(c->sw.seg);
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
-                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ticket};
+                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ticket, c->seg_off, c->seg_mass};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -299,6 +304,9 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
     a.work = c->prof.on ? c->work : nullptr;
     a.menc = c->has_mass ? c->menc : nullptr;
+    a.nseg = c->nseg;
+    a.seg_off = c->seg_off;
+    a.seg_mass = c->seg_mass;
     a.work_stride = c->work_stride;
     return a;
 }
@@ -558,6 +566,8 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         cfg.alg = (int)value;
         if (value < 0 || value > 1 || !ns::sort_config_supported(cfg))
             return set_error(NS_ERR_ARG, "NS_OPT_SORT_ALG takes 0 (reduce-then-scan) or 1 (onesweep)");
+        if (value == ns::kSortOnesweep && c->nseg > 0)
+            return set_error(NS_ERR_STATE, "NS_OPT_SORT_ALG: an ensemble sorts by segments, reduce-then-scan only");
         c->sw.config = cfg;
         return NS_OK;
     }
@@ -842,12 +852,14 @@ int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
     const int s = c->cur;
     if (c->f32)
         HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], c->perm, c->has_mass ? c->mf[s] : nullptr,
-                                         menc, c->n, m, p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity,
-                                         c->energy_partial, c->energy_out, c->stream));
+                                         menc, c->seg_off, c->seg_mass, c->nseg, c->n, m, p->vel_conv_sq * p->G,
+                                         p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
+                                         c->stream));
     else
         HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], c->perm, c->has_mass ? c->md[s] : nullptr,
-                                          menc, c->n, m, p->vel_conv_sq * p->G, p->halo_mass, p->identity_gravity,
-                                          c->energy_partial, c->energy_out, c->stream));
+                                          menc, c->seg_off, c->seg_mass, c->nseg, c->n, m, p->vel_conv_sq * p->G,
+                                          p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
+                                          c->stream));
     c->prof.end(ns::K_ENERGY);
     double out[2];
     HIP_TRY(hipMemcpyAsync(out, c->energy_out, sizeof out, hipMemcpyDeviceToHost, c->stream));
@@ -893,6 +905,8 @@ int ns_gpu_set_masses(ns_gpu_ctx *c, const double *mass)
         c->has_mass = false;
         return NS_OK;
     }
+    if (c->nseg > 0)
+        return set_error(NS_ERR_STATE, "ns_gpu_set_masses: per-particle masses and halos do not combine");
     if (c->n == 0)
         return NS_OK;
     long double total = 0.0L;
@@ -918,6 +932,65 @@ int ns_gpu_set_masses(ns_gpu_ctx *c, const double *mass)
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     c->has_mass = true;
+    return NS_OK;
+}
+
+int ns_gpu_set_halos(ns_gpu_ctx *c, int nhalos, const int64_t *counts, const double *halo_mass)
+{
+    if (int rc = check_ctx(c, "ns_gpu_set_halos"))
+        return rc;
+    DeviceGuard g(c->device);
+    if (nhalos == 0 || !counts) {
+        c->nseg = 0;
+        c->sw.seg.sizes.clear();
+        return NS_OK;
+    }
+    if (nhalos < 0 || nhalos > (1 << 20) || !halo_mass)
+        return set_error(NS_ERR_ARG, "ns_gpu_set_halos: nhalos %d out of range [1, 2^20] or no masses", nhalos);
+    if (c->has_mass)
+        return set_error(NS_ERR_STATE, "ns_gpu_set_halos: per-particle masses and halos do not combine");
+    if (c->sw.config.alg != ns::kSortReduceScan)
+        return set_error(NS_ERR_STATE, "ns_gpu_set_halos: needs the reduce-then-scan sort (NS_OPT_SORT_ALG 0)");
+    std::vector<int64_t> off(nhalos + 1, 0);
+    for (int h = 0; h < nhalos; h++) {
+        if (counts[h] < 1)
+            return set_error(NS_ERR_ARG, "ns_gpu_set_halos: halo %d has %lld particles", h, (long long)counts[h]);
+        if (!(halo_mass[h] > 0.0) || !std::isfinite(halo_mass[h]))
+            return set_error(NS_ERR_ARG, "ns_gpu_set_halos: halo %d mass %g", h, halo_mass[h]);
+        off[h + 1] = off[h] + counts[h];
+    }
+    if (off[nhalos] != c->n)
+        return set_error(NS_ERR_ARG, "ns_gpu_set_halos: counts add up to %lld, not npts %lld",
+                         (long long)off[nhalos], (long long)c->n);
+    // device halo tables and the sort's segment table (sized for the
+    // smallest supported tile, kSortTile keys)
+    ns::seg_table_release(c->sw.seg);
+    if (c->seg_off)
+        (void)hipFree(c->seg_off);
+    if (c->seg_mass)
+        (void)hipFree(c->seg_mass);
+    c->seg_off = nullptr;
+    c->seg_mass = nullptr;
+    c->nseg = 0;
+    ns::SegTable &t = c->sw.seg;
+    t.sizes.assign(counts, counts + nhalos);
+    t.capacity = ns::sort_tiles(c->n) + nhalos + 1;
+    hipError_t e = dalloc(&c->seg_off, nhalos + 1);
+    if (e == hipSuccess) e = dalloc(&c->seg_mass, nhalos);
+    if (e == hipSuccess) e = dalloc(&t.tile_start, t.capacity + 1);
+    if (e == hipSuccess) e = dalloc(&t.tile_seg, t.capacity);
+    if (e == hipSuccess) e = dalloc(&t.seg_first, nhalos + 1);
+    if (e == hipSuccess) e = dalloc(&t.seg_start, nhalos + 1);
+    if (e == hipSuccess) e = dalloc(&t.base, (int64_t)nhalos * ns::kRadix);
+    if (e == hipSuccess) e = dalloc(&t.counts, t.capacity * ns::kRadix);
+    if (e == hipSuccess) e = hipMemcpy(c->seg_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->seg_mass, halo_mass, (size_t)nhalos * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        t.sizes.clear();
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "ns_gpu_set_halos: %s",
+                         hipGetErrorString(e));
+    }
+    c->nseg = nhalos;
     return NS_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "nsphere_gpu.h"
 
@@ -94,6 +95,32 @@ struct SortConfig {
 };
 bool sort_config_supported(SortConfig c);
 
+// Segmented sort: the tile table handed to the reduce-then-scan kernels.
+// Tiles never straddle a segment boundary; tile t covers
+// [tile_start[t], tile_start[t + 1]) of segment tile_seg[t], and
+// base[s * kRadix + d] turns a tile's scanned digit count into its global
+// offset (k_seg_bases).  All null: one unsegmented array.
+struct SegView {
+    const int64_t *tile_start = nullptr;
+    const uint32_t *tile_seg = nullptr;
+    const uint32_t *base = nullptr;
+};
+
+// Segments of the sort (halos of an ensemble): sizes on the host, the tile
+// table of the last tile size on the device.
+struct SegTable {
+    std::vector<int64_t> sizes;     // particles per segment (empty: unsegmented)
+    int tile = 0;                   // tile size the device table was built for
+    int64_t tiles = 0;
+    int64_t *tile_start = nullptr;  // tiles + 1
+    uint32_t *tile_seg = nullptr;   // tiles
+    uint32_t *seg_first = nullptr;  // nseg + 1: first tile of each segment
+    uint32_t *seg_start = nullptr;  // nseg + 1: first element of each segment
+    uint32_t *base = nullptr;       // nseg * kRadix
+    uint32_t *counts = nullptr;     // tiles * kRadix (32-bit tile counts)
+    int64_t capacity = 0;           // tiles the device table can hold
+};
+
 // Device workspace of one radix sort.  Key buffers hold 64-bit keys, or
 // 32-bit keys in their first half.
 struct SortWork {
@@ -106,7 +133,10 @@ struct SortWork {
                          // the reduce-then-scan tile counts (32-bit)
     uint32_t *counters;  // kKeyPasses tile tickets (+ spare)
     uint32_t epoch;      // lookback epoch of the next pass (host-owned)
+    SegTable seg;        // segments (reduce-then-scan only)
 };
+// Frees the device part of a segment table.
+void seg_table_release(SegTable &t);
 
 // Launchers (ns_sort.hip).  All asynchronous on `stream`.
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream);

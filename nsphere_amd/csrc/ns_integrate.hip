@@ -245,8 +245,9 @@ constexpr int kEnergyThreads = 256;
 template <typename S>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
     const S *__restrict__ r, const S *__restrict__ v, const S *__restrict__ ell, const uint32_t *__restrict__ perm,
-    const S *__restrict__ mass, const double *__restrict__ menc, int64_t n, double m, double gpre, double halo_mass,
-    int idg, double *__restrict__ partial)
+    const S *__restrict__ mass, const double *__restrict__ menc, const int64_t *__restrict__ seg_off,
+    const double *__restrict__ seg_mass, int nseg, int64_t n, double m, double gpre, double halo_mass, int idg,
+    double *__restrict__ partial)
 {
     __shared__ double sk[kEnergyThreads], sw[kEnergyThreads];
     double k = 0.0, w = 0.0;
@@ -254,10 +255,26 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(
     for (int64_t i = (int64_t)blockIdx.x * kEnergyThreads + threadIdx.x; i < n; i += stride) {
         const int64_t p = perm ? (int64_t)perm[i] : i;
         const double rr = r[p], vv = v[p], ll = ell[p];
-        const double mi = mass ? (double)mass[p] : m;
+        int64_t il = i, nl = n;  // the rank and count of i's halo
+        double Ml = halo_mass, ml = m;
+        if (nseg > 0) {
+            int lo = 0, hi = nseg;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (seg_off[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            il = i - seg_off[lo];
+            nl = seg_off[lo + 1] - seg_off[lo];
+            Ml = seg_mass[lo];
+            ml = Ml / (double)nl;
+        }
+        const double mi = mass ? (double)mass[p] : ml;
         k += mi * 0.5 * (vv * vv + (ll * ll) / (rr * rr));
         if (!idg) {
-            const double me = menc ? menc[i] : ((double)i / (double)n) * halo_mass;
+            const double me = menc ? menc[i] : ((double)il / (double)nl) * Ml;
             w -= gpre * me * mi / rr;
         }
     }
@@ -399,12 +416,13 @@ int energy_blocks(int64_t n)
 
 template <typename S>
 hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, const S *mass,
-                         const double *menc, int64_t n, double m, double gpre, double halo_mass, int idg,
-                         double *partial, double *out, hipStream_t stream)
+                         const double *menc, const int64_t *seg_off, const double *seg_mass, int nseg, int64_t n,
+                         double m, double gpre, double halo_mass, int idg, double *partial, double *out,
+                         hipStream_t stream)
 {
     const int blocks = energy_blocks(n);
     hipLaunchKernelGGL(k_energy_partial<S>, dim3(blocks), dim3(kEnergyThreads), 0, stream, r, v, ell, perm, mass,
-                       menc, n, m, gpre, halo_mass, idg, partial);
+                       menc, seg_off, seg_mass, nseg, n, m, gpre, halo_mass, idg, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
@@ -412,10 +430,10 @@ hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *p
     return hipGetLastError();
 }
 template hipError_t launch_energy<double>(const double *, const double *, const double *, const uint32_t *,
-                                          const double *, const double *, int64_t, double, double, double, int,
-                                          double *, double *, hipStream_t);
+                                          const double *, const double *, const int64_t *, const double *, int,
+                                          int64_t, double, double, double, int, double *, double *, hipStream_t);
 template hipError_t launch_energy<float>(const float *, const float *, const float *, const uint32_t *,
-                                         const float *, const double *, int64_t, double, double, double, int,
-                                         double *, double *, hipStream_t);
+                                         const float *, const double *, const int64_t *, const double *, int,
+                                         int64_t, double, double, double, int, double *, double *, hipStream_t);
 
 } // namespace ns

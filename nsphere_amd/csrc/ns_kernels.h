@@ -25,6 +25,11 @@ struct StepArgs {
     int64_t budget;              // force evaluations allowed per particle per step
     int32_t out_type, N_taumin, identity_gravity;
     // binned integrator: queues of ranks handed between its kernels
+    // ensembles (nseg > 0): ranks [seg_off[h], seg_off[h + 1]) are halo h, of
+    // total mass seg_mass[h]; rank and npts in the force are the halo's own
+    const int64_t *seg_off;
+    const double *seg_mass;
+    int32_t nseg;
     // per-particle masses (else null: M_enc from the rank, as the reference):
     // menc[i] = mass enclosed below rank i, menc[n + 1] = mass of rank 0
     const double *menc;
@@ -79,11 +84,13 @@ hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream);
 int energy_blocks(int64_t n);
 // mass (rank order via perm, may be null: equal masses m) and menc (may be
-// null: M_enc from the rank) select the per-particle-mass energy.
+// null: M_enc from the rank) select the per-particle-mass energy; seg_off /
+// seg_mass (nseg > 0) an ensemble of halos, each with its own ranks and mass.
 template <typename S>
 hipError_t launch_energy(const S *r, const S *v, const S *ell, const uint32_t *perm, const S *mass,
-                         const double *menc, int64_t n, double m, double gpre, double halo_mass, int idg,
-                         double *partial, double *out, hipStream_t stream);
+                         const double *menc, const int64_t *seg_off, const double *seg_mass, int nseg, int64_t n,
+                         double m, double gpre, double halo_mass, int idg, double *partial, double *out,
+                         hipStream_t stream);
 // Element-wise precision conversion (float64 <-> float32, round to nearest).
 template <typename D, typename S>
 hipError_t launch_convert(const S *src, D *dst, int64_t n, hipStream_t stream);

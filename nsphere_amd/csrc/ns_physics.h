@@ -39,20 +39,44 @@ struct Phys {
     bool idg;   // identity gravity (nsphere.c:476)
 };
 
+// The rank, particle count and total mass the force sees at global rank i:
+// the whole array, or in an ensemble the halo holding rank i (a binary search
+// over the halo offsets).
+struct RankInfo {
+    int64_t i, n;
+    double M;
+};
+
+__device__ inline RankInfo rank_info(const StepArgs &a, int64_t i)
+{
+    if (a.nseg <= 0)
+        return {i, a.n, a.halo_mass};
+    int lo = 0, hi = a.nseg;  // seg_off[0] = 0 <= i < seg_off[nseg] = n
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.seg_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return {i - a.seg_off[lo], a.seg_off[lo + 1] - a.seg_off[lo], a.seg_mass[lo]};
+}
+
 // Enclosed mass at rank i: the reference's (i / npts) M (nsphere.c:513), or
 // with per-particle masses the scanned sum of the masses below rank i.
-__device__ inline double enclosed_mass(const StepArgs &a, int64_t i)
+__device__ inline double enclosed_mass(const StepArgs &a, const RankInfo &k)
 {
-    return a.menc ? a.menc[i] : ((double)i / (double)a.n) * a.halo_mass;
+    return a.menc ? a.menc[k.i] : ((double)k.i / (double)k.n) * k.M;
 }
 
 __device__ inline Phys make_phys(const StepArgs &a, int64_t i, double ell)
 {
     Phys ph;
     ph.idg = a.identity_gravity != 0;
+    const RankInfo k = rank_info(a, i);
     // the reference's association, ((-(vel_conv_sq G)) (i / npts)) M
     ph.gm = a.menc ? -(a.vel_conv_sq * a.G) * a.menc[i]
-                   : -(a.vel_conv_sq * a.G) * ((double)i / (double)a.n) * a.halo_mass;
+                   : -(a.vel_conv_sq * a.G) * ((double)k.i / (double)k.n) * k.M;
     ph.l2 = ell * ell;
     return ph;
 }
@@ -63,11 +87,12 @@ __device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r,
     double r_crit = 0.0;
     if (fabs(ell) > 1.0e-30) {
         // rank 0: a tenth of one particle's mass (with masses: of its own)
+        const RankInfo k = rank_info(a, i);
         double M_enc;
-        if (i == 0)
-            M_enc = a.menc ? 0.1 * a.menc[a.n + 1] : 0.1 * (1.0 / (double)a.n) * a.halo_mass;
+        if (k.i == 0)
+            M_enc = a.menc ? 0.1 * a.menc[a.n + 1] : 0.1 * (1.0 / (double)k.n) * k.M;
         else
-            M_enc = enclosed_mass(a, i);
+            M_enc = enclosed_mass(a, k);
         const double grav_part = (a.vel_conv_sq * a.G) * M_enc;
         r_crit = (ell * ell) * a.alpha / grav_part;
     }

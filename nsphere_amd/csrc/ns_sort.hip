@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__r
 // ones; other waves add per lane.
 template <typename KeyT, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict__ keys, int64_t n, int shift,
-                                                         uint32_t *__restrict__ counts, int64_t tiles)
+                                                         uint32_t *__restrict__ counts, int64_t tiles,
+                                                         const int64_t *__restrict__ tile_start)
 {
     constexpr int WAVES = THREADS / 64;
     __shared__ uint32_t s_cnt[WAVES][kRadix];
@@ -211,17 +212,24 @@ __global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict_
     __syncthreads();
 
     const int64_t tile = blockIdx.x;
-    const int64_t wave_base = tile * (THREADS * ITEMS) + (int64_t)wave * (ITEMS * 64);
+    int64_t start = tile * (THREADS * ITEMS), end = start + THREADS * ITEMS;
+    if (tile_start) {  // segmented: tiles never straddle a segment boundary
+        start = tile_start[tile];
+        end = tile_start[tile + 1];
+    }
+    if (end > n)
+        end = n;
+    const int64_t wave_base = start + (int64_t)wave * (ITEMS * 64);
     KeyT key[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        key[j] = idx < n ? keys[idx] : (KeyT)0;
+        key[j] = idx < end ? keys[idx] : (KeyT)0;
     }
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         // valid lanes are a prefix of the wave (lane 0 first)
-        const bool valid = wave_base + (int64_t)j * 64 + lane < n;
+        const bool valid = wave_base + (int64_t)j * 64 + lane < end;
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // whole wave: no divergence yet
@@ -299,6 +307,26 @@ __global__ __launch_bounds__(kScanThreads) void k_rts_scan(uint32_t *__restrict_
         totals[d] = carry;
 }
 
+// Segmented sort, after k_rts_scan: for segment s (one workgroup, a thread
+// per digit) the digit counts of its tiles are the scanned column values from
+// its first tile to the next segment's first; base[s][d] = segment start +
+// the segment's exclusive digit prefix - the scanned value at its first tile,
+// so that base[s][d] + scanned[t][d] is the global offset of tile t's digit d.
+__global__ __launch_bounds__(kRadix) void k_seg_bases(const uint32_t *__restrict__ scanned,
+                                                       const uint32_t *__restrict__ totals,
+                                                       const uint32_t *__restrict__ seg_first,
+                                                       const uint32_t *__restrict__ seg_start, int nseg,
+                                                       uint32_t *__restrict__ seg_base)
+{
+    __shared__ uint32_t s_wave[4];
+    const int s = blockIdx.x, d = threadIdx.x;
+    const uint32_t t0 = seg_first[s], t1 = seg_first[s + 1];
+    const uint32_t first = scanned[(int64_t)t0 * kRadix + d];
+    const uint32_t next = (s + 1 < nseg) ? scanned[(int64_t)t1 * kRadix + d] : totals[d];
+    const uint32_t ex = digit_exclusive_scan(next - first, s_wave);
+    seg_base[(int64_t)s * kRadix + d] = seg_start[s] + ex - first;
+}
+
 // One scatter pass over digit `shift`.  FIRST: payload = input position.
 // LAST: keys are not written.  LOOKBACK (onesweep): tiles are taken by
 // ticket and each tile's global digit offsets come from decoupled lookback
@@ -310,7 +338,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     const KeyT *__restrict__ keys_in, const uint32_t *__restrict__ vals_in, KeyT *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, const uint32_t *__restrict__ hist_pass,
     uint64_t *__restrict__ lookback, uint32_t *__restrict__ tile_counter, uint32_t epoch,
-    const uint32_t *__restrict__ tile_offsets, int64_t tiles)
+    const uint32_t *__restrict__ tile_offsets, int64_t tiles, SegView seg)
 {
     constexpr int WAVES = THREADS / 64;
     constexpr int TILE = THREADS * ITEMS;
@@ -332,14 +360,21 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     for (int i = tid; i < WAVES * kRadix; i += THREADS)
         (&s_wcnt[0][0])[i] = 0;
     {
-        const uint32_t b = digit_exclusive_scan(tid < kRadix ? hist_pass[tid] : 0u, s_wave);
+        // segmented: the digit bases are in seg.base, per segment
+        const uint32_t b = digit_exclusive_scan((tid < kRadix && !seg.tile_start) ? hist_pass[tid] : 0u, s_wave);
         if (tid < kRadix)
             s_dbase[tid] = b;
     }
     __syncthreads();
 
     const uint32_t tile = s_tile;
-    const int64_t tile_base = (int64_t)tile * TILE;
+    int64_t tile_base = (int64_t)tile * TILE, tile_end = tile_base + TILE;
+    if (seg.tile_start) {
+        tile_base = seg.tile_start[tile];
+        tile_end = seg.tile_start[tile + 1];
+    }
+    if (tile_end > n)
+        tile_end = n;
     const int64_t wave_base = tile_base + (int64_t)wave * (ITEMS * 64);
 
     KeyT key[ITEMS];
@@ -348,7 +383,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        const bool valid = idx < n;
+        const bool valid = idx < tile_end;
         key[j] = valid ? keys_in[idx] : (KeyT)~(KeyT)0;
         if (FIRST)
             val[j] = (uint32_t)idx;
@@ -362,7 +397,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        const bool valid = idx < n;
+        const bool valid = idx < tile_end;
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
         const uint64_t peers = match_digit(d, __ballot(valid));
         const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
@@ -387,6 +422,8 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
         }
         if (!LOOKBACK) {
             excl = tile_offsets[(int64_t)tile * kRadix + d];
+            if (seg.tile_start)
+                excl += seg.base[(int64_t)seg.tile_seg[tile] * kRadix + d];
         } else if (tile == 0) {
             lb_store(lookback + d, lb_pack(epoch, kStatusPrefix, total));
         } else {
@@ -423,15 +460,14 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
         pos[j] += s_lstart[d] + s_wcnt[wave][d];  // slot in the tile, digit order
     }
-    const int64_t remain = n - tile_base;
-    const int count = remain < TILE ? (int)remain : TILE;
+    const int count = (int)(tile_end - tile_base);
 
     if (!LAST) {
         // keys through LDS in digit order, then out in coalesced digit runs
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-            if (idx < n) {
+            if (idx < tile_end) {
                 s_stage[pos[j]] = key[j];
                 s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
             }
@@ -445,7 +481,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        if (idx < n) {
+        if (idx < tile_end) {
             s_pay[pos[j]] = val[j];
             if (LAST)
                 s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
@@ -473,35 +509,46 @@ hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int 
             prof->begin(K_SORT_PASS);
         hipLaunchKernelGGL((k_radix_scatter<KeyT, THREADS, ITEMS, FIRST, LAST, true>), dim3((unsigned)tiles),
                            dim3(THREADS), 0, stream, kin, vin, kout, vals_out, n, shift, w.hist + pass * kRadix,
-                           w.lookback, w.counters + pass, epoch, nullptr, tiles);
+                           w.lookback, w.counters + pass, epoch, nullptr, tiles, SegView{});
         if (prof)
             prof->end(K_SORT_PASS);
         return hipGetLastError();
     }
     // reduce-then-scan: the tile counts live in the lookback buffer (which
-    // holds at least sort_tiles(n) * kRadix 64-bit words), the digit totals
-    // in the first kRadix words of hist.
-    uint32_t *counts = reinterpret_cast<uint32_t *>(w.lookback);
+    // holds at least sort_tiles(n) * kRadix 64-bit words) or, segmented, in
+    // the segment table; the digit totals in the first kRadix words of hist.
+    const bool segmented = !w.seg.sizes.empty();
+    const int64_t ntiles = segmented ? w.seg.tiles : tiles;
+    uint32_t *counts = segmented ? w.seg.counts : reinterpret_cast<uint32_t *>(w.lookback);
+    SegView sv;
+    if (segmented) {
+        sv.tile_start = w.seg.tile_start;
+        sv.tile_seg = w.seg.tile_seg;
+        sv.base = w.seg.base;
+    }
     if (prof)
         prof->begin(K_SORT_UPSWEEP);
-    hipLaunchKernelGGL((k_rts_upsweep<KeyT, THREADS, ITEMS>), dim3((unsigned)tiles), dim3(THREADS), 0, stream, kin,
-                       n, shift, counts, tiles);
+    hipLaunchKernelGGL((k_rts_upsweep<KeyT, THREADS, ITEMS>), dim3((unsigned)ntiles), dim3(THREADS), 0, stream,
+                       kin, n, shift, counts, ntiles, sv.tile_start);
     if (prof)
         prof->end(K_SORT_UPSWEEP);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if (prof)
         prof->begin(K_SORT_SCAN);
-    hipLaunchKernelGGL(k_rts_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, tiles, w.hist);
+    hipLaunchKernelGGL(k_rts_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, ntiles, w.hist);
+    if (segmented)
+        hipLaunchKernelGGL(k_seg_bases, dim3((unsigned)w.seg.sizes.size()), dim3(kRadix), 0, stream, counts, w.hist,
+                           w.seg.seg_first, w.seg.seg_start, (int)w.seg.sizes.size(), w.seg.base);
     if (prof)
         prof->end(K_SORT_SCAN);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if (prof)
         prof->begin(K_SORT_PASS);
-    hipLaunchKernelGGL((k_radix_scatter<KeyT, THREADS, ITEMS, FIRST, LAST, false>), dim3((unsigned)tiles),
+    hipLaunchKernelGGL((k_radix_scatter<KeyT, THREADS, ITEMS, FIRST, LAST, false>), dim3((unsigned)ntiles),
                        dim3(THREADS), 0, stream, kin, vin, kout, vals_out, n, shift, w.hist, nullptr, nullptr, 0u,
-                       counts, tiles);
+                       counts, ntiles, sv);
     if (prof)
         prof->end(K_SORT_PASS);
     return hipGetLastError();
@@ -532,11 +579,59 @@ hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, 
     return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof);
 }
 
+// (Re)builds the device tile table of the segments for tile size `tile`:
+// every segment is cut into ceil(size / tile) tiles of its own.
+hipError_t seg_table_build(SegTable &t, int tile, int64_t n)
+{
+    const int nseg = (int)t.sizes.size();
+    std::vector<int64_t> start;
+    std::vector<uint32_t> tseg, first(nseg + 1), sstart(nseg + 1);
+    int64_t pos = 0;
+    for (int s = 0; s < nseg; s++) {
+        first[s] = (uint32_t)tseg.size();
+        sstart[s] = (uint32_t)pos;
+        for (int64_t k = 0; k < t.sizes[s]; k += tile) {
+            start.push_back(pos + k);
+            tseg.push_back((uint32_t)s);
+        }
+        pos += t.sizes[s];
+    }
+    first[nseg] = (uint32_t)tseg.size();
+    sstart[nseg] = (uint32_t)pos;
+    start.push_back(n);
+    const int64_t tiles = (int64_t)tseg.size();
+    if (pos != n || tiles > t.capacity)
+        return hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = hipMemcpy(t.tile_start, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return e;
+    if ((e = hipMemcpy(t.tile_seg, tseg.data(), tseg.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return e;
+    if ((e = hipMemcpy(t.seg_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return e;
+    if ((e = hipMemcpy(t.seg_start, sstart.data(), sstart.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return e;
+    t.tiles = tiles;
+    t.tile = tile;
+    return hipSuccess;
+}
+
 template <typename KeyT>
 hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
 {
     constexpr int kPasses = (int)sizeof(KeyT);
     hipError_t e;
+    if (!w.seg.sizes.empty()) {
+        if (w.config.alg != kSortReduceScan)
+            return hipErrorNotSupported;
+        const int tile = w.config.threads * w.config.items;
+        if (w.seg.tile != tile && (e = seg_table_build(w.seg, tile, n)) != hipSuccess)
+            return e;
+    }
     if (w.config.alg == kSortOnesweep) {
         // The epoch lives in 30 bits of the lookback word; wrap by clearing
         // (a fill kernel, not a memset node, so that the sort stays
@@ -583,6 +678,18 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
 }
 
 } // namespace
+
+void seg_table_release(SegTable &t)
+{
+    void *bufs[] = {t.tile_start, t.tile_seg, t.seg_first, t.seg_start, t.base, t.counts};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    t.tile_start = nullptr;
+    t.tile_seg = t.seg_first = t.seg_start = t.base = t.counts = nullptr;
+    t.capacity = t.tiles = 0;
+    t.tile = 0;
+}
 
 bool sort_config_supported(SortConfig c)
 {

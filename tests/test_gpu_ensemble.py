@@ -1,0 +1,151 @@
+"""Ensembles of independent halos in one context (ns_gpu_set_halos).
+
+Each halo must evolve exactly as it would alone: the segmented radix sort
+gives every halo the oracle's stable permutation of its own radii, and every
+particle is integrated with its rank, count and mass within its halo, so the
+strict forms equal the oracle run halo by halo, bit for bit, and the fast
+form equals a context holding only that halo.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(256, 16), (512, 12), (256, 24)]
+
+
+@pytest.fixture(scope="module")
+def ns():
+    import nsphere_amd
+    return nsphere_amd
+
+
+def assert_bits(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype == np.float64:
+        bad = np.flatnonzero(a.view(np.uint64) != b.view(np.uint64))
+    else:
+        bad = np.flatnonzero(a != b)
+    assert bad.size == 0, f"{what}: {bad.size} of {a.size} differ, first at {bad[0]}: {a[bad[0]]!r} vs {b[bad[0]]!r}"
+
+
+def ensemble(ns, sizes, seed, lowl=0.02):
+    """Halos alternating Plummer / Hernquist, each sorted by radius, ids unique."""
+    parts = []
+    base = 0
+    for h, n in enumerate(sizes):
+        kind = h % 2
+        r, v, ell, ids = ns.generate_ic(kind, n, seed + h, 100.0 if kind == 0 else 200.0, lowl, 1e-3)
+        q = oracle.sort_perm(r)
+        parts.append((r[q], v[q], ell[q], ids[q] + base))
+        base += n
+    return [np.concatenate([p[k] for p in parts]) for k in range(4)]
+
+
+@pytest.mark.parametrize("threads,items", TILES)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_segmented_sort_matches_oracle_per_halo(ns, threads, items, precision):
+    from nsphere_amd import _lib
+    sizes = [1, 63, 4096, 4097, 6144, 10_000, 33_333, 2, 70_001]
+    n = sum(sizes)
+    rng = np.random.default_rng(threads + items + precision)
+    r = np.concatenate([np.sort(rng.uniform(0, 10, s)) + rng.normal(0, 1e-2, s) for s in sizes])
+    r[rng.integers(0, n, n // 40)] = 2.5  # ties across and within halos
+    if precision:
+        r = r.astype(np.float32).astype(np.float64)
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, precision)
+        ctx.set_option(_lib.NS_OPT_SORT_TILE, (threads << 16) | items)
+        ctx.set_halos(sizes)
+        ctx.upload(r, np.zeros(n), np.zeros(n), np.arange(n, dtype=np.uint32))
+        ctx.sort_particles()
+        _, _, _, gid = ctx.download()
+    off = 0
+    for h, s in enumerate(sizes):
+        want = oracle.sort_perm(r[off:off + s]) + off
+        assert_bits(gid[off:off + s], want.astype(np.uint32), f"halo {h}")
+        off += s
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_ensemble_steps_equal_oracle_per_halo(ns, form):
+    sizes = [1000, 23_457, 4096, 51_000, 777, 12_345]
+    masses = [1.0, 0.5, 2.0, 1.0, 3.0, 0.25]
+    r, v, ell, ids = ensemble(ns, sizes, 500)
+    n = r.size
+    p = ns.unit_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
+        ctx.set_halos(sizes, masses)
+        ctx.upload(r, v, ell, ids)
+        for s in range(3):
+            ctx.step(p)
+        gr, gv, gl, gid = ctx.download()
+        glv = ctx.levels()
+        K, W = ctx.energy(p)
+    off = 0
+    k_sum = w_sum = 0.0
+    for h, (s, m) in enumerate(zip(sizes, masses)):
+        hr, hv, hl = (x[off:off + s].copy() for x in (r, v, ell))
+        hi = (ids[off:off + s] - off).astype(np.uint32)  # the oracle's ids index its own halo
+        po = oracle.from_step_params(p)
+        po.halo_mass = m
+        for _ in range(3):
+            lv, _ = oracle.step(hr, hv, hl, hi, po)
+        assert_bits(gid[off:off + s], hi + np.uint32(off), f"halo {h} id")
+        assert_bits(gr[off:off + s], hr, f"halo {h} r")
+        assert_bits(gv[off:off + s], hv, f"halo {h} v")
+        assert_bits(glv[off:off + s], lv, f"halo {h} levels")
+        q = oracle.sort_perm(hr)
+        k, w = oracle.energy(hr[q], hv[q], hl[q], po)
+        k_sum += k
+        w_sum += w
+        off += s
+    assert abs(K - k_sum) <= 1e-12 * abs(k_sum)
+    assert abs(W - w_sum) <= 1e-12 * abs(w_sum)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ensemble_equals_halos_alone_fast_form(ns, precision):
+    sizes = [30_000, 5_000, 64_000, 999]
+    r, v, ell, ids = ensemble(ns, sizes, 700, lowl=0.05)
+    n = r.size
+    p = ns.unit_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, precision)
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+        ctx.set_option(ns.NS_OPT_BUDGET, 1 << 16)
+        ctx.set_halos(sizes)
+        ctx.upload(r, v, ell, ids)
+        ctx.run(p, 4)
+        ens = ctx.download()
+    off = 0
+    for h, s in enumerate(sizes):
+        with ns.GpuContext(s) as ctx:
+            ctx.set_option(ns.NS_OPT_PRECISION, precision)
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+            ctx.set_option(ns.NS_OPT_BUDGET, 1 << 16)
+            ctx.upload(r[off:off + s], v[off:off + s], ell[off:off + s], ids[off:off + s])
+            ctx.run(p, 4)
+            alone = ctx.download()
+        for k, what in enumerate(("r", "v", "ell", "id")):
+            assert_bits(ens[k][off:off + s], alone[k], f"halo {h} {what}")
+        off += s
+
+
+def test_halos_rejected(ns):
+    from nsphere_amd import _lib
+    with ns.GpuContext(100) as ctx:
+        with pytest.raises(ns.NsError):
+            ctx.set_halos([50, 49])          # does not add up
+        with pytest.raises(ns.NsError):
+            ctx.set_halos([50, 50, 0])       # empty halo
+        ctx.set_halos([60, 40])
+        with pytest.raises(ns.NsError):
+            ctx.set_option(_lib.NS_OPT_SORT_ALG, 1)
+        with pytest.raises(ns.NsError):
+            ctx.set_masses(np.ones(100))
+        ctx.set_halos(None)
+        ctx.set_option(_lib.NS_OPT_SORT_ALG, 1)
