@@ -90,6 +90,11 @@ enum {
                               digit counts, their scan, scatter; default),
                               1 onesweep (one scatter kernel per pass with
                               decoupled lookback).  Same result.             */
+    NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures two steps into a HIP graph
+                              once (per parameter set and option state) and
+                              replays it, one launch per two steps; needs
+                              NS_OPT_SORT_ALG 0 and profiling off (else the
+                              steps are launched one by one).  Same result. */
     NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
                               float32 r, v_r, ell in HBM and 32-bit radius
                               keys (4 sort passes), float64 arithmetic in the

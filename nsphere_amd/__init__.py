@@ -21,7 +21,7 @@ import numpy as np
 
 from ._lib import (FORM_BINNED, FORM_BINNED_FAST, FORM_PLAIN, NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV,
                    NS_OPT_BUDGET, NS_OPT_INTEGRATOR, NS_OPT_PRECISION, NS_OPT_PROFILE,
-                   NS_OPT_SORT_ALG, NS_OPT_SORT_TILE, NS_OPT_TIEBREAK, SORT_ONESWEEP, SORT_REDUCE_SCAN,
+                   NS_OPT_GRAPH, NS_OPT_SORT_ALG, NS_OPT_SORT_TILE, NS_OPT_TIEBREAK, SORT_ONESWEEP, SORT_REDUCE_SCAN,
                    NsError, StepParams, check, lib)
 
 __all__ = [
@@ -29,7 +29,7 @@ __all__ = [
     "sort_particles", "method5_step", "philox4x32_10",
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
-    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
+    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]
 
 IC_PLUMMER, IC_HERNQUIST, IC_MIX = 0, 1, 2

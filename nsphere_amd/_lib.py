@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libnsphere_gpu.so")
 NS_OK, NS_ERR_ARG, NS_ERR_HIP, NS_ERR_NOMEM, NS_ERR_STATE, NS_ERR_NODEV = range(6)
 NS_OPT_TIEBREAK, NS_OPT_PROFILE, NS_OPT_INTEGRATOR, NS_OPT_BUDGET, NS_OPT_PRECISION, NS_OPT_SORT_TILE = 1, 2, 3, 4, 5, 6
 NS_OPT_SORT_ALG = 7
+NS_OPT_GRAPH = 8
 SORT_REDUCE_SCAN, SORT_ONESWEEP = 0, 1
 FORM_PLAIN, FORM_BINNED, FORM_BINNED_FAST = 0, 1, 2
 NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80

@@ -57,6 +57,16 @@ struct ns_gpu_ctx {
     int integrator = 0;
     int64_t budget = (int64_t)1 << 26;
     ns::Profiler prof;
+    // NS_OPT_GRAPH: ns_gpu_run replays two captured steps
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    struct GraphKey {
+        ns_step_params p;
+        int cur, integrator, f32, nseg, has_mass;
+        int64_t budget;
+        ns::SortConfig cfg;
+    } gkey = {};
 };
 
 namespace ns {
@@ -225,7 +235,7 @@ void free_all(ns_gpu_ctx *c)
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
-                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ticket, c->seg_off, c->seg_mass};
+                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -324,10 +334,16 @@ int ensure_mass(ns_gpu_ctx *c)
                 e = dalloc(&c->mf[b], c->n);
     if (e == hipSuccess && !c->menc)
         e = dalloc(&c->menc, c->n + 2);
-    if (e == hipSuccess && !c->ms.words)
+    if (e == hipSuccess && !c->ms.words) {
         e = dalloc(&c->ms.words, ns::mass_scan_words(c->n));
-    if (e == hipSuccess && !c->ms.ticket)
-        e = dalloc(&c->ms.ticket, 1);
+        if (e == hipSuccess)
+            e = hipMemset(c->ms.words, 0, (size_t)ns::mass_scan_words(c->n) * sizeof(uint64_t));
+    }
+    if (e == hipSuccess && !c->ms.ctl) {
+        e = dalloc(&c->ms.ctl, 2);
+        if (e == hipSuccess)
+            e = hipMemset(c->ms.ctl, 0, 2 * sizeof(uint32_t));
+    }
     if (e != hipSuccess)
         return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "mass buffers (npts=%lld): %s",
                          (long long)c->n, hipGetErrorString(e));
@@ -350,6 +366,37 @@ int mass_scan(ns_gpu_ctx *c, const uint32_t *perm, bool gather)
                                              c->stream));
     c->prof.end(ns::K_MASS_SCAN);
     return NS_OK;
+}
+
+void release_graph(ns_gpu_ctx *c)
+{
+    if (c->graph_exec)
+        (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph)
+        (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
+ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p)
+{
+    ns_gpu_ctx::GraphKey k = {};
+    k.p = *p;
+    k.cur = c->cur;
+    k.integrator = c->integrator;
+    k.f32 = c->f32;
+    k.nseg = c->nseg;
+    k.has_mass = c->has_mass;
+    k.budget = c->budget;
+    k.cfg = c->sw.config;
+    return k;
+}
+
+bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
+{
+    return std::memcmp(&a.p, &b.p, sizeof a.p) == 0 && a.cur == b.cur && a.integrator == b.integrator &&
+           a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.budget == b.budget &&
+           a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
 }
 
 // Sort r[cur] into c->perm (keys from the integrate epilogue when valid).
@@ -518,6 +565,7 @@ int ns_gpu_ctx_destroy(ns_gpu_ctx *c)
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->prof.release();
+    release_graph(c);
     free_all(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -530,7 +578,13 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
 {
     if (int rc = check_ctx(c, "ns_gpu_set_option"))
         return rc;
+    release_graph(c);  // any option may change what a captured step does
     switch (option) {
+    case NS_OPT_GRAPH:
+        if (value != 0 && value != 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_GRAPH takes 0 or 1");
+        c->use_graph = value == 1;
+        return NS_OK;
     case NS_OPT_TIEBREAK:
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_TIEBREAK takes 0 or 1");
@@ -777,9 +831,53 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
 {
     if (nsteps < 0)
         return set_error(NS_ERR_ARG, "ns_gpu_run: nsteps < 0");
-    for (int64_t s = 0; s < nsteps; s++)
-        if (int rc = ns_gpu_step(c, p))
-            return rc;
+    const bool graph = c && c->use_graph && !c->prof.on && c->sw.config.alg == ns::kSortReduceScan && nsteps >= 3;
+    if (!graph) {
+        for (int64_t s = 0; s < nsteps; s++)
+            if (int rc = ns_gpu_step(c, p))
+                return rc;
+        return NS_OK;
+    }
+    // One step eagerly (radius keys from the integrate epilogue, segment
+    // table built), then pairs of steps from a captured graph: two steps
+    // return the double buffers to the captured parity.  Capture records
+    // the launches without running them.
+    if (int rc = ns_gpu_step(c, p))
+        return rc;
+    nsteps--;
+    DeviceGuard g(c->device);
+    const ns_gpu_ctx::GraphKey key = graph_key(c, p);
+    if (!c->graph_exec || !same_key(key, c->gkey)) {
+        release_graph(c);
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = sort_perm(c);
+        if (!rc)
+            rc = integrate(c, p, true);
+        if (!rc)
+            rc = sort_perm(c);
+        if (!rc)
+            rc = integrate(c, p, true);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc || e != hipSuccess) {
+            if (gr)
+                (void)hipGraphDestroy(gr);
+            if (rc)
+                return rc;
+            return set_error(NS_ERR_HIP, "ns_gpu_run: stream capture failed: %s", hipGetErrorString(e));
+        }
+        c->graph = gr;
+        const hipError_t ei = hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            release_graph(c);
+            return set_error(NS_ERR_HIP, "ns_gpu_run: graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        c->gkey = key;
+    }
+    for (int64_t k = 0; k < nsteps / 2; k++)
+        HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    if (nsteps & 1)
+        return ns_gpu_step(c, p);
     return NS_OK;
 }
 
@@ -901,6 +999,7 @@ int ns_gpu_set_masses(ns_gpu_ctx *c, const double *mass)
 {
     if (int rc = check_ctx(c, "ns_gpu_set_masses"))
         return rc;
+    release_graph(c);
     if (!mass) {
         c->has_mass = false;
         return NS_OK;
@@ -939,6 +1038,7 @@ int ns_gpu_set_halos(ns_gpu_ctx *c, int nhalos, const int64_t *counts, const dou
 {
     if (int rc = check_ctx(c, "ns_gpu_set_halos"))
         return rc;
+    release_graph(c);
     DeviceGuard g(c->device);
     if (nhalos == 0 || !counts) {
         c->nseg = 0;

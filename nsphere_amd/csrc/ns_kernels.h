@@ -60,9 +60,8 @@ enum IntegratorForm : int {
 
 // Workspace of the enclosed-mass scan (ns_scan.hip).
 struct MassScan {
-    uint64_t *words;   // mass_scan_words(n) lookback words
-    uint32_t *ticket;  // tile ticket
-    uint32_t epoch;    // lookback epoch of the next launch (0: clear first)
+    uint64_t *words;   // mass_scan_words(n) lookback words (zeroed at allocation)
+    uint32_t *ctl;     // [0] tile ticket, [1] lookback epoch (device-owned)
     double scale;      // fixed-point scale, a power of two with total mass * scale < 2^62
 };
 int64_t mass_scan_words(int64_t n);

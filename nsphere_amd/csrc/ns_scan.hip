@@ -16,7 +16,9 @@
 //     and the tile's exclusive prefix is found by decoupled lookback, one
 //     wave reading 64 predecessor tiles per round.  Integer partial sums do
 //     not depend on how far a walk had to go, so M(r) is the same bit for bit
-//     on every run.
+//     on every run.  The lookback epoch and the tile ticket live in device
+//     memory and a one-block kernel advances them before each scan, so the
+//     scan replays correctly from a captured HIP graph.
 // menc[i] = (double)(tile prefix) / scale + (within-tile exclusive prefix);
 // menc[n] = the total, menc[n + 1] = the mass of the particle at rank 0.
 #include <hip/hip_runtime.h>
@@ -108,13 +110,17 @@ __device__ inline void publish(uint64_t *words, int64_t tile, uint32_t epoch, ui
     word_store(words + 2 * tile + 1, tag | (x & 0xFFFFFFFFull));
 }
 
+// ctl[0] = tile ticket, ctl[1] = lookback epoch (30 bits; on wrap-around the
+// words are cleared and the epochs restart).
 template <typename M>
 __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict__ mass_in,
                                                              const uint32_t *__restrict__ perm,
                                                              M *__restrict__ mass_out, double *__restrict__ menc,
                                                              int64_t n, double scale, uint64_t *__restrict__ words,
-                                                             uint32_t *__restrict__ ticket, uint32_t epoch)
+                                                             uint32_t *__restrict__ ctl)
 {
+    uint32_t *ticket = ctl;
+    const uint32_t epoch = ctl[1];
     using A = Acc<M>;
     using T = typename A::T;
     __shared__ T s_wave[kScanWaves];
@@ -231,17 +237,20 @@ __global__ __launch_bounds__(kScanThreads) void k_mass_scan(const M *__restrict_
         menc[n + 1] = (double)m[0];
 }
 
-__global__ void k_zero_ticket(uint32_t *t)
+// Before each scan: ticket to 0, epoch + 1 (clearing the words and
+// restarting at 1 when the 30-bit epoch would wrap).
+__global__ __launch_bounds__(1024) void k_scan_begin(uint32_t *ctl, uint64_t *words, int64_t nwords)
 {
-    if (threadIdx.x == 0)
-        *t = 0;
-}
-
-__global__ void k_clear_words(uint64_t *p, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        p[i] = 0;
+    const uint32_t e = ctl[1];  // read by every thread before anyone writes
+    __syncthreads();
+    const bool wrap = e + 1 >= (1u << 30) - 1;
+    if (wrap)
+        for (int64_t i = threadIdx.x; i < nwords; i += blockDim.x)
+            words[i] = 0;
+    if (threadIdx.x == 0) {
+        ctl[0] = 0;
+        ctl[1] = wrap ? 1u : e + 1;
+    }
 }
 
 } // namespace
@@ -256,20 +265,11 @@ hipError_t launch_mass_scan(MassScan &w, const M *mass_in, const uint32_t *perm,
         return hipSuccess;
     hipError_t e;
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    if (w.epoch == 0 || w.epoch >= (1u << 30) - 1) {  // 30-bit epochs: clear and restart
-        const int64_t words = mass_scan_words(n);
-        hipLaunchKernelGGL(k_clear_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, w.words,
-                           words);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        w.epoch = 1;
-    }
-    hipLaunchKernelGGL(k_zero_ticket, dim3(1), dim3(64), 0, stream, w.ticket);
+    hipLaunchKernelGGL(k_scan_begin, dim3(1), dim3(1024), 0, stream, w.ctl, w.words, mass_scan_words(n));
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     hipLaunchKernelGGL((k_mass_scan<M>), dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, mass_in, perm,
-                       mass_out, menc, n, w.scale, w.words, w.ticket, w.epoch);
-    w.epoch++;
+                       mass_out, menc, n, w.scale, w.words, w.ctl);
     return hipGetLastError();
 }
 
