@@ -90,6 +90,12 @@ enum {
                               digit counts, their scan, scatter; default),
                               1 onesweep (one scatter kernel per pass with
                               decoupled lookback).  Same result.             */
+    NS_OPT_LANE_LEVELS = 9,  /* binned forms: leapfrog levels 0..value-1 run one
+                                lane per particle before a particle moves to a
+                                wave (1..6, default 4).  Same result.         */
+    NS_OPT_DEEP_BLOCKS = 10, /* binned forms: workgroups (4 waves each) of the
+                                wave-per-particle kernel (1..4096, default
+                                1024).  Same result.                          */
     NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures two steps into a HIP graph
                               once (per parameter set and option state) and
                               replays it, one launch per two steps; needs
@@ -175,6 +181,27 @@ int ns_gpu_set_halos(ns_gpu_ctx *ctx, int nhalos, const int64_t *counts, const d
 /* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
  * [npts] the total, [npts + 1] the rank-0 particle's mass. */
 int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);
+
+/* Autotuner (no reference counterpart): times `steps` steps (after one
+ * untimed step) of the resident state under each candidate launch
+ * configuration -- the sort's pass algorithm and tile shape (threads x keys
+ * per thread; the digit width is fixed at 8 bits), the lane-per-particle
+ * leapfrog levels and the wave kernel's grid -- one knob at a time, keeps the
+ * fastest and restores the state, so the tuning leaves no trace but the
+ * chosen options.  Every candidate gives the same result. */
+typedef struct ns_tune_config {
+    int32_t sort_alg, sort_threads, sort_items; /* NS_OPT_SORT_ALG / _TILE    */
+    int32_t lane_levels;                        /* NS_OPT_LANE_LEVELS        */
+    int32_t deep_blocks;                        /* NS_OPT_DEEP_BLOCKS        */
+    double ms_per_step;                         /* measured (HIP events)     */
+} ns_tune_config;
+#define NS_TUNE_MAX 32
+typedef struct ns_tune_result {
+    ns_tune_config best;
+    int32_t ntried;
+    ns_tune_config tried[NS_TUNE_MAX];
+} ns_tune_result;
+int ns_gpu_autotune(ns_gpu_ctx *ctx, const ns_step_params *p, int steps, ns_tune_result *out);
 
 /* Level records of the last integrate (npts bytes, rank order). */
 int ns_gpu_levels(ns_gpu_ctx *ctx, uint8_t *host_out);

@@ -214,6 +214,14 @@ class GpuContext:
             raise ValueError("halo_mass must match counts")
         check(lib.ns_gpu_set_halos(self._h, int(c.size), c.ctypes.data, m.ctypes.data))
 
+    def autotune(self, p: StepParams, steps: int = 5) -> dict:
+        """Time each launch configuration on the resident state, keep the
+        fastest (state restored); returns the best and every trial."""
+        from ._lib import TuneResult
+        res = TuneResult()
+        check(lib.ns_gpu_autotune(self._h, ctypes.byref(p), int(steps), ctypes.byref(res)))
+        return {"best": res.best.as_dict(), "tried": [res.tried[i].as_dict() for i in range(res.ntried)]}
+
     def enclosed_mass(self) -> np.ndarray:
         """M_enc of the last scan: [i] below rank i, [n] total, [n + 1] rank-0 mass."""
         out = np.empty(self.npts + 2, dtype=np.float64)

@@ -15,6 +15,7 @@ NS_OK, NS_ERR_ARG, NS_ERR_HIP, NS_ERR_NOMEM, NS_ERR_STATE, NS_ERR_NODEV = range(
 NS_OPT_TIEBREAK, NS_OPT_PROFILE, NS_OPT_INTEGRATOR, NS_OPT_BUDGET, NS_OPT_PRECISION, NS_OPT_SORT_TILE = 1, 2, 3, 4, 5, 6
 NS_OPT_SORT_ALG = 7
 NS_OPT_GRAPH = 8
+NS_OPT_LANE_LEVELS, NS_OPT_DEEP_BLOCKS = 9, 10
 SORT_REDUCE_SCAN, SORT_ONESWEEP = 0, 1
 FORM_PLAIN, FORM_BINNED, FORM_BINNED_FAST = 0, 1, 2
 NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80
@@ -59,6 +60,22 @@ class StepParams(ctypes.Structure):
         return p
 
 
+class TuneConfig(ctypes.Structure):
+    """Mirror of ns_tune_config."""
+
+    _fields_ = [("sort_alg", ctypes.c_int32), ("sort_threads", ctypes.c_int32), ("sort_items", ctypes.c_int32),
+                ("lane_levels", ctypes.c_int32), ("deep_blocks", ctypes.c_int32), ("ms_per_step", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class TuneResult(ctypes.Structure):
+    """Mirror of ns_tune_result."""
+
+    _fields_ = [("best", TuneConfig), ("ntried", ctypes.c_int32), ("tried", TuneConfig * 32)]
+
+
 def _load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise NativeLibraryMissing(
@@ -96,6 +113,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_force_evals": (i32, [vp, c.POINTER(c.c_uint64)]),
         "ns_gpu_set_masses": (i32, [vp, vp]),
         "ns_gpu_set_halos": (i32, [vp, i32, vp, vp]),
+        "ns_gpu_autotune": (i32, [vp, c.POINTER(StepParams), i32, c.POINTER(TuneResult)]),
         "ns_gpu_enclosed_mass": (i32, [vp, vp]),
         "ns_ic_generate": (i32, [i32, i64, c.c_uint64, dbl, dbl, dbl, vp, vp, vp, vp]),
         "ns_philox4x32_10": (None, [c.POINTER(c.c_uint32), c.POINTER(c.c_uint32), c.POINTER(c.c_uint32)]),

@@ -56,6 +56,8 @@ struct ns_gpu_ctx {
     int tiebreak = 0;
     int integrator = 0;
     int64_t budget = (int64_t)1 << 26;
+    int64_t cap_N = 8;       // NS_OPT_LANE_LEVELS: levels 0..log2(cap_N) lane-per-particle
+    int deep_blocks = 1024;  // NS_OPT_DEEP_BLOCKS
     ns::Profiler prof;
     // NS_OPT_GRAPH: ns_gpu_run replays two captured steps
     bool use_graph = false;
@@ -64,7 +66,8 @@ struct ns_gpu_ctx {
     struct GraphKey {
         ns_step_params p;
         int cur, integrator, f32, nseg, has_mass;
-        int64_t budget;
+        int64_t budget, cap_N;
+        int deep_blocks;
         ns::SortConfig cfg;
     } gkey = {};
 };
@@ -311,7 +314,8 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.q_lf = c->q_lf;
     a.q_lc = c->q_lc;
     a.q_count = c->q_count;
-    a.cap_N = 8;   // leapfrog levels 0..3 lane-per-particle
+    a.cap_N = c->cap_N;  // leapfrog levels lane-per-particle (default 0..3)
+    a.deep_blocks = c->deep_blocks;
     a.work = c->prof.on ? c->work : nullptr;
     a.menc = c->has_mass ? c->menc : nullptr;
     a.nseg = c->nseg;
@@ -388,6 +392,8 @@ ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p)
     k.nseg = c->nseg;
     k.has_mass = c->has_mass;
     k.budget = c->budget;
+    k.cap_N = c->cap_N;
+    k.deep_blocks = c->deep_blocks;
     k.cfg = c->sw.config;
     return k;
 }
@@ -396,6 +402,7 @@ bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
 {
     return std::memcmp(&a.p, &b.p, sizeof a.p) == 0 && a.cur == b.cur && a.integrator == b.integrator &&
            a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.budget == b.budget &&
+           a.cap_N == b.cap_N && a.deep_blocks == b.deep_blocks &&
            a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
 }
 
@@ -584,6 +591,16 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_GRAPH takes 0 or 1");
         c->use_graph = value == 1;
+        return NS_OK;
+    case NS_OPT_LANE_LEVELS:
+        if (value < 1 || value > 6)
+            return set_error(NS_ERR_ARG, "NS_OPT_LANE_LEVELS takes 1..6 (leapfrog levels 0..value-1 per lane)");
+        c->cap_N = (int64_t)1 << (value - 1);
+        return NS_OK;
+    case NS_OPT_DEEP_BLOCKS:
+        if (value < 1 || value > 4096)
+            return set_error(NS_ERR_ARG, "NS_OPT_DEEP_BLOCKS takes 1..4096");
+        c->deep_blocks = (int)value;
         return NS_OK;
     case NS_OPT_TIEBREAK:
         if (value != 0 && value != 1)
@@ -992,6 +1009,177 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
     c->prof.reset();
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long)));
+    return NS_OK;
+}
+
+namespace {
+
+// Device copy of the current state buffer (and masses) for the autotuner.
+struct Snapshot {
+    void *r = nullptr, *v = nullptr, *l = nullptr, *m = nullptr;
+    uint32_t *id = nullptr;
+    size_t bytes = 0, mbytes = 0;
+    unsigned long long hits = 0;
+};
+
+hipError_t snapshot_copy(ns_gpu_ctx *c, Snapshot &s, bool save)
+{
+    const int b = c->cur;
+    void *r = c->f32 ? (void *)c->rf[b] : (void *)c->r[b];
+    void *v = c->f32 ? (void *)c->vf[b] : (void *)c->v[b];
+    void *l = c->f32 ? (void *)c->lf[b] : (void *)c->l[b];
+    void *m = c->has_mass ? (c->f32 ? (void *)c->mf[b] : (void *)c->md[b]) : nullptr;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(save ? s.r : r, save ? r : s.r, s.bytes, k, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(save ? s.v : v, save ? v : s.v, s.bytes, k, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(save ? s.l : l, save ? l : s.l, s.bytes, k, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(save ? (void *)s.id : (void *)c->id[b], save ? (void *)c->id[b] : (void *)s.id,
+                            (size_t)c->n * sizeof(uint32_t), k, c->stream)) != hipSuccess)
+        return e;
+    if (m && (e = hipMemcpyAsync(save ? s.m : m, save ? m : s.m, s.mbytes, k, c->stream)) != hipSuccess)
+        return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+void snapshot_free(Snapshot &s)
+{
+    for (void *p : {s.r, s.v, s.l, s.m, (void *)s.id})
+        if (p)
+            (void)hipFree(p);
+    s = Snapshot{};
+}
+
+} // namespace
+
+int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_result *out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_autotune"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_autotune"))
+        return rc;
+    if (!out || steps < 1 || steps > 10000)
+        return set_error(NS_ERR_ARG, "ns_gpu_autotune: null result or steps outside 1..10000");
+    if (!c->uploaded || c->n == 0)
+        return set_error(NS_ERR_STATE, "ns_gpu_autotune: no state uploaded");
+    DeviceGuard g(c->device);
+    release_graph(c);
+    Snapshot snap;
+    const size_t el = c->f32 ? sizeof(float) : sizeof(double);
+    snap.bytes = (size_t)c->n * el;
+    snap.mbytes = c->has_mass ? snap.bytes : 0;
+    hipError_t e = hipMalloc(&snap.r, snap.bytes);
+    if (e == hipSuccess) e = hipMalloc(&snap.v, snap.bytes);
+    if (e == hipSuccess) e = hipMalloc(&snap.l, snap.bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&snap.id, (size_t)c->n * sizeof(uint32_t));
+    if (e == hipSuccess && c->has_mass) e = hipMalloc(&snap.m, snap.mbytes);
+    if (e == hipSuccess) e = hipMemcpy(&snap.hits, c->budget_hits, sizeof snap.hits, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = snapshot_copy(c, snap, true);
+    if (e != hipSuccess) {
+        snapshot_free(snap);
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "ns_gpu_autotune: snapshot: %s",
+                         hipGetErrorString(e));
+    }
+    const int cur0 = c->cur;
+    const bool prof_on = c->prof.on;
+    c->prof.on = false;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    (void)hipEventCreate(&ea);
+    (void)hipEventCreate(&eb);
+    out->ntried = 0;
+    auto current = [&]() {
+        ns_tune_config t = {};
+        t.sort_alg = c->sw.config.alg;
+        t.sort_threads = c->sw.config.threads;
+        t.sort_items = c->sw.config.items;
+        int lv = 1;
+        while (((int64_t)1 << (lv - 1)) < c->cap_N)
+            lv++;
+        t.lane_levels = lv;
+        t.deep_blocks = c->deep_blocks;
+        return t;
+    };
+    auto apply = [&](const ns_tune_config &t) {
+        c->sw.config.alg = t.sort_alg;
+        c->sw.config.threads = t.sort_threads;
+        c->sw.config.items = t.sort_items;
+        c->cap_N = (int64_t)1 << (t.lane_levels - 1);
+        c->deep_blocks = t.deep_blocks;
+    };
+    int rc = NS_OK;
+    // one trial: restore, one untimed step, `steps` timed steps
+    auto trial = [&](ns_tune_config &t) -> int {
+        apply(t);
+        c->cur = cur0;
+        c->keys_valid = false;
+        if (snapshot_copy(c, snap, false) != hipSuccess)
+            return set_error(NS_ERR_HIP, "ns_gpu_autotune: restore failed");
+        if (int r2 = ns_gpu_step(c, p))
+            return r2;
+        HIP_TRY(hipEventRecord(ea, c->stream));
+        for (int s = 0; s < steps; s++)
+            if (int r2 = ns_gpu_step(c, p))
+                return r2;
+        HIP_TRY(hipEventRecord(eb, c->stream));
+        HIP_TRY(hipEventSynchronize(eb));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ea, eb));
+        t.ms_per_step = ms / steps;
+        if (out->ntried < NS_TUNE_MAX)
+            out->tried[out->ntried++] = t;
+        return NS_OK;
+    };
+    const ns_tune_config orig = current();
+    ns_tune_config best = orig;
+    best.ms_per_step = 1e300;
+    // knob 1: pass algorithm x tile shape
+    static const int kTiles[4][2] = {{256, 16}, {256, 24}, {512, 12}, {512, 16}};
+    for (int alg = 0; alg < 2 && !rc; alg++) {
+        if (alg == ns::kSortOnesweep && c->nseg > 0)
+            continue;  // an ensemble sorts by segments
+        for (int k = 0; k < 4 && !rc; k++) {
+            ns_tune_config t = best;
+            t.sort_alg = alg;
+            t.sort_threads = kTiles[k][0];
+            t.sort_items = kTiles[k][1];
+            if ((rc = trial(t)) == NS_OK && t.ms_per_step < best.ms_per_step)
+                best = t;
+        }
+    }
+    // knob 2: leapfrog levels per lane; knob 3: the wave kernel's grid
+    for (int lv = 2; lv <= 5 && !rc; lv++) {
+        ns_tune_config t = best;
+        t.lane_levels = lv;
+        if ((rc = trial(t)) == NS_OK && t.ms_per_step < best.ms_per_step)
+            best = t;
+    }
+    for (int blocks : {512, 1024, 2048, 4096}) {
+        if (rc)
+            break;
+        ns_tune_config t = best;
+        t.deep_blocks = blocks;
+        if ((rc = trial(t)) == NS_OK && t.ms_per_step < best.ms_per_step)
+            best = t;
+    }
+    // leave the state as it was, with the chosen options
+    c->cur = cur0;
+    c->keys_valid = false;
+    c->levels_valid = false;
+    const hipError_t er = snapshot_copy(c, snap, false);
+    if (er == hipSuccess)
+        (void)hipMemcpy(c->budget_hits, &snap.hits, sizeof snap.hits, hipMemcpyHostToDevice);
+    snapshot_free(snap);
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    c->prof.on = prof_on;
+    if (rc) {
+        apply(orig);
+        return rc;
+    }
+    if (er != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_autotune: restore failed: %s", hipGetErrorString(er));
+    apply(best);
+    out->best = best;
     return NS_OK;
 }
 

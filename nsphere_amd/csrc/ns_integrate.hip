@@ -32,7 +32,7 @@ namespace ns {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kDeepBlocks = 1024;     // 4096 waves for the wave-per-particle queues
+constexpr int kMaxDeepBlocks = 4096;  // bound of StepArgs::deep_blocks (work-slot sizing)
 
 __device__ inline uint64_t lanemask_lt()
 {
@@ -335,7 +335,8 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     begin(K_INT_DEEP);
-    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(kDeepBlocks), dim3(kThreads), 0, stream, a);
+    const int deep_blocks = a.deep_blocks > 0 && a.deep_blocks <= kMaxDeepBlocks ? a.deep_blocks : 1024;
+    hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(deep_blocks), dim3(kThreads), 0, stream, a);
     end(K_INT_DEEP);
     return hipGetLastError();
 }
@@ -356,7 +357,7 @@ hipError_t launch_typed(const StepArgs &a, bool gather, int form, hipStream_t st
 int64_t work_stride_for(int64_t n)
 {
     const int64_t bulk = (n + kThreads - 1) / kThreads * (kThreads / 64);
-    const int64_t deep = (int64_t)kDeepBlocks * (kThreads / 64);
+    const int64_t deep = (int64_t)kMaxDeepBlocks * (kThreads / 64);
     return bulk > deep ? bulk : deep;
 }
 

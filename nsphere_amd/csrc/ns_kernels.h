@@ -38,6 +38,7 @@ struct StepArgs {
     uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave
                                  // ticket (kQueueCounters)
     int64_t cap_N;               // deepest leapfrog N tried lane-per-particle
+    int32_t deep_blocks;         // grid of the wave-per-particle kernel (256 threads each)
     // profiling only (else null): force evaluations spent, one slot per
     // wave of a launch (slot = global wave index; each wave adds its own with
     // a plain read-modify-write, launches being stream-ordered), in rows of
