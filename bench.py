@@ -75,7 +75,23 @@ WORKLOADS = {
     "plummer_n1.2e7_lowl5pct_dt1e-3": (0, 12_000_000, 100.0, 0.05),
     "hernquist_n5e7_lowl5pct_dt1e-3": (1, 50_000_000, 200.0, 0.05),
     "mix_n1.2e7_lowl10pct_dt1e-3": (2, 12_000_000, 100.0, 0.10),
+    # 256 independent halos, N_h log-uniform in [1e3, 1e6], alternating Plummer / Hernquist
+    "ensemble_256halos_lowl2pct_dt1e-3": (-1, 0, 0.0, 0.02),
 }
+
+
+def ensemble_ic(ns, seed: int, nhalos: int = 256, lowl: float = 0.02):
+    """Halos of N_h = 10^U(3, 6) particles, Plummer (even h, r < 100) or Hernquist
+    (odd h, r < 200), each in its own radius order, ids unique over the ensemble."""
+    sizes = np.round(10.0 ** np.random.default_rng(seed).uniform(3.0, 6.0, nhalos)).astype(np.int64)
+    parts, base = [], 0
+    for h, nh in enumerate(sizes):
+        kind = h % 2
+        r, v, ell, ids = ns.generate_ic(kind, int(nh), seed + 1 + h, 100.0 if kind == 0 else 200.0, lowl, 1e-3)
+        q = np.argsort(r, kind="stable")
+        parts.append((r[q], v[q], ell[q], (ids[q] + base).astype(np.uint32)))
+        base += int(nh)
+    return [np.concatenate([pt[k] for pt in parts]) for k in range(4)], sizes
 
 FORMS = {"plain": 0, "binned": 1, "binned-fast": 2}
 PRECISIONS = {"f64": 0, "f32": 1}
@@ -92,20 +108,30 @@ def cpu_info() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(state, params, budget_s: float, work_cap: int) -> dict:
+def cpu_baseline(state, params, budget_s: float, work_cap: int, halos=None) -> dict:
     """The oracle with the reference's -Ofast flags, OpenMP on the host cores
     and the reference's default chunked sort, timed on whole steps of the same
-    workload from the same initial state, under the same work cap."""
+    workload from the same initial state, under the same work cap.  An
+    ensemble is stepped halo by halo, each as its own system (the reference
+    runs one halo per process), for whole passes over all halos."""
     import oracle
 
     threads = oracle.lib(True).orc_num_threads()
     po = oracle.from_step_params(params, work_cap)
     r, v, ell, ids = (a.copy() for a in state)
     n = r.size
+    if halos is None:
+        bounds = [(0, n)]
+    else:
+        off = np.concatenate([[0], np.cumsum(halos)])
+        bounds = list(zip(off[:-1], off[1:]))
+        ids = np.concatenate([ids[a:b] - a for a, b in bounds]).astype(np.uint32)  # per-halo ids
     steps = 0
     t0 = time.perf_counter()
     while True:
-        oracle.step(r, v, ell, ids, po, chunked=True, fast=True)
+        for a, b in bounds:
+            hr, hv, hl, hi = r[a:b], v[a:b], ell[a:b], ids[a:b]
+            oracle.step(hr, hv, hl, hi, po, chunked=True, fast=True)
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 200:
@@ -122,7 +148,11 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int) -> dict:
 
 
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
-            mass=None) -> dict:
+            mass=None, halos=None, graph=False, autotune=0) -> dict:
+    """Time `steps` steps after `warmup`.  Per-kernel HIP-event timing runs over
+    the timed region, unless graph: then the timed region replays captured
+    steps (no per-kernel events) and the kernel table comes from 10 profiled
+    steps after it."""
     n = state[0].size
     ctx = ns.GpuContext(n)
     chunk = 10
@@ -131,14 +161,18 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
+        if halos is not None:
+            ctx.set_halos(halos)
         ctx.upload(*state)
         if mass is not None:
             ctx.set_masses(mass)
         K0, W0 = ctx.energy(p)
         ctx.run(p, warmup)
         ctx.sync()
+        tune = ctx.autotune(p, autotune) if autotune else None
+        ctx.set_option(ns.NS_OPT_GRAPH, int(graph))
         ctx.cap_hits(reset=True)
-        ctx.set_option(ns.NS_OPT_PROFILE, 1)
+        ctx.set_option(ns.NS_OPT_PROFILE, 0 if graph else 1)
         ctx.reset_kernel_stats()
         ctx.sync()
         t0 = time.perf_counter()
@@ -152,12 +186,17 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
                 print(f"[bench] {form}: {done}/{steps} steps, {time.perf_counter() - t0:.2f}s",
                       file=sys.stderr, flush=True)
         el = time.perf_counter() - t0
-        stats = ctx.kernel_stats()
-        evals = ctx.force_evals()
-        ctx.set_option(ns.NS_OPT_PROFILE, 0)
         hits = ctx.cap_hits()
         K1, W1 = ctx.energy(p)
         lv = ctx.levels()
+        if graph:  # kernel table from a short profiled run after the timed region
+            ctx.set_option(ns.NS_OPT_PROFILE, 1)
+            ctx.reset_kernel_stats()
+            ctx.run(p, 10)
+            ctx.sync()
+        stats = ctx.kernel_stats()
+        evals = ctx.force_evals()
+        ctx.set_option(ns.NS_OPT_PROFILE, 0)
     finally:
         ctx.close()
     kernels = {}
@@ -184,6 +223,8 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         "E0": K0 + W0,
         "E1": K1 + W1,
         "cap_hits": hits,
+        "graph": graph,
+        "autotune": tune,
         "levels": {"hist": np.bincount(lev, minlength=8)[:24].tolist(),
                    "levi_civita": int(((lv & ns.NS_LVL_LC) != 0).sum())},
     }
@@ -240,6 +281,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the chosen form with captured HIP graphs (NS_OPT_GRAPH)")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="autotune the chosen form's launch configuration first (STEPS timed steps per trial)")
     ap.add_argument("--budget", type=int, default=1 << 18,
                     help="force evaluations allowed per particle-step, the same for every form and "
                          "the CPU baseline (the reference has no cap; see DESIGN.md)")
@@ -253,15 +298,20 @@ def main() -> int:
     import nsphere_amd as ns
 
     kind, n_default, rmax, lowl = WORKLOADS[args.workload]
-    n = args.n or n_default
     t_ic = time.perf_counter()
-    r, v, ell, ids = ns.generate_ic(kind, n, args.seed, rmax, lowl, 1e-3)
     p = ns.unit_params()
-    # the reference's exact initial sort (nsphere.c:5339-5342), done once
-    with ns.GpuContext(n) as ctx:
-        ctx.upload(r, v, ell, ids)
-        ctx.sort_particles()
-        state = ctx.download()
+    halos = None
+    if kind < 0:
+        state, halos = ensemble_ic(ns, args.seed, lowl=lowl)
+        n = state[0].size
+    else:
+        n = args.n or n_default
+        r, v, ell, ids = ns.generate_ic(kind, n, args.seed, rmax, lowl, 1e-3)
+        # the reference's exact initial sort (nsphere.c:5339-5342), done once
+        with ns.GpuContext(n) as ctx:
+            ctx.upload(r, v, ell, ids)
+            ctx.sort_particles()
+            state = ctx.download()
     print(f"[bench] N={n} initial state in {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
 
     mass = None
@@ -270,11 +320,12 @@ def main() -> int:
             np.random.default_rng(args.seed + 1).uniform(0.5, 1.5, n)
         by_id /= by_id.sum()
         mass = by_id[state[3]]  # in the order of the sorted initial state
-    chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass)
+    chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass, halos,
+                     args.graph, args.autotune)
     plain = None
     if not args.no_plain and (args.form, args.precision) != ("plain", "f64"):
         plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget,
-                        mass)
+                        mass, halos)
 
     dom = max(chosen["stats"].items(), key=lambda kv: kv[1][1])[0]
     out = {
@@ -292,7 +343,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "dt": p.dt, "form": args.form,
                    "state": args.precision, "lowl_frac": lowl, "lowl_scale": 1e-3, "seed": args.seed,
-                   "masses": args.masses},
+                   "masses": args.masses, "halos": None if halos is None else int(len(halos)),
+                   "graph": args.graph},
+        "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
         "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
@@ -315,7 +368,7 @@ def main() -> int:
             out["energy_vs_plain"] = abs(chosen["E1"] - plain["E1"]) / abs(plain["E1"])
     if not args.no_cpu:
         print("[bench] timing CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds, args.budget)
+        out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds, args.budget, halos)
         if mass is not None:
             out["cpu_baseline"]["sample"] += "; equal masses from the rank (the oracle step has no mass scan)"
     else:
