@@ -148,7 +148,7 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int, halos=None) -> d
 
 
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
-            mass=None, halos=None, graph=False, autotune=0) -> dict:
+            mass=None, halos=None, graph=False, autotune=0, mlp=None) -> dict:
     """Time `steps` steps after `warmup`.  Per-kernel HIP-event timing runs over
     the timed region, unless graph: then the timed region replays captured
     steps (no per-kernel events) and the kernel table comes from 10 profiled
@@ -166,6 +166,8 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.upload(*state)
         if mass is not None:
             ctx.set_masses(mass)
+        if mlp is not None:
+            ctx.set_mlp(mlp)
         K0, W0 = ctx.energy(p)
         ctx.run(p, warmup)
         ctx.sync()
@@ -230,6 +232,42 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
     }
 
 
+def train_level_predictor(ns, kind: int, n_log: int, rmax: float, lowl: float, seed: int, p, work_cap: int,
+                          log_steps: int = 10, epochs: int = 4):
+    """Levels logged by the plain adaptive integrator (FP64) over `log_steps`
+    steps of an n_log-particle run of the recipe, the network trained on them
+    by the library's own kernels; a second, smaller run gives held-out pairs."""
+    t0 = time.perf_counter()
+
+    def logged(n, s, steps):
+        r, v, ell, ids = ns.generate_ic(max(kind, 0) if kind >= 0 else 2, n, s, rmax or 100.0, lowl, 1e-3)
+        with ns.GpuContext(n) as ctx:
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, ns.FORM_PLAIN)
+            ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
+            ctx.upload(r, v, ell, ids)
+            return ns.level_samples(ctx, p, steps)
+
+    X, y = logged(n_log, seed + 101, log_steps)
+    Xt, yt = logged(max(n_log // 4, 1000), seed + 202, 2)
+    t_log = time.perf_counter() - t0
+    mlp = ns.Mlp(seed=seed)
+    t1 = time.perf_counter()
+    loss = mlp.train(X, y, epochs=epochs, batch=16384, lr=3e-3)
+    t_train = time.perf_counter() - t1
+    pred = mlp.predict(Xt)
+    conf = np.zeros((8, 8), np.int64)
+    np.add.at(conf, (yt.astype(np.int64), pred.astype(np.int64)), 1)
+    deep = yt > 0
+    print(f"[bench] level predictor: {X.shape[0]} pairs logged in {t_log:.1f}s, trained in {t_train:.1f}s",
+          file=sys.stderr, flush=True)
+    return mlp, {"log_particles": n_log, "log_steps": log_steps, "train_pairs": int(X.shape[0]),
+                 "epochs": epochs, "train_seconds": round(t_train, 3), "final_loss": loss,
+                 "held_out_pairs": int(Xt.shape[0]),
+                 "held_out_level_gt0_recall": float(np.mean(pred[deep] > 0)) if deep.any() else None,
+                 "held_out_level0_false_positive": float(np.mean(pred[~deep] > 0)),
+                 "held_out_confusion": conf.tolist()}
+
+
 def traffic_for(run: dict, n: int, kernel: str):
     return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
 
@@ -283,6 +321,10 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true",
                     help="time the chosen form with captured HIP graphs (NS_OPT_GRAPH)")
+    ap.add_argument("--mlp", type=int, default=0, metavar="N_LOG",
+                    help="learned binning: log levels of the plain adaptive integrator on a separate "
+                         "N_LOG-particle run of the same recipe (10 steps), train the level predictor on "
+                         "them with the library's kernels, and time the chosen form with and without it")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="autotune the chosen form's launch configuration first (STEPS timed steps per trial)")
     ap.add_argument("--budget", type=int, default=1 << 18,
@@ -320,8 +362,18 @@ def main() -> int:
             np.random.default_rng(args.seed + 1).uniform(0.5, 1.5, n)
         by_id /= by_id.sum()
         mass = by_id[state[3]]  # in the order of the sorted initial state
+    mlp, learned = None, None
+    if args.mlp:
+        mlp, learned = train_level_predictor(ns, kind, args.mlp, rmax, lowl, args.seed, p, args.budget)
     chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass, halos,
-                     args.graph, args.autotune)
+                     args.graph, args.autotune, mlp)
+    if mlp is not None:  # the same form without the network
+        without = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass,
+                          halos, args.graph, args.autotune)
+        learned.update({"value_without": without["value"], "ms_per_step_without": without["ms_per_step"],
+                        "gain": chosen["value"] / without["value"],
+                        "kernels_without": without["kernels"],
+                        "identical_energy": chosen["E1"] == without["E1"]})
     plain = None
     if not args.no_plain and (args.form, args.precision) != ("plain", "f64"):
         plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget,
@@ -358,6 +410,9 @@ def main() -> int:
         "levels": chosen["levels"],
         "work_cap": {"force_evals_per_particle_step": args.budget, "capped_particle_steps": chosen["cap_hits"]},
     }
+    if learned is not None:
+        out["config"]["learned_binning"] = True
+        out["learned_binning"] = learned
     if plain is not None:
         out["plain_fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                              "kernels": plain["kernels"],

@@ -178,6 +178,43 @@ int ns_gpu_set_masses(ns_gpu_ctx *ctx, const double *mass);
  * the ensemble.  Needs NS_OPT_SORT_ALG 0; excludes per-particle masses. */
 int ns_gpu_set_halos(ns_gpu_ctx *ctx, int nhalos, const int64_t *counts, const double *halo_mass);
 
+/* Learned refinement binning (no reference counterpart).  A small network,
+ * 5 inputs (log r, v_r, log ell, log M_enc, log dt) -> 64 -> 64 -> 8 levels,
+ * BF16 weights on the matrix cores with FP32 accumulation, trained and run by
+ * this library's own kernels on the levels that the adaptive integrator
+ * logged.  With a network set (ns_gpu_set_mlp), every step of the binned
+ * forms predicts each particle's leapfrog level, orders the particles by it
+ * (one stable radix pass), and sends particles predicted past the lane
+ * levels straight to a wave; the adaptive search still starts at level 0 for
+ * every particle, so results are those without the network, bit for bit. */
+typedef struct ns_mlp ns_mlp;
+#define NS_MLP_PARAMS 6288  /* W1[64][16] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]
+                               (input columns >= 5 and class rows >= 8 zero) */
+int ns_mlp_create(int device, uint64_t seed, ns_mlp **out);
+int ns_mlp_destroy(ns_mlp *mlp);
+/* Train on n (features X[n][5] as ns_gpu_features gives them, level y[n] in
+ * 0..7) for `epochs` passes of minibatches of `batch`, Adam at rate lr; the
+ * inputs are normalised by the set's mean and spread and the classes
+ * weighted by 1/sqrt(frequency).  loss: the last epoch's weighted mean
+ * cross-entropy. */
+int ns_mlp_train(ns_mlp *mlp, const float *X, const uint8_t *y, int64_t n, int epochs, int64_t batch, float lr,
+                 float *loss);
+/* Gradient of the weighted mean cross-entropy over (X, y) at the current
+ * parameters, FP32, in the NS_MLP_PARAMS layout (a check of the training
+ * kernels against a CPU restatement); loss as in ns_mlp_train. */
+int ns_mlp_gradient(ns_mlp *mlp, const float *X, const uint8_t *y, int64_t n, float *grad, float *loss);
+int ns_mlp_predict(ns_mlp *mlp, const float *X, int64_t n, uint8_t *level);
+/* params[NS_MLP_PARAMS], norm[10] (mean[5], 1/spread[5]), class_weight[8];
+ * any may be NULL. */
+int ns_mlp_get_params(ns_mlp *mlp, float *params, float *norm, float *class_weight);
+int ns_mlp_set_params(ns_mlp *mlp, const float *params, const float *norm, const float *class_weight);
+/* Use (NULL: stop using) a network for binning; needs NS_OPT_SORT_ALG 0.  The
+ * network must outlive its use by the context. */
+int ns_gpu_set_mlp(ns_gpu_ctx *ctx, const ns_mlp *mlp);
+/* The network's raw inputs for the current state, in the current order taken
+ * as rank order (as after ns_gpu_sort_particles): host_out[npts][5]. */
+int ns_gpu_features(ns_gpu_ctx *ctx, const ns_step_params *p, float *host_out);
+
 /* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
  * [npts] the total, [npts + 1] the rank-0 particle's mass. */
 int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);

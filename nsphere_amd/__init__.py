@@ -29,7 +29,7 @@ __all__ = [
     "sort_particles", "method5_step", "philox4x32_10",
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
-    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
+    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "Mlp", "level_samples", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]
 
 IC_PLUMMER, IC_HERNQUIST, IC_MIX = 0, 1, 2
@@ -222,6 +222,17 @@ class GpuContext:
         check(lib.ns_gpu_autotune(self._h, ctypes.byref(p), int(steps), ctypes.byref(res)))
         return {"best": res.best.as_dict(), "tried": [res.tried[i].as_dict() for i in range(res.ntried)]}
 
+    def set_mlp(self, mlp) -> None:
+        """Learned binning with a trained Mlp (None: off)."""
+        check(lib.ns_gpu_set_mlp(self._h, None if mlp is None else mlp._h))
+        self._mlp = mlp  # keep the network alive while the context uses it
+
+    def features(self, p: StepParams) -> np.ndarray:
+        """The network's raw inputs for the current state (current order = rank order)."""
+        out = np.empty((self.npts, 5), dtype=np.float32)
+        check(lib.ns_gpu_features(self._h, ctypes.byref(p), out.ctypes.data))
+        return out
+
     def enclosed_mass(self) -> np.ndarray:
         """M_enc of the last scan: [i] below rank i, [n] total, [n + 1] rank-0 mass."""
         out = np.empty(self.npts + 2, dtype=np.float64)
@@ -233,6 +244,94 @@ class GpuContext:
         out = (ctypes.c_uint64 * 2)()
         check(lib.ns_gpu_force_evals(self._h, out))
         return {"bulk_or_plain": out[0], "deep": out[1]}
+
+
+class Mlp:
+    """The 5 -> 64 -> 64 -> 8 level predictor (BF16 MFMA, FP32 accumulation)."""
+
+    NPARAMS = 6288
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        h = ctypes.c_void_p()
+        check(lib.ns_mlp_create(int(device), int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.ns_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _xy(X, y=None):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != 5:
+            raise ValueError("X must be (n, 5)")
+        if y is None:
+            return X, None
+        y = np.ascontiguousarray(y, dtype=np.uint8)
+        if y.shape != (X.shape[0],):
+            raise ValueError("y must be (n,)")
+        return X, y
+
+    def train(self, X, y, epochs: int = 2, batch: int = 16384, lr: float = 3e-3) -> float:
+        X, y = self._xy(X, y)
+        loss = ctypes.c_float()
+        check(lib.ns_mlp_train(self._h, X.ctypes.data, y.ctypes.data, X.shape[0], int(epochs), int(batch),
+                               float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def gradient(self, X, y):
+        X, y = self._xy(X, y)
+        g = np.empty(self.NPARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        check(lib.ns_mlp_gradient(self._h, X.ctypes.data, y.ctypes.data, X.shape[0], g.ctypes.data,
+                                  ctypes.byref(loss)))
+        return g, loss.value
+
+    def predict(self, X) -> np.ndarray:
+        X, _ = self._xy(X)
+        out = np.empty(X.shape[0], dtype=np.uint8)
+        check(lib.ns_mlp_predict(self._h, X.ctypes.data, X.shape[0], out.ctypes.data))
+        return out
+
+    def params(self):
+        P = np.empty(self.NPARAMS, np.float32)
+        norm = np.empty(10, np.float32)
+        cw = np.empty(8, np.float32)
+        check(lib.ns_mlp_get_params(self._h, P.ctypes.data, norm.ctypes.data, cw.ctypes.data))
+        return P, norm, cw
+
+    def set_params(self, P=None, norm=None, cw=None) -> None:
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (P, norm, cw)]
+        check(lib.ns_mlp_set_params(self._h, *[None if a is None else a.ctypes.data for a in arrs]))
+
+
+def level_samples(ctx: GpuContext, params: StepParams, steps: int, every: int = 1):
+    """Training pairs for the level predictor, logged by the adaptive
+    integrator itself: before every `every`-th of `steps` steps the state is
+    put in rank order and its features read; the step's level records then
+    label them.  Levi-Civita particles and particles that did not converge
+    are left out; levels above 7 count as 7.  Returns (X float32 (m, 5),
+    y uint8 (m,))."""
+    xs, ys = [], []
+    for s in range(int(steps)):
+        if s % every == 0:
+            ctx.sort_particles()
+            X = ctx.features(params)
+            ctx.step(params)
+            lv = ctx.levels()
+            keep = (lv & (NS_LVL_LC | NS_LVL_NOCONV)) == 0
+            xs.append(X[keep])
+            ys.append(np.minimum(lv[keep] & NS_LVL_MASK, 7).astype(np.uint8))
+        else:
+            ctx.step(params)
+    return np.concatenate(xs), np.concatenate(ys)
 
 
 # -- reference-interface mirror -------------------------------------------

@@ -114,6 +114,15 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_set_masses": (i32, [vp, vp]),
         "ns_gpu_set_halos": (i32, [vp, i32, vp, vp]),
         "ns_gpu_autotune": (i32, [vp, c.POINTER(StepParams), i32, c.POINTER(TuneResult)]),
+        "ns_mlp_create": (i32, [i32, c.c_uint64, c.POINTER(vp)]),
+        "ns_mlp_destroy": (i32, [vp]),
+        "ns_mlp_train": (i32, [vp, vp, vp, i64, i32, i64, c.c_float, c.POINTER(c.c_float)]),
+        "ns_mlp_gradient": (i32, [vp, vp, vp, i64, vp, c.POINTER(c.c_float)]),
+        "ns_mlp_predict": (i32, [vp, vp, i64, vp]),
+        "ns_mlp_get_params": (i32, [vp, vp, vp, vp]),
+        "ns_mlp_set_params": (i32, [vp, vp, vp, vp]),
+        "ns_gpu_set_mlp": (i32, [vp, vp]),
+        "ns_gpu_features": (i32, [vp, c.POINTER(StepParams), vp]),
         "ns_gpu_enclosed_mass": (i32, [vp, vp]),
         "ns_ic_generate": (i32, [i32, i64, c.c_uint64, dbl, dbl, dbl, vp, vp, vp, vp]),
         "ns_philox4x32_10": (None, [c.POINTER(c.c_uint32), c.POINTER(c.c_uint32), c.POINTER(c.c_uint32)]),

@@ -43,6 +43,9 @@ struct ns_gpu_ctx {
     double *menc = nullptr;
     bool has_mass = false;
     ns::MassScan ms = {};
+    // learned binning: the network and the rank order by predicted level
+    const ns_mlp *mlp = nullptr;
+    uint32_t *order = nullptr;
     // ensemble of halos (nseg > 0): halo h holds ranks [seg_off[h], seg_off[h+1])
     int nseg = 0;
     int64_t *seg_off = nullptr;
@@ -94,7 +97,7 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
-                                         "integrate_deep", "mass_scan"};
+                                         "integrate_deep", "mass_scan", "mlp_binning"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -238,7 +241,8 @@ void free_all(ns_gpu_ctx *c)
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
-                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass};
+                    c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
+                    c->order};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -451,6 +455,18 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
     a.id1 = gather ? c->id[d] : nullptr;
     a.keys = c->sw.keys[0];
     a.lvl = c->lvl;
+    if (c->mlp && c->integrator != ns::FORM_PLAIN) {
+        // learned binning: predicted levels (as 32-bit labels in the key
+        // buffer, free until the integrator writes the next radius keys),
+        // then the ranks in stable order of predicted level
+        c->prof.begin(ns::K_MLP);
+        HIP_TRY(ns::launch_mlp_predict(a, gather, c->f32, ns::mlp_params(c->mlp), ns::mlp_norm(c->mlp),
+                                       reinterpret_cast<uint32_t *>(c->sw.keys[0]), c->stream));
+        HIP_TRY(ns::radix_sort_perm_u8(c->sw, c->n, c->order, c->stream));
+        c->prof.end(ns::K_MLP);
+        a.order = c->order;
+        a.bin_count = c->sw.hist;
+    }
     HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
     c->cur = d;
     c->keys_valid = true;
@@ -637,6 +653,8 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         cfg.alg = (int)value;
         if (value < 0 || value > 1 || !ns::sort_config_supported(cfg))
             return set_error(NS_ERR_ARG, "NS_OPT_SORT_ALG takes 0 (reduce-then-scan) or 1 (onesweep)");
+        if (value == ns::kSortOnesweep && c->mlp)
+            return set_error(NS_ERR_STATE, "NS_OPT_SORT_ALG: learned binning sorts by reduce-then-scan only");
         if (value == ns::kSortOnesweep && c->nseg > 0)
             return set_error(NS_ERR_STATE, "NS_OPT_SORT_ALG: an ensemble sorts by segments, reduce-then-scan only");
         c->sw.config = cfg;
@@ -1279,6 +1297,63 @@ int ns_gpu_set_halos(ns_gpu_ctx *c, int nhalos, const int64_t *counts, const dou
                          hipGetErrorString(e));
     }
     c->nseg = nhalos;
+    return NS_OK;
+}
+
+int ns_gpu_set_mlp(ns_gpu_ctx *c, const ns_mlp *mlp)
+{
+    if (int rc = check_ctx(c, "ns_gpu_set_mlp"))
+        return rc;
+    release_graph(c);
+    if (!mlp) {
+        c->mlp = nullptr;
+        return NS_OK;
+    }
+    if (c->sw.config.alg != ns::kSortReduceScan)
+        return set_error(NS_ERR_STATE, "ns_gpu_set_mlp: needs the reduce-then-scan sort (NS_OPT_SORT_ALG 0)");
+    if (ns::mlp_device(mlp) != c->device)
+        return set_error(NS_ERR_ARG, "ns_gpu_set_mlp: network on device %d, context on device %d",
+                         ns::mlp_device(mlp), c->device);
+    DeviceGuard g(c->device);
+    if (!c->order) {
+        hipError_t e = dalloc(&c->order, c->n);
+        if (e != hipSuccess)
+            return set_error(NS_ERR_NOMEM, "ns_gpu_set_mlp: %s", hipGetErrorString(e));
+    }
+    c->mlp = mlp;
+    return NS_OK;
+}
+
+int ns_gpu_features(ns_gpu_ctx *c, const ns_step_params *p, float *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_features"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_features"))
+        return rc;
+    if (!host_out)
+        return set_error(NS_ERR_ARG, "ns_gpu_features: null output");
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_features: no state uploaded");
+    if (c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    if (int rc = mass_scan(c, nullptr, false))
+        return rc;
+    ns::StepArgs a = make_args(c, p);
+    const int s = c->cur;
+    a.r0 = c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s];
+    a.v0 = c->f32 ? (const void *)c->vf[s] : (const void *)c->v[s];
+    a.l0 = c->f32 ? (const void *)c->lf[s] : (const void *)c->l[s];
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, (size_t)c->n * 5 * sizeof(float)));
+    hipError_t e = ns::launch_mlp_features(a, c->f32, d, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host_out, d, (size_t)c->n * 5 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_features: %s", hipGetErrorString(e));
     return NS_OK;
 }
 

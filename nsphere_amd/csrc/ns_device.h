@@ -144,6 +144,9 @@ hipError_t launch_radius_keys(const float *r, uint32_t *keys, int64_t n, hipStre
 // Sorts the key_bits-wide (32 or 64) keys in keys[0], stably.  On return
 // perm_out[k] = input position of rank k (perm_out aliases no vals buffer).
 struct Profiler;
+// One stable pass on the low byte of 32-bit keys in keys[0], ignoring any
+// segments (reduce-then-scan only): the order of a small integer label.
+hipError_t radix_sort_perm_u8(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream);
 hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_out, hipStream_t stream,
                            Profiler *prof);
 
@@ -163,6 +166,7 @@ enum KernelId : int {
     K_INT_BULK,
     K_INT_DEEP,
     K_MASS_SCAN,
+    K_MLP,
     K_COUNT
 };
 const char *kernel_name(int id);

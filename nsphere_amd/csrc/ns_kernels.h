@@ -33,6 +33,10 @@ struct StepArgs {
     // per-particle masses (else null: M_enc from the rank, as the reference):
     // menc[i] = mass enclosed below rank i, menc[n + 1] = mass of rank 0
     const double *menc;
+    // learned binning (else null): bulk thread t takes rank order[t]; ranks
+    // are grouped by predicted level, bin b holding bin_count[b] of them
+    const uint32_t *order;
+    const uint32_t *bin_count;
     uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
     uint32_t *q_lc;              // Levi-Civita particles (wave per particle)
     uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave
@@ -72,6 +76,19 @@ int64_t mass_scan_words(int64_t n);
 template <typename M>
 hipError_t launch_mass_scan(MassScan &w, const M *mass_in, const uint32_t *perm, M *mass_out, double *menc,
                             int64_t n, hipStream_t stream);
+
+} // namespace ns
+struct ns_mlp;
+namespace ns {
+// Learned binning (ns_mlp.hip): predicted level (0..7) of every rank of the
+// step described by a (gathering through a.perm when gather), as 32-bit
+// labels; raw features of the current state in rank order (n x 5 floats).
+hipError_t launch_mlp_predict(const StepArgs &a, bool gather, bool f32, const float *P, const float *norm,
+                              uint32_t *pred, hipStream_t stream);
+hipError_t launch_mlp_features(const StepArgs &a, bool f32, float *out, hipStream_t stream);
+const float *mlp_params(const ns_mlp *m);
+const float *mlp_norm(const ns_mlp *m);
+int mlp_device(const ns_mlp *m);
 
 struct Profiler;
 // Times the plain form as K_INTEGRATE, the binned forms' three kernels as

@@ -558,6 +558,8 @@ template <typename KeyT, int THREADS, int ITEMS>
 hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
                            hipStream_t stream, Profiler *prof)
 {
+    if (passes == 1)
+        return launch_pass<KeyT, THREADS, ITEMS, true, true>(w, src, vals_out, n, pass, stream, prof);
     if (pass == 0)
         return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream, prof);
     if (pass == passes - 1)
@@ -621,10 +623,23 @@ hipError_t seg_table_build(SegTable &t, int tile, int64_t n)
 }
 
 template <typename KeyT>
-hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
+hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof,
+                      int kPasses = (int)sizeof(KeyT), bool use_segments = true)
 {
-    constexpr int kPasses = (int)sizeof(KeyT);
     hipError_t e;
+    // the segment table is set aside for an unsegmented sort of this call
+    std::vector<int64_t> parked;
+    if (!use_segments)
+        parked.swap(w.seg.sizes);
+    struct Restore {
+        SortWork &w;
+        std::vector<int64_t> &v;
+        ~Restore()
+        {
+            if (!v.empty())
+                w.seg.sizes.swap(v);
+        }
+    } restore{w, parked};
     if (!w.seg.sizes.empty()) {
         if (w.config.alg != kSortReduceScan)
             return hipErrorNotSupported;
@@ -723,6 +738,15 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
     if (key_bits == 32)
         return sort_typed<uint32_t>(w, n, perm_out, stream, prof);
     return sort_typed<uint64_t>(w, n, perm_out, stream, prof);
+}
+
+hipError_t radix_sort_perm_u8(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    if (w.config.alg != kSortReduceScan)
+        return hipErrorNotSupported;
+    return sort_typed<uint32_t>(w, n, perm_out, stream, nullptr, 1, false);
 }
 
 } // namespace ns
