@@ -310,6 +310,8 @@ def main() -> int:
                     help="per-particle masses, so that M(r) comes from the enclosed-mass scan instead of "
                          "the rank (equal: M/N each; random: uniform in [0.5, 1.5] M/N, normalised)")
     ap.add_argument("--seed", type=int, default=20250401)
+    ap.add_argument("--dt", type=float, default=1e-3,
+                    help="base time step (the workloads are quoted at 1e-3)")
     ap.add_argument("--form", choices=sorted(FORMS), default="binned-fast")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="f32",
                     help="state precision of the chosen form (f32: float32 state and 32-bit keys, "
@@ -341,7 +343,7 @@ def main() -> int:
 
     kind, n_default, rmax, lowl = WORKLOADS[args.workload]
     t_ic = time.perf_counter()
-    p = ns.unit_params()
+    p = ns.unit_params(dt=args.dt)
     halos = None
     if kind < 0:
         state, halos = ensemble_ic(ns, args.seed, lowl=lowl)

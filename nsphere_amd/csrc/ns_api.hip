@@ -465,7 +465,6 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
         HIP_TRY(ns::radix_sort_perm_u8(c->sw, c->n, c->order, c->stream));
         c->prof.end(ns::K_MLP);
         a.order = c->order;
-        a.bin_count = c->sw.hist;
     }
     HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
     c->cur = d;

@@ -143,19 +143,9 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 {
     const int64_t t = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = t < a.n;
-    // learned binning: the rank of slot t in predicted-level order, and the
-    // predicted level (the bin holding slot t)
-    int64_t i = t;
-    int predicted = 0;
-    if (a.order && valid) {
-        i = a.order[t];
-        uint32_t end = 0;
-        for (predicted = 0; predicted < 7; predicted++) {
-            end += a.bin_count[predicted];
-            if ((uint32_t)t < end)
-                break;
-        }
-    }
+    // learned binning: slot t holds the rank order[t], ranks in stable order
+    // of predicted level, so that a wave's lanes share their expected work
+    const int64_t i = (a.order && valid) ? (int64_t)a.order[t] : t;
     bool to_lc = false, to_lf = false;
     unsigned long long spent = 0;
     if (valid) {
@@ -164,8 +154,6 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
         carry_ell_id<GATHER, S>(a, i, ell);
         if (takes_levi_civita(a, i, r, ell)) {
             to_lc = true;
-        } else if (a.order && ((int64_t)1 << predicted) > a.cap_N) {
-            to_lf = true;  // predicted past the lane levels: straight to a wave
         } else {
             const Phys ph = make_phys(a, i, ell);
             Budget bud{a.budget, false};

@@ -33,10 +33,9 @@ struct StepArgs {
     // per-particle masses (else null: M_enc from the rank, as the reference):
     // menc[i] = mass enclosed below rank i, menc[n + 1] = mass of rank 0
     const double *menc;
-    // learned binning (else null): bulk thread t takes rank order[t]; ranks
-    // are grouped by predicted level, bin b holding bin_count[b] of them
+    // learned binning (else null): bulk thread t takes rank order[t], the
+    // ranks in stable order of predicted level
     const uint32_t *order;
-    const uint32_t *bin_count;
     uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
     uint32_t *q_lc;              // Levi-Civita particles (wave per particle)
     uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave

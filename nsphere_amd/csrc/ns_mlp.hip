@@ -7,9 +7,9 @@
 // the network predicts a leapfrog particle's converged level from
 //   (log r, v_r, log ell, log M_enc, log dt),
 // normalised by the training set's mean and spread, and the integrator uses
-// the prediction only to schedule: particles are grouped by predicted level
-// and those predicted past the lane-per-particle levels go straight to a
-// wave.  The level search itself still starts at level 0 for every particle,
+// the prediction only to schedule: the bulk kernel's lanes take the ranks in
+// stable order of predicted level, so that a wave holds particles of like
+// work.  The level search itself still starts at level 0 for every particle,
 // so a misprediction costs time, never accuracy.
 //
 // Matrix layout (v_mfma_f32_16x16x16_bf16, one wave): A 16x16 with lane l
@@ -132,16 +132,18 @@ __device__ inline void forward(const Weights &W, v4s xb, v4f h1[4], v4f h2[4], v
     z = a;
 }
 
-// The five raw features of the particle at rank i (before normalisation).
+// The five raw features of the particle at rank i (before normalisation):
+// single-precision logarithms of the values rounded to float (the network's
+// inputs, not physics: training and inference both come through here).
 __device__ inline void raw_features(const StepArgs &a, int64_t i, double r, double v, double ell, float f[kIn])
 {
     const RankInfo k = rank_info(a, i);
     const double menc = a.menc ? a.menc[i] : ((double)k.i / (double)k.n) * k.M;
-    f[0] = (float)log(fmax(r, 1e-30));
+    f[0] = logf(fmaxf((float)r, 1e-30f));
     f[1] = (float)v;
-    f[2] = (float)log(fmax(fabs(ell), 1e-30));
-    f[3] = (float)log(fmax(menc, 1e-30));
-    f[4] = (float)log(a.dt);
+    f[2] = logf(fmaxf(fabsf((float)ell), 1e-30f));
+    f[3] = logf(fmaxf((float)menc, 1e-30f));
+    f[4] = logf((float)a.dt);
 }
 
 // Layer-1 B operand of this lane from the 16 samples' normalised features
@@ -158,61 +160,199 @@ __device__ inline v4s layer1_operand(const float f[kIn])
     return pack4(0.f, 0.f, 0.f, 0.f);
 }
 
-// argmax over the 8 logits of sample l % 16 (valid in lanes 0..15)
+// ---- inference ------------------------------------------------------------
+//
+// Layers 2 and 3 contract over 64 hidden features with the K = 32 BF16
+// instruction (v_mfma_f32_16x16x32_bf16: A lane l holds A[l % 16][8 (l/16) + j],
+// B lane l holds B[8 (l/16) + j][l % 16], j = 0..7).  The order of the
+// contraction index is free as long as A and B agree, so chunk kk's index
+// 8 lq + j is taken to be the hidden feature that lane group lq already holds
+// in D tiles 2 kk (j < 4) and 2 kk + 1 (j >= 4): the B operand is the two
+// tiles converted to BF16 as they stand, and the weights are gathered to
+// match once per wave.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+// Two floats to packed BF16 in one instruction (v_cvt_pk_bf16_f32, round to
+// nearest even, as to_bf16); with relu, negative values (sign bit set, a
+// negative 16-bit integer) become +0 by a packed 16-bit integer max -- the
+// same as ReLU before rounding, since rounding keeps the sign.
+template <bool RELU>
+__device__ inline v2s cvt2(float a, float b)
+{
+    const v2f x = {a, b};
+    v2s s = __builtin_bit_cast(v2s, __builtin_convertvector(x, v2bf));
+    if (RELU) {
+        const v2s zero = {0, 0};
+        s = __builtin_elementwise_max(s, zero);
+    }
+    return s;
+}
+
+__device__ inline v4f mfma32(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+__device__ inline int chunk_feature(int kk, int lq, int j) { return 32 * kk + 16 * (j >> 2) + 4 * lq + (j & 3); }
+
+// ReLU of two D tiles (pre-activations) as one K = 32 B operand
+__device__ inline v8bf relu_cat_bf16(v4f lo, v4f hi)
+{
+    const v2s p0 = cvt2<true>(lo[0], lo[1]), p1 = cvt2<true>(lo[2], lo[3]), p2 = cvt2<true>(hi[0], hi[1]),
+              p3 = cvt2<true>(hi[2], hi[3]);
+    const v8s s = {p0[0], p0[1], p1[0], p1[1], p2[0], p2[1], p3[0], p3[1]};
+    return __builtin_bit_cast(v8bf, s);
+}
+
+struct InferWeights {
+    v4s w1[4];       // A of layer 1 (K = 16): W1[16 t + lr][4 lq + j]
+    v8bf w2[4][2];   // A of layer 2, tile t, chunk kk: W2[16 t + lr][chunk_feature(kk, lq, j)]
+    v8bf w3[2];      // A of layer 3: W3[lr][chunk_feature(kk, lq, j)]
+    v4f b1[4], b2[4], b3;  // biases of D rows 4 lq + r (the accumulators' initial values)
+};
+
+__device__ inline void load_infer_weights(const float *__restrict__ P, InferWeights &W)
+{
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    for (int t = 0; t < 4; t++) {
+        const float *w1 = P + kW1 + (16 * t + lr) * kInPad + 4 * lq;
+        W.w1[t] = pack4(w1[0], w1[1], w1[2], w1[3]);
+        for (int kk = 0; kk < 2; kk++) {
+            v8s s;
+            for (int j = 0; j < 8; j++)
+                s[j] = (short)to_bf16(P[kW2 + (16 * t + lr) * kHid + chunk_feature(kk, lq, j)]);
+            W.w2[t][kk] = __builtin_bit_cast(v8bf, s);
+        }
+        for (int r = 0; r < 4; r++) {
+            W.b1[t][r] = P[kB1 + 16 * t + 4 * lq + r];
+            W.b2[t][r] = P[kB2 + 16 * t + 4 * lq + r];
+        }
+    }
+    for (int kk = 0; kk < 2; kk++) {
+        v8s s;
+        for (int j = 0; j < 8; j++)
+            s[j] = (short)to_bf16(P[kW3 + lr * kHid + chunk_feature(kk, lq, j)]);
+        W.w3[kk] = __builtin_bit_cast(v8bf, s);
+    }
+    for (int r = 0; r < 4; r++)
+        W.b3[r] = P[kB3 + 4 * lq + r];
+}
+
+// Logits of the 16 samples of one sub-group (D layout, rows 0..7 valid).
+// The pre-activations stay in the accumulators; ReLU is applied as they are
+// converted to the next layer's BF16 operand.
+__device__ inline v4f forward_infer(const InferWeights &W, v4s xb)
+{
+    v4f p1[4], p2[4];
+    for (int t = 0; t < 4; t++)
+        p1[t] = mfma(W.w1[t], xb, W.b1[t]);
+    const v8bf a0 = relu_cat_bf16(p1[0], p1[1]), a1 = relu_cat_bf16(p1[2], p1[3]);
+    for (int t = 0; t < 4; t++)
+        p2[t] = mfma32(W.w2[t][1], a1, mfma32(W.w2[t][0], a0, W.b2[t]));
+    const v8bf c0 = relu_cat_bf16(p2[0], p2[1]), c1 = relu_cat_bf16(p2[2], p2[3]);
+    return mfma32(W.w3[1], c1, mfma32(W.w3[0], c0, W.b3));
+}
+
+// argmax over the 8 logits of sample l % 16, valid in lanes 0..15: each of
+// lanes l and l + 16 takes the best of its four classes, then lane l keeps
+// the upper half's only if strictly larger -- the first of equal maxima, as
+// numpy's argmax.
 __device__ inline int argmax8(v4f z)
 {
-    const int lane = threadIdx.x & 63;
-    float v[8];
-    for (int r = 0; r < 4; r++) {
-        v[r] = z[r];
-        v[4 + r] = __shfl(z[r], (lane & 15) + 16, 64);
-    }
+    const int lq = (threadIdx.x & 63) >> 4;
+    float best_v = z[0];
     int best = 0;
-    for (int c = 1; c < 8; c++)
-        if (v[c] > v[best])
-            best = c;
-    return best;
+    for (int r = 1; r < 4; r++)
+        if (z[r] > best_v) {
+            best_v = z[r];
+            best = r;
+        }
+    best += 4 * lq;
+    const float other_v = __shfl_xor(best_v, 16, 64);
+    const int other = __shfl_xor(best, 16, 64);
+    return other_v > best_v ? other : best;
 }
 
 // Predicted level of every rank of a step (gathering through perm when set),
-// or of every row of a raw feature array X (when X is set).
+// or of every row of a raw feature array X (when X is set).  Each wave takes
+// 64 ranks at a time, one per lane for the loads and the features, then runs
+// them through the network as four sub-groups of 16; the next 64 ranks' loads
+// are issued before the current ones are evaluated.
 template <bool GATHER, typename S>
 __global__ __launch_bounds__(256) void k_mlp_predict(StepArgs a, const float *__restrict__ X,
                                                       const float *__restrict__ P, const float *__restrict__ norm,
                                                       uint32_t *__restrict__ pred)
 {
-    const int lane = threadIdx.x & 63;
-    Weights W;
-    load_weights(P, W);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    InferWeights W;
+    load_infer_weights(P, W);
     float mean[kIn], inv[kIn];
     for (int k = 0; k < kIn; k++) {
         mean[k] = norm[k];
         inv[k] = norm[kIn + k];
     }
-    const int64_t groups = (a.n + 15) / 16;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t g = wave; g < groups; g += waves) {
-        const int64_t i = g * 16 + (lane & 15);
-        float f[kIn] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        if (lane < 16 && i < a.n) {
-            if (X) {
-                for (int k = 0; k < kIn; k++)
-                    f[k] = X[i * kIn + k];
-            } else {
-                const int64_t src = GATHER ? (int64_t)a.perm[i] : i;
-                raw_features(a, i, (double)static_cast<const S *>(a.r0)[src],
-                             (double)static_cast<const S *>(a.v0)[src], (double)static_cast<const S *>(a.l0)[src],
-                             f);
-            }
+    const S *r0 = static_cast<const S *>(a.r0), *v0 = static_cast<const S *>(a.v0),
+            *l0 = static_cast<const S *>(a.l0);
+    // the next 64 ranks' inputs: the state at its own precision, or the five
+    // raw features
+    int64_t base = wave * 64;
+    S nr = 0, nv = 0, nl = 0;
+    float nx[kIn] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    auto prefetch = [&](int64_t b) {
+        const int64_t i = b + lane;
+        if (i >= a.n)
+            return;
+        if (X) {
+            for (int k = 0; k < kIn; k++)
+                nx[k] = X[i * kIn + k];
+        } else {
+            const int64_t src = GATHER ? (int64_t)a.perm[i] : i;
+            nr = r0[src];
+            nv = v0[src];
+            nl = l0[src];
+        }
+    };
+    if (base < a.n)
+        prefetch(base);
+    for (; base < a.n; base += waves * 64) {
+        const int64_t i = base + lane;
+        const S cr = nr, cv = nv, cl = nl;
+        float f[kIn];
+        for (int k = 0; k < kIn; k++)
+            f[k] = nx[k];
+        if (base + waves * 64 < a.n)
+            prefetch(base + waves * 64);
+        if (i < a.n) {
+            if (!X)
+                raw_features(a, i, (double)cr, (double)cv, (double)cl, f);
             for (int k = 0; k < kIn; k++)
                 f[k] = (f[k] - mean[k]) * inv[k];
+        } else {
+            for (int k = 0; k < kIn; k++)
+                f[k] = 0.f;
         }
-        v4f h1[4], h2[4], z;
-        forward(W, layer1_operand(f), h1, h2, z);
-        const int c = argmax8(z);
-        if (lane < 16 && i < a.n)
-            pred[i] = (uint32_t)c;
+        uint32_t mine = 0;
+        for (int q = 0; q < 4; q++) {
+            if (base + 16 * q >= a.n)  // wave-uniform
+                break;
+            const int src = 16 * q + lr;
+            float g[kIn];
+            for (int k = 0; k < kIn; k++)
+                g[k] = __shfl(f[k], src, 64);
+            const float x0 = lq == 0 ? g[0] : (lq == 1 ? g[4] : 0.f), x1 = lq == 0 ? g[1] : 0.f,
+                        x2 = lq == 0 ? g[2] : 0.f, x3 = lq == 0 ? g[3] : 0.f;
+            const v2s s01 = cvt2<false>(x0, x1), s23 = cvt2<false>(x2, x3);
+            const v4s xb = {s01[0], s01[1], s23[0], s23[1]};
+            const int c = argmax8(forward_infer(W, xb));
+            const int cq = __shfl(c, lr, 64);
+            if (lq == q)
+                mine = (uint32_t)cq;
+        }
+        if (i < a.n)
+            pred[i] = mine;
     }
 }
 

@@ -185,15 +185,19 @@ def loss_f64_of(Pvec64, norm, cw, X, y) -> float:
 
 
 def features(r, v, ell, dt, halo_mass=1.0) -> np.ndarray:
-    """Raw network inputs of a state in rank order (float64 math, float32 out)."""
+    """Raw network inputs of a state in rank order: the natural logarithm of
+    each value rounded to float32 (taken in float64 here and rounded to
+    float32; the kernels use a single-precision log, within 2 ulp of this)."""
     n = len(r)
     menc = (np.arange(n, dtype=np.float64) / np.float64(n)) * np.float64(halo_mass)
+    f32 = lambda x: np.asarray(x, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
+    tiny = np.float64(np.float32(1e-30))
     out = np.empty((n, N_IN), np.float32)
-    out[:, 0] = np.log(np.maximum(np.asarray(r, np.float64), 1e-30))
+    out[:, 0] = np.log(np.maximum(f32(r), tiny))
     out[:, 1] = np.asarray(v, np.float64)
-    out[:, 2] = np.log(np.maximum(np.abs(np.asarray(ell, np.float64)), 1e-30))
-    out[:, 3] = np.log(np.maximum(menc, 1e-30))
-    out[:, 4] = np.log(np.float64(dt))
+    out[:, 2] = np.log(np.maximum(np.abs(f32(ell)), tiny))
+    out[:, 3] = np.log(np.maximum(f32(menc), tiny))
+    out[:, 4] = np.log(f32(dt))
     return out
 
 

