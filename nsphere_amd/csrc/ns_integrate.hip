@@ -124,11 +124,12 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
         carry_ell_id<GATHER, S>(a, i, ell);
-        const Phys ph = make_phys(a, i, ell);
+        const RankMass rm = rank_mass(a, i);
+        const Phys ph = make_phys(a, i, rm, ell);
         Budget bud{a.budget, false};
         double r_new, v_new;
         int rec;
-        if (takes_levi_civita(a, i, r, ell))
+        if (takes_levi_civita(a, rm, r, ell))
             rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
         else
             rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
@@ -152,10 +153,11 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
         carry_ell_id<GATHER, S>(a, i, ell);
-        if (takes_levi_civita(a, i, r, ell)) {
+        const RankMass rm = rank_mass(a, i);
+        if (takes_levi_civita(a, rm, r, ell)) {
             to_lc = true;
         } else {
-            const Phys ph = make_phys(a, i, ell);
+            const Phys ph = make_phys(a, i, rm, ell);
             Budget bud{a.budget, false};
             double r_new, v_new;
             const int rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_N);

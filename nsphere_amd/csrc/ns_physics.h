@@ -69,34 +69,77 @@ __device__ inline double enclosed_mass(const StepArgs &a, const RankInfo &k)
     return a.menc ? a.menc[k.i] : ((double)k.i / (double)k.n) * k.M;
 }
 
-__device__ inline Phys make_phys(const StepArgs &a, int64_t i, double ell)
+// What a particle's force needs of its rank: its halo's rank and count, and
+// the rank fraction i / npts of the reference's enclosed mass (one division,
+// shared by the force constants and the r_crit switch).
+struct RankMass {
+    RankInfo k;
+    double frac;  // (double) i / (double) npts
+};
+
+__device__ inline RankMass rank_mass(const StepArgs &a, int64_t i)
+{
+    const RankInfo k = rank_info(a, i);
+    return {k, a.menc ? 0.0 : (double)k.i / (double)k.n};
+}
+
+__device__ inline Phys make_phys(const StepArgs &a, int64_t i, const RankMass &rm, double ell)
 {
     Phys ph;
     ph.idg = a.identity_gravity != 0;
-    const RankInfo k = rank_info(a, i);
     // the reference's association, ((-(vel_conv_sq G)) (i / npts)) M
-    ph.gm = a.menc ? -(a.vel_conv_sq * a.G) * a.menc[i]
-                   : -(a.vel_conv_sq * a.G) * ((double)k.i / (double)k.n) * k.M;
+    ph.gm = a.menc ? -(a.vel_conv_sq * a.G) * a.menc[i] : -(a.vel_conv_sq * a.G) * rm.frac * rm.k.M;
     ph.l2 = ell * ell;
     return ph;
 }
 
+__device__ inline Phys make_phys(const StepArgs &a, int64_t i, double ell)
+{
+    return make_phys(a, i, rank_mass(a, i), ell);
+}
+
 // r_crit switch of nsphere.c:5803-5828.
-__device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r, double ell)
+__device__ inline bool takes_levi_civita(const StepArgs &a, const RankMass &rm, double r, double ell)
 {
     double r_crit = 0.0;
     if (fabs(ell) > 1.0e-30) {
         // rank 0: a tenth of one particle's mass (with masses: of its own)
-        const RankInfo k = rank_info(a, i);
         double M_enc;
-        if (k.i == 0)
-            M_enc = a.menc ? 0.1 * a.menc[a.n + 1] : 0.1 * (1.0 / (double)k.n) * k.M;
+        if (rm.k.i == 0)
+            M_enc = a.menc ? 0.1 * a.menc[a.n + 1] : 0.1 * (1.0 / (double)rm.k.n) * rm.k.M;
         else
-            M_enc = enclosed_mass(a, k);
+            M_enc = a.menc ? a.menc[rm.k.i] : rm.frac * rm.k.M;  // enclosed_mass
         const double grav_part = (a.vel_conv_sq * a.G) * M_enc;
         r_crit = (ell * ell) * a.alpha / grav_part;
     }
     return r > 1.0e-30 && r < r_crit;
+}
+
+__device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r, double ell)
+{
+    return takes_levi_civita(a, rank_mass(a, i), r, ell);
+}
+
+// h / 2^e for the power-of-two divisors of the leapfrog passes: exact, as
+// the division it replaces (no result here comes near the subnormals).
+__device__ inline double div_pow2(double h, int e) { return ldexp(h, -e); }
+
+// log2 of a power of two
+__device__ inline int ilog2(int64_t N) { return 63 - __clzll((unsigned long long)N); }
+
+// The reference's convergence test (nsphere.c:1700-1712): relative change
+// of the fine over the coarse pass below both tolerances.  The fast form
+// multiplies instead of dividing, which differs from it only where a
+// quotient rounds across the tolerance.
+template <bool FAST>
+__device__ inline bool converged(const StepArgs &a, double rc, double vc, double rf, double vf)
+{
+    if (FAST)
+        return fabs(rf - rc) < a.radius_tol * (fabs(rf) + 1.0e-30) &&
+               fabs(vf - vc) < a.velocity_tol * (fabs(vf) + 1.0e-30);
+    const double rd = fabs(rf - rc) / (fabs(rf) + 1.0e-30);
+    const double vd = fabs(vf - vc) / (fabs(vf) + 1.0e-30);
+    return (rd < a.radius_tol) && (vd < a.velocity_tol);
 }
 
 // Fast-form reciprocal: v_rcp_f64 (y = (1 + d)/x, |d| < 2^-24 on gfx950)
@@ -204,9 +247,10 @@ template <bool FAST>
 __device__ inline void micro_leap(const Phys &ph, double r_in, double v_in, double h, int64_t N, bool fine,
                                   double &r_out, double &v_out)
 {
-    const double Nd = (double)N;
-    const double hk = fine ? h / (4.0 * Nd) : h / (2.0 * Nd);
-    const double ms = fine ? h / (2.0 * Nd) : h / (1.0 * Nd);
+    // h / (4N), h / (2N) (fine) or h / (2N), h / N (coarse); N = 2^e
+    const int e = ilog2(N) + (fine ? 1 : 0);
+    const double hk = div_pow2(h, e + 1);
+    const double ms = div_pow2(h, e);
     const int64_t pairs = fine ? 2 * N : N;
     const Kick kh = make_kick(ph, hk), km = make_kick(ph, ms);
     double r = r_in, v = v_in;
@@ -266,9 +310,7 @@ __device__ int adaptive_leap(const Phys &ph, const StepArgs &a, double r_in, dou
         }
         micro_leap<FAST>(ph, r_in, v_in, h, N, true, rf, vf);
         last = level_of(N);
-        const double rd = fabs(rf - rc) / (fabs(rf) + 1.0e-30);
-        const double vd = fabs(vf - vc) / (fabs(vf) + 1.0e-30);
-        if ((rd < a.radius_tol) && (vd < a.velocity_tol)) {
+        if (converged<FAST>(a, rc, vc, rf, vf)) {
             leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
             return last;
         }
@@ -305,9 +347,10 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
     const bool coarse_lane = lane == 63;
     const bool active = coarse_lane || lane <= kmax;
     const int64_t N = coarse_lane ? 1 : ((int64_t)1 << (lane <= kmax ? lane : 0));
-    const double Nd = (double)N;
-    const double hk = coarse_lane ? h / (2.0 * Nd) : h / (4.0 * Nd);
-    const double ms = coarse_lane ? h / (1.0 * Nd) : h / (2.0 * Nd);
+    // h / (2N), h / N (coarse) or h / (4N), h / (2N) (fine); N = 2^e
+    const int e = coarse_lane ? 0 : (lane <= kmax ? lane : 0) + 1;
+    const double hk = div_pow2(h, e + 1);
+    const double ms = div_pow2(h, e);
     const int64_t pairs = coarse_lane ? N : 2 * N;
     const Kick kh = make_kick(ph, hk), km = make_kick(ph, ms);
     double r = r_in, v = v_in;
@@ -350,9 +393,7 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
         const int src_c = (k == 0) ? 63 : k - 1;
         const double rc = __shfl(r, src_c, 64), vc = __shfl(v, src_c, 64);
         const double rf = __shfl(r, k, 64), vf = __shfl(v, k, 64);
-        const double rd = fabs(rf - rc) / (fabs(rf) + 1.0e-30);
-        const double vd = fabs(vf - vc) / (fabs(vf) + 1.0e-30);
-        if ((rd < a.radius_tol) && (vd < a.velocity_tol)) {
+        if (converged<FAST>(a, rc, vc, rf, vf)) {
             leap_output(a.out_type, rc, vc, rf, vf, r_out, v_out);
             return level_of(Nk);
         }
