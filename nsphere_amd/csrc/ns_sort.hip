@@ -219,25 +219,56 @@ __global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict_
     }
     if (end > n)
         end = n;
-    const int64_t wave_base = start + (int64_t)wave * (ITEMS * 64);
+    // Counting ignores order, so a full unsegmented tile is read 16 bytes per
+    // lane (the tile start is a multiple of the tile, hence aligned); a
+    // segment's or the last, partial tile key by key.
+    constexpr int VEC = 16 / (int)sizeof(KeyT);
+    constexpr bool kVector = ITEMS % VEC == 0;
     KeyT key[ITEMS];
+    bool full = false;
+    if constexpr (kVector)
+        full = !tile_start && end - start == (int64_t)THREADS * ITEMS;
+    if (full) {
+        typedef KeyT vec_t __attribute__((ext_vector_type(VEC)));
+        const vec_t *src = reinterpret_cast<const vec_t *>(keys + start);
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        key[j] = idx < end ? keys[idx] : (KeyT)0;
-    }
+        for (int j = 0; j < ITEMS / VEC; j++) {
+            const vec_t x = src[j * THREADS + tid];  // kept in cache: the scatter re-reads the keys
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        // valid lanes are a prefix of the wave (lane 0 first)
-        const bool valid = wave_base + (int64_t)j * 64 + lane < end;
-        const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // whole wave: no divergence yet
-        if (__ballot(valid && d != d0) == 0) {
-            if (lane == 0 && valid)
-                atomicAdd(&s_cnt[wave][d0], nvalid);
-        } else if (valid) {
-            atomicAdd(&s_cnt[wave][d], 1u);
+            for (int k = 0; k < VEC; k++)
+                key[j * VEC + k] = x[k];
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (__ballot(d != d0) == 0) {
+                if (lane == 0)
+                    atomicAdd(&s_cnt[wave][d0], 64u);
+            } else {
+                atomicAdd(&s_cnt[wave][d], 1u);
+            }
+        }
+    } else {
+        const int64_t wave_base = start + (int64_t)wave * (ITEMS * 64);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+            key[j] = idx < end ? keys[idx] : (KeyT)0;
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            // valid lanes are a prefix of the wave (lane 0 first)
+            const bool valid = wave_base + (int64_t)j * 64 + lane < end;
+            const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // whole wave: no divergence yet
+            if (__ballot(valid && d != d0) == 0) {
+                if (lane == 0 && valid)
+                    atomicAdd(&s_cnt[wave][d0], nvalid);
+            } else if (valid) {
+                atomicAdd(&s_cnt[wave][d], 1u);
+            }
         }
     }
     __syncthreads();
@@ -342,9 +373,14 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 {
     constexpr int WAVES = THREADS / 64;
     constexpr int TILE = THREADS * ITEMS;
+    // 32-bit keys: keys and payload staged together and written in one sweep
+    // (a slot's digit comes from its staged key); 64-bit keys: keys and their
+    // digits first, then the payload in the key buffer
+    constexpr bool kOneSweep = sizeof(KeyT) == 4;
     static_assert(THREADS >= kRadix, "one thread per digit");
-    __shared__ KeyT s_stage[TILE];          // keys in digit order, then the payload
-    __shared__ uint8_t s_dig[TILE];         // digit of each staged slot
+    __shared__ KeyT s_stage[TILE];                     // keys in digit order
+    __shared__ uint32_t s_pay1[kOneSweep ? TILE : 1];  // payload in digit order (one sweep)
+    __shared__ uint8_t s_dig[kOneSweep ? 1 : TILE];    // digit of each staged slot (two sweeps)
     __shared__ uint32_t s_wcnt[WAVES][kRadix];
     __shared__ uint32_t s_off[kRadix];      // global base minus tile-local start, per digit
     __shared__ uint32_t s_lstart[kRadix];   // tile-local digit start
@@ -355,6 +391,15 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
 
+    // reduce-then-scan: the tile is the block's own, so its scanned digit
+    // offsets are requested now and arrive while the keys are ranked
+    uint32_t pre_excl = 0;
+    if (!LOOKBACK && tid < kRadix) {
+        const uint32_t t = xcd_contiguous_block(blockIdx.x, gridDim.x);
+        pre_excl = tile_offsets[(int64_t)t * kRadix + tid];
+        if (seg.tile_start)
+            pre_excl += seg.base[(int64_t)seg.tile_seg[t] * kRadix + tid];
+    }
     if (tid == 0)
         s_tile = LOOKBACK ? atomicAdd(tile_counter, 1u) : xcd_contiguous_block(blockIdx.x, gridDim.x);
     for (int i = tid; i < WAVES * kRadix; i += THREADS)
@@ -421,9 +466,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
             total += c;
         }
         if (!LOOKBACK) {
-            excl = tile_offsets[(int64_t)tile * kRadix + d];
-            if (seg.tile_start)
-                excl += seg.base[(int64_t)seg.tile_seg[tile] * kRadix + d];
+            excl = pre_excl;
         } else if (tile == 0) {
             lb_store(lookback + d, lb_pack(epoch, kStatusPrefix, total));
         } else {
@@ -455,41 +498,61 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
     }
     __syncthreads();
 
+    // slot in the tile, digit order
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
-        pos[j] += s_lstart[d] + s_wcnt[wave][d];  // slot in the tile, digit order
+        pos[j] += s_lstart[d] + s_wcnt[wave][d];
     }
     const int count = (int)(tile_end - tile_base);
 
-    if (!LAST) {
-        // keys through LDS in digit order, then out in coalesced digit runs
+    if constexpr (kOneSweep) {
+        // keys and payload through LDS, out in coalesced digit runs
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const int64_t idx = wave_base + (int64_t)j * 64 + lane;
             if (idx < tile_end) {
                 s_stage[pos[j]] = key[j];
-                s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
+                s_pay1[pos[j]] = val[j];
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < count; j += THREADS) {
+            const KeyT k = s_stage[j];
+            const uint32_t dst = s_off[(uint32_t)(k >> shift) & (kRadix - 1)] + (uint32_t)j;
+            if (!LAST)
+                keys_out[dst] = k;
+            vals_out[dst] = s_pay1[j];
+        }
+    } else {
+        if (!LAST) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+                if (idx < tile_end) {
+                    s_stage[pos[j]] = key[j];
+                    s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
+                }
+            }
+            __syncthreads();
+            for (int j = tid; j < count; j += THREADS)
+                keys_out[s_off[s_dig[j]] + (uint32_t)j] = s_stage[j];
+            __syncthreads();
+        }
+        uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_stage);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int64_t idx = wave_base + (int64_t)j * 64 + lane;
+            if (idx < tile_end) {
+                s_pay[pos[j]] = val[j];
+                if (LAST)
+                    s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
             }
         }
         __syncthreads();
         for (int j = tid; j < count; j += THREADS)
-            keys_out[s_off[s_dig[j]] + (uint32_t)j] = s_stage[j];
-        __syncthreads();
+            vals_out[s_off[s_dig[j]] + (uint32_t)j] = s_pay[j];
     }
-    uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_stage);
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int64_t idx = wave_base + (int64_t)j * 64 + lane;
-        if (idx < tile_end) {
-            s_pay[pos[j]] = val[j];
-            if (LAST)
-                s_dig[pos[j]] = (uint8_t)((key[j] >> shift) & (kRadix - 1));
-        }
-    }
-    __syncthreads();
-    for (int j = tid; j < count; j += THREADS)
-        vals_out[s_off[s_dig[j]] + (uint32_t)j] = s_pay[j];
 }
 
 template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
