@@ -268,6 +268,50 @@ def train_level_predictor(ns, kind: int, n_log: int, rmax: float, lowl: float, s
                  "held_out_confusion": conf.tolist()}
 
 
+def measure_snapshots(ns, precision: str, state, p, every: int, steps: int, work_cap: int, path: str) -> dict:
+    """The snapshot path beside the step: `steps` steps of the chosen form
+    with an all_particle_data snapshot (ns_apd_writer) every `every` steps,
+    against the same steps without; the records kernel timed with HIP events."""
+    n = state[0].size
+    out = {"every": every, "steps": steps, "path": path, "bytes_per_snapshot": 16 * n}
+    for with_snap in (False, True):
+        ctx = ns.GpuContext(n)
+        try:
+            ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS["binned-fast"])
+            ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
+            ctx.upload(*state)
+            ctx.run(p, 2)
+            ctx.sync()
+            w = ns.ApdWriter(ctx, path, block_size=1) if with_snap else None
+            ctx.set_option(ns.NS_OPT_PROFILE, 1)
+            ctx.reset_kernel_stats()
+            t0 = time.perf_counter()
+            for _ in range(steps // every):
+                ctx.run(p, every)
+                if w is not None:
+                    w.write()
+            kept = w.close() if w is not None else 0
+            ctx.sync()
+            el = time.perf_counter() - t0
+            st = ctx.kernel_stats()
+        finally:
+            ctx.close()
+        key = "with" if with_snap else "without"
+        out[f"ms_per_step_{key}"] = el / (steps // every * every) * 1e3
+        if with_snap:
+            launches, ms = st.get("snapshot_records", (0, 0.0))
+            avg = ms / max(launches, 1) * 1e-3
+            out["snapshots"] = kept
+            out["records_kernel_us"] = round(avg * 1e6, 2)
+            # algorithmic bytes: read r, v, ell, id at the state's width, write a 16 B record
+            bpp = (3 * (4 if precision == "f32" else 8) + 4) + 16
+            out["records_kernel_GBps"] = round(bpp * n / avg / 1e9, 1) if avg > 0 else None
+            out["records_bytes_per_particle"] = bpp
+    out["cost_per_snapshot_ms"] = (out["ms_per_step_with"] - out["ms_per_step_without"]) * every
+    return out
+
+
 def traffic_for(run: dict, n: int, kernel: str):
     return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
 
@@ -327,6 +371,10 @@ def main() -> int:
                     help="learned binning: log levels of the plain adaptive integrator on a separate "
                          "N_LOG-particle run of the same recipe (10 steps), train the level predictor on "
                          "them with the library's kernels, and time the chosen form with and without it")
+    ap.add_argument("--snapshot-every", type=int, default=0, metavar="K",
+                    help="also time the snapshot path: an all_particle_data snapshot every K steps "
+                         "(ns_apd_writer, written to --snapshot-path) against the same steps without")
+    ap.add_argument("--snapshot-path", default="/dev/null")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="autotune the chosen form's launch configuration first (STEPS timed steps per trial)")
     ap.add_argument("--budget", type=int, default=1 << 18,
@@ -415,6 +463,11 @@ def main() -> int:
     if learned is not None:
         out["config"]["learned_binning"] = True
         out["learned_binning"] = learned
+    if args.snapshot_every:
+        print("[bench] timing the snapshot path", file=sys.stderr, flush=True)
+        out["snapshot"] = measure_snapshots(ns, args.precision, state, p, args.snapshot_every,
+                                            max(args.snapshot_every, min(args.steps, 50)), args.budget,
+                                            args.snapshot_path)
     if plain is not None:
         out["plain_fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                              "kernels": plain["kernels"],

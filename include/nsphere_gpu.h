@@ -37,7 +37,8 @@ enum {
     NS_ERR_HIP = 2,     /* a HIP runtime call failed                           */
     NS_ERR_NOMEM = 3,   /* device or host allocation failed                    */
     NS_ERR_STATE = 4,   /* call out of order (e.g. step before upload)         */
-    NS_ERR_NODEV = 5    /* no usable gfx950 device                             */
+    NS_ERR_NODEV = 5,   /* no usable gfx950 device                             */
+    NS_ERR_IO = 6       /* a file could not be opened, written or read         */
 };
 
 /* Parameters of one step.  Field meaning follows the method-5 constants of
@@ -215,6 +216,46 @@ int ns_gpu_set_mlp(ns_gpu_ctx *ctx, const ns_mlp *mlp);
  * as rank order (as after ns_gpu_sort_particles): host_out[npts][5]. */
 int ns_gpu_features(ns_gpu_ctx *ctx, const ns_step_params *p, float *host_out);
 
+/* ---- snapshot path (nsphere.c:6452-6533, 1295-1324) ---------------------
+ *
+ * One record of all_particle_data<suffix>.dat: the particle's rank (its
+ * position in the rank-ordered arrays, inverse_map[id]) and its r, v_r, ell
+ * rounded to float32 -- the int32 + 3 x float32 written by
+ * append_all_particle_data_chunk_to_file (nsphere.c:1295-1324). */
+typedef struct ns_apd_record {
+    int32_t rank;
+    float r, v, ell;
+} ns_apd_record;
+
+/* The records of every particle of the current state in original-id order
+ * (record k for id k), as the per-dtwrite block fills them
+ * (nsphere.c:6456-6471); ids must be a permutation of 0..npts-1.
+ * host_out: npts records. */
+int ns_gpu_snapshot_records(ns_gpu_ctx *ctx, ns_apd_record *host_out);
+
+/* r, v_r, ell and rank of k particles given by original id (the tracked
+ * particles of the reference's diagnostics); rank -1 for an id not held. */
+int ns_gpu_gather_ids(ns_gpu_ctx *ctx, const uint32_t *ids, int k, double *r, double *v, double *ell,
+                      int32_t *rank);
+
+/* all_particle_data writer: each ns_apd_writer_write appends the current
+ * state's records as one snapshot (npts x 16 bytes) to the file, the
+ * snapshot-major layout of append_all_particle_data_chunk_to_file; snapshot s
+ * lies at byte s * npts * 16 from the start of the writer's output, the
+ * offset of the reference's reader (nsphere.c:1380-1385).  The records
+ * kernel runs on the context's stream into one of two device buffers and the
+ * copy to pinned host memory on a stream of the writer's own, so steps go on
+ * while a snapshot travels; the file write of a snapshot happens on the next
+ * call (or at close).  As the reference appends only full blocks of
+ * block_size snapshots, close keeps whole blocks and cuts a partial last
+ * one.  The file is opened for appending, as the reference opens it. */
+typedef struct ns_apd_writer ns_apd_writer;
+int ns_apd_writer_open(ns_gpu_ctx *ctx, const char *path, int block_size, ns_apd_writer **out);
+int ns_apd_writer_write(ns_apd_writer *w);
+/* Writes what is pending, cuts a partial block, frees the writer; kept
+ * (may be NULL) receives the number of snapshots in the file's whole blocks. */
+int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept);
+
 /* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
  * [npts] the total, [npts + 1] the rank-0 particle's mass. */
 int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);
@@ -254,7 +295,7 @@ int ns_gpu_energy(ns_gpu_ctx *ctx, const ns_step_params *p, double *K, double *W
 /* Kernel timing (NS_OPT_PROFILE=1): for each kernel family, the number of
  * launches and total milliseconds measured with HIP events on the context's
  * stream since the last reset.  names[i] point at static strings. */
-#define NS_MAX_KERNEL_STATS 16
+#define NS_MAX_KERNEL_STATS 32 /* the arrays hold this many entries */
 int ns_gpu_kernel_stats(ns_gpu_ctx *ctx, int *count, const char **names,
                         int64_t *launches, double *total_ms);
 int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);

@@ -29,7 +29,8 @@ __all__ = [
     "sort_particles", "method5_step", "philox4x32_10",
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
-    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "Mlp", "level_samples", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
+    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "Mlp", "level_samples", "ApdWriter", "APD_RECORD",
+    "read_all_particle_data", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]
 
 IC_PLUMMER, IC_HERNQUIST, IC_MIX = 0, 1, 2
@@ -183,9 +184,10 @@ class GpuContext:
 
     def kernel_stats(self) -> dict:
         n = ctypes.c_int()
-        names = (ctypes.c_char_p * 16)()
-        launches = (ctypes.c_int64 * 16)()
-        ms = (ctypes.c_double * 16)()
+        from ._lib import NS_MAX_KERNEL_STATS as cap
+        names = (ctypes.c_char_p * cap)()
+        launches = (ctypes.c_int64 * cap)()
+        ms = (ctypes.c_double * cap)()
         check(lib.ns_gpu_kernel_stats(self._h, ctypes.byref(n), names, launches, ms))
         return {names[i].decode(): (launches[i], ms[i]) for i in range(n.value) if launches[i]}
 
@@ -222,6 +224,23 @@ class GpuContext:
         check(lib.ns_gpu_autotune(self._h, ctypes.byref(p), int(steps), ctypes.byref(res)))
         return {"best": res.best.as_dict(), "tried": [res.tried[i].as_dict() for i in range(res.ntried)]}
 
+    def snapshot_records(self) -> np.ndarray:
+        """all_particle_data records of the current state in id order
+        (rank, float32 r, v_r, ell), as nsphere.c:6456-6471 fills them."""
+        out = np.empty(self.npts, APD_RECORD)
+        check(lib.ns_gpu_snapshot_records(self._h, out.ctypes.data))
+        return out
+
+    def gather_ids(self, ids):
+        """(r, v_r, ell, rank) of the particles with these original ids."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        k = ids.size
+        r, v, ell = (np.empty(k) for _ in range(3))
+        rank = np.empty(k, np.int32)
+        check(lib.ns_gpu_gather_ids(self._h, ids.ctypes.data, k, r.ctypes.data, v.ctypes.data, ell.ctypes.data,
+                                    rank.ctypes.data))
+        return r, v, ell, rank
+
     def set_mlp(self, mlp) -> None:
         """Learned binning with a trained Mlp (None: off)."""
         check(lib.ns_gpu_set_mlp(self._h, None if mlp is None else mlp._h))
@@ -244,6 +263,44 @@ class GpuContext:
         out = (ctypes.c_uint64 * 2)()
         check(lib.ns_gpu_force_evals(self._h, out))
         return {"bulk_or_plain": out[0], "deep": out[1]}
+
+
+APD_RECORD = np.dtype([("rank", "<i4"), ("r", "<f4"), ("v", "<f4"), ("ell", "<f4")])
+
+
+class ApdWriter:
+    """Appends snapshots of a context's state to all_particle_data<suffix>.dat
+    in the reference's format (16-byte records, snapshot-major, whole blocks
+    of block_size snapshots kept at close)."""
+
+    def __init__(self, ctx: "GpuContext", path: str, block_size: int = 100):
+        h = ctypes.c_void_p()
+        check(lib.ns_apd_writer_open(ctx._h, str(path).encode(), int(block_size), ctypes.byref(h)))
+        self._h, self._ctx = h, ctx
+
+    def write(self) -> None:
+        check(lib.ns_apd_writer_write(self._h))
+
+    def close(self) -> int:
+        """Finish the file; returns the number of snapshots kept."""
+        if not self._h:
+            return 0
+        kept = ctypes.c_int64()
+        h, self._h = self._h, None
+        check(lib.ns_apd_writer_close(h, ctypes.byref(kept)))
+        return kept.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_all_particle_data(path: str, npts: int, snap: int) -> np.ndarray:
+    """Snapshot `snap` of an all_particle_data file (offset snap * npts * 16,
+    as the reference's reader computes it, nsphere.c:1380-1385)."""
+    return np.fromfile(path, APD_RECORD, count=npts, offset=int(snap) * int(npts) * APD_RECORD.itemsize)
 
 
 class Mlp:

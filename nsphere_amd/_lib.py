@@ -19,7 +19,7 @@ NS_OPT_LANE_LEVELS, NS_OPT_DEEP_BLOCKS = 9, 10
 SORT_REDUCE_SCAN, SORT_ONESWEEP = 0, 1
 FORM_PLAIN, FORM_BINNED, FORM_BINNED_FAST = 0, 1, 2
 NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80
-NS_MAX_KERNEL_STATS = 16
+NS_MAX_KERNEL_STATS = 32
 
 
 class NativeLibraryMissing(ImportError):
@@ -122,6 +122,11 @@ def _load() -> ctypes.CDLL:
         "ns_mlp_get_params": (i32, [vp, vp, vp, vp]),
         "ns_mlp_set_params": (i32, [vp, vp, vp, vp]),
         "ns_gpu_set_mlp": (i32, [vp, vp]),
+        "ns_gpu_snapshot_records": (i32, [vp, vp]),
+        "ns_gpu_gather_ids": (i32, [vp, vp, i32, vp, vp, vp, vp]),
+        "ns_apd_writer_open": (i32, [vp, c.c_char_p, i32, c.POINTER(vp)]),
+        "ns_apd_writer_write": (i32, [vp]),
+        "ns_apd_writer_close": (i32, [vp, c.POINTER(i64)]),
         "ns_gpu_features": (i32, [vp, c.POINTER(StepParams), vp]),
         "ns_gpu_enclosed_mass": (i32, [vp, vp]),
         "ns_ic_generate": (i32, [i32, i64, c.c_uint64, dbl, dbl, dbl, vp, vp, vp, vp]),

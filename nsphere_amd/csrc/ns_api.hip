@@ -14,6 +14,8 @@
 #include <cstring>
 #include <new>
 
+#include <unistd.h>
+
 #include "ns_device.h"
 #include "ns_internal.h"
 #include "ns_kernels.h"
@@ -97,7 +99,8 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"radius_keys", "sort_zero", "sort_histogram", "sort_scatter",
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
-                                         "integrate_deep", "mass_scan", "mlp_binning"};
+                                         "integrate_deep", "mass_scan", "mlp_binning",
+                                         "snapshot_records"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -835,6 +838,81 @@ int ns_gpu_inverse_map(ns_gpu_ctx *c, int32_t *host_out)
     return NS_OK;
 }
 
+int ns_gpu_snapshot_records(ns_gpu_ctx *c, ns_apd_record *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_snapshot_records"))
+        return rc;
+    if (!host_out)
+        return set_error(NS_ERR_ARG, "ns_gpu_snapshot_records: null output");
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_snapshot_records: no state uploaded");
+    if (c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, (size_t)c->n * sizeof(ns_apd_record)));
+    const int s = c->cur;
+    c->prof.begin(ns::K_APD);
+    hipError_t e = c->f32 ? ns::launch_apd_records(c->rf[s], c->vf[s], c->lf[s], true, c->id[s], c->n, d, c->stream)
+                          : ns::launch_apd_records(c->r[s], c->v[s], c->l[s], false, c->id[s], c->n, d, c->stream);
+    c->prof.end(ns::K_APD);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host_out, d, (size_t)c->n * sizeof(ns_apd_record), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_snapshot_records: %s", hipGetErrorString(e));
+    return NS_OK;
+}
+
+int ns_gpu_gather_ids(ns_gpu_ctx *c, const uint32_t *ids, int k, double *r, double *v, double *ell, int32_t *rank)
+{
+    if (int rc = check_ctx(c, "ns_gpu_gather_ids"))
+        return rc;
+    if (k < 0 || (k > 0 && !ids))
+        return set_error(NS_ERR_ARG, "ns_gpu_gather_ids: bad ids");
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_gather_ids: no state uploaded");
+    if (k == 0 || c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    HIP_TRY(ns::launch_inverse_map(c->id[c->cur], c->inv, c->n, c->stream));
+    uint32_t *d_ids = nullptr;
+    double *d_out = nullptr;
+    HIP_TRY(hipMalloc((void **)&d_ids, (size_t)k * sizeof(uint32_t)));
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)k * 4 * sizeof(double));
+    std::vector<double> h((size_t)k * 4);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_ids, ids, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+    const int s = c->cur;
+    if (e == hipSuccess)
+        e = c->f32 ? ns::launch_gather_ids(c->rf[s], c->vf[s], c->lf[s], true, c->inv, c->n, d_ids, k, d_out,
+                                           c->stream)
+                   : ns::launch_gather_ids(c->r[s], c->v[s], c->l[s], false, c->inv, c->n, d_ids, k, d_out,
+                                           c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_ids);
+    if (d_out)
+        (void)hipFree(d_out);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_gather_ids: %s", hipGetErrorString(e));
+    for (int j = 0; j < k; j++) {
+        if (r)
+            r[j] = h[4 * j];
+        if (v)
+            v[j] = h[4 * j + 1];
+        if (ell)
+            ell[j] = h[4 * j + 2];
+        if (rank)
+            rank[j] = (int32_t)h[4 * j + 3];
+    }
+    return NS_OK;
+}
+
 int ns_gpu_integrate(ns_gpu_ctx *c, const ns_step_params *p)
 {
     if (int rc = check_ctx(c, "ns_gpu_integrate"))
@@ -1007,6 +1085,7 @@ int ns_gpu_kernel_stats(ns_gpu_ctx *c, int *count, const char **names, int64_t *
         return rc;
     if (!count)
         return set_error(NS_ERR_ARG, "ns_gpu_kernel_stats: null count");
+    static_assert(ns::K_COUNT <= NS_MAX_KERNEL_STATS, "kernel families exceed the documented array size");
     DeviceGuard g(c->device);
     HIP_TRY(c->prof.fold());
     *count = ns::K_COUNT;
@@ -1396,6 +1475,165 @@ int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[2])
     }
     free(h);
     return NS_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------ snapshot writer
+
+struct ns_apd_writer {
+    ns_gpu_ctx *ctx = nullptr;
+    FILE *f = nullptr;
+    int block_size = 1;
+    int64_t n = 0;
+    long long start = 0;     // file size when opened (the writer appends)
+    int64_t written = 0;     // snapshots appended to the file
+    void *d_rec[2] = {};     // device records, one buffer per slot
+    void *h_rec[2] = {};     // pinned host copies
+    hipStream_t copy = nullptr;
+    hipEvent_t made[2] = {}, copied[2] = {};
+    bool in_flight[2] = {};  // slot copied (or copying) but not yet in the file
+    int slot = 0;
+};
+
+namespace {
+
+void apd_release(ns_apd_writer *w)
+{
+    for (int k = 0; k < 2; k++) {
+        if (w->d_rec[k])
+            (void)hipFree(w->d_rec[k]);
+        if (w->h_rec[k])
+            (void)hipHostFree(w->h_rec[k]);
+        if (w->made[k])
+            (void)hipEventDestroy(w->made[k]);
+        if (w->copied[k])
+            (void)hipEventDestroy(w->copied[k]);
+    }
+    if (w->copy)
+        (void)hipStreamDestroy(w->copy);
+    if (w->f)
+        fclose(w->f);
+    delete w;
+}
+
+// the snapshot of slot k: wait for its copy, then append it to the file
+int apd_flush_slot(ns_apd_writer *w, int k)
+{
+    if (!w->in_flight[k])
+        return NS_OK;
+    HIP_TRY(hipEventSynchronize(w->copied[k]));
+    const size_t bytes = (size_t)w->n * sizeof(ns_apd_record);
+    if (fwrite(w->h_rec[k], 1, bytes, w->f) != bytes)
+        return set_error(NS_ERR_IO, "ns_apd_writer: short write");
+    w->in_flight[k] = false;
+    w->written++;
+    return NS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int ns_apd_writer_open(ns_gpu_ctx *c, const char *path, int block_size, ns_apd_writer **out)
+{
+    if (int rc = check_ctx(c, "ns_apd_writer_open"))
+        return rc;
+    if (!out || !path || block_size < 1)
+        return set_error(NS_ERR_ARG, "ns_apd_writer_open: bad argument");
+    *out = nullptr;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_apd_writer_open: no state uploaded");
+    ns_apd_writer *w = new (std::nothrow) ns_apd_writer;
+    if (!w)
+        return set_error(NS_ERR_NOMEM, "ns_apd_writer_open");
+    w->ctx = c;
+    w->block_size = block_size;
+    w->n = c->n;
+    w->f = fopen(path, "ab");
+    if (!w->f) {
+        apd_release(w);
+        return set_error(NS_ERR_IO, "ns_apd_writer_open: cannot open %s for appending", path);
+    }
+    fseek(w->f, 0, SEEK_END);
+    w->start = ftell(w->f);
+    DeviceGuard g(c->device);
+    const size_t bytes = (size_t)(w->n > 0 ? w->n : 1) * sizeof(ns_apd_record);
+    hipError_t e = hipStreamCreateWithFlags(&w->copy, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = hipMalloc(&w->d_rec[k], bytes);
+        if (e == hipSuccess)
+            e = hipHostMalloc(&w->h_rec[k], bytes, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&w->made[k], hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&w->copied[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        apd_release(w);
+        return set_error(NS_ERR_NOMEM, "ns_apd_writer_open: %s", hipGetErrorString(e));
+    }
+    *out = w;
+    return NS_OK;
+}
+
+int ns_apd_writer_write(ns_apd_writer *w)
+{
+    if (!w)
+        return set_error(NS_ERR_ARG, "ns_apd_writer_write: null writer");
+    ns_gpu_ctx *c = w->ctx;
+    if (c->n != w->n)
+        return set_error(NS_ERR_STATE, "ns_apd_writer_write: context size changed");
+    DeviceGuard g(c->device);
+    const int k = w->slot;
+    // the slot's previous snapshot must be in the file before its buffers are reused
+    if (int rc = apd_flush_slot(w, k))
+        return rc;
+    if (w->n > 0) {
+        const int s = c->cur;
+        c->prof.begin(ns::K_APD);
+        HIP_TRY(c->f32 ? ns::launch_apd_records(c->rf[s], c->vf[s], c->lf[s], true, c->id[s], c->n, w->d_rec[k],
+                                                c->stream)
+                       : ns::launch_apd_records(c->r[s], c->v[s], c->l[s], false, c->id[s], c->n, w->d_rec[k],
+                                                c->stream));
+        c->prof.end(ns::K_APD);
+        HIP_TRY(hipEventRecord(w->made[k], c->stream));
+        HIP_TRY(hipStreamWaitEvent(w->copy, w->made[k], 0));
+        HIP_TRY(hipMemcpyAsync(w->h_rec[k], w->d_rec[k], (size_t)w->n * sizeof(ns_apd_record), hipMemcpyDeviceToHost,
+                               w->copy));
+        HIP_TRY(hipEventRecord(w->copied[k], w->copy));
+    }
+    w->in_flight[k] = true;
+    w->slot = k ^ 1;
+    // the other slot's snapshot (the previous call's) goes to the file now,
+    // while this one is being made and copied
+    return apd_flush_slot(w, k ^ 1);
+}
+
+int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept)
+{
+    if (!w)
+        return set_error(NS_ERR_ARG, "ns_apd_writer_close: null writer");
+    int rc = NS_OK;
+    {
+        DeviceGuard g(w->ctx->device);
+        // oldest first: the slot not written last
+        const int first = w->slot;
+        rc = apd_flush_slot(w, first);
+        if (rc == NS_OK)
+            rc = apd_flush_slot(w, first ^ 1);
+    }
+    const int64_t whole = (w->written / w->block_size) * w->block_size;
+    if (rc == NS_OK) {
+        // the reference appends only full blocks: drop a partial last block
+        const long long size = w->start + (long long)whole * w->n * (long long)sizeof(ns_apd_record);
+        if (fflush(w->f) != 0 || (whole < w->written && ftruncate(fileno(w->f), (off_t)size) != 0))
+            rc = set_error(NS_ERR_IO, "ns_apd_writer_close: cannot finish the file");
+    }
+    if (kept)
+        *kept = rc == NS_OK ? whole : 0;
+    apd_release(w);
+    return rc;
 }
 
 } // extern "C"

@@ -167,6 +167,7 @@ enum KernelId : int {
     K_INT_DEEP,
     K_MASS_SCAN,
     K_MLP,
+    K_APD,
     K_COUNT
 };
 const char *kernel_name(int id);
