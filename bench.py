@@ -375,6 +375,8 @@ def main() -> int:
                     help="also time the snapshot path: an all_particle_data snapshot every K steps "
                          "(ns_apd_writer, written to --snapshot-path) against the same steps without")
     ap.add_argument("--snapshot-path", default="/dev/null")
+    ap.add_argument("--ic-on-gpu", action="store_true",
+                    help="draw the initial conditions on the device (ns_gpu_generate_ic) instead of the host")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="autotune the chosen form's launch configuration first (STEPS timed steps per trial)")
     ap.add_argument("--budget", type=int, default=1 << 18,
@@ -393,15 +395,20 @@ def main() -> int:
     t_ic = time.perf_counter()
     p = ns.unit_params(dt=args.dt)
     halos = None
+    t_drawn = None
     if kind < 0:
         state, halos = ensemble_ic(ns, args.seed, lowl=lowl)
         n = state[0].size
     else:
         n = args.n or n_default
-        r, v, ell, ids = ns.generate_ic(kind, n, args.seed, rmax, lowl, 1e-3)
         # the reference's exact initial sort (nsphere.c:5339-5342), done once
         with ns.GpuContext(n) as ctx:
-            ctx.upload(r, v, ell, ids)
+            if args.ic_on_gpu:
+                ctx.generate_ic(kind, args.seed, rmax, lowl, 1e-3)
+            else:
+                ctx.upload(*ns.generate_ic(kind, n, args.seed, rmax, lowl, 1e-3))
+            ctx.sync()
+            t_drawn = time.perf_counter() - t_ic
             ctx.sort_particles()
             state = ctx.download()
     print(f"[bench] N={n} initial state in {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
@@ -459,6 +466,8 @@ def main() -> int:
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),
         "levels": chosen["levels"],
         "work_cap": {"force_evals_per_particle_step": args.budget, "capped_particle_steps": chosen["cap_hits"]},
+        "initial_conditions": {"drawn_on": "gpu" if args.ic_on_gpu else "host",
+                               "seconds": round(t_drawn, 4) if t_drawn is not None else None},
     }
     if learned is not None:
         out["config"]["learned_binning"] = True

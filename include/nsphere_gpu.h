@@ -256,6 +256,28 @@ int ns_apd_writer_write(ns_apd_writer *w);
  * (may be NULL) receives the number of snapshots in the file's whole blocks. */
 int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept);
 
+/* ---- initial conditions (nsphere.c:2386-2487, 4908-4920) -----------------
+ *
+ * Initial conditions drawn on the device straight into the context's state:
+ * the recipes and draws of ns_ic_generate (the same per-particle Philox
+ * counters; values can differ from the host's only where the device math
+ * library rounds pow / log / sin / cos differently), ids 0..npts-1. */
+int ns_gpu_generate_ic(ns_gpu_ctx *ctx, int kind, uint64_t seed, double rmax, double lowl_frac,
+                       double lowl_scale);
+/* The reference's initial-conditions file (write_initial_conditions /
+ * read_initial_conditions, nsphere.c:2386-2487): int32 npts, then for each
+ * particle 5 doubles (r, |v|, ell, id, mu) in the units before the
+ * conversion of nsphere.c:4908-4920.  records: npts x 5 doubles. */
+int ns_ic_file_write(const char *path, int32_t npts, const double *records);
+/* records NULL: *npts receives the file's particle count.  Otherwise *npts is
+ * the count expected, and a file holding another count is refused
+ * (NS_ERR_IO), as the reference refuses it. */
+int ns_ic_file_read(const char *path, int32_t *npts, double *records);
+/* Uploads such records as the state, with the reference's conversion after
+ * reading: v_r = (|v| mu) conv, ell = ell conv (conv = kmsec_to_kpcmyr in the
+ * reference's units, 1 for none); ids from the fourth column. */
+int ns_gpu_upload_ic(ns_gpu_ctx *ctx, const double *records, double conv);
+
 /* The enclosed mass of the last scan (npts + 2 doubles): [i] below rank i,
  * [npts] the total, [npts + 1] the rank-0 particle's mass. */
 int ns_gpu_enclosed_mass(ns_gpu_ctx *ctx, double *host_out);

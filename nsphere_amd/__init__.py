@@ -30,7 +30,7 @@ __all__ = [
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
     "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "Mlp", "level_samples", "ApdWriter", "APD_RECORD",
-    "read_all_particle_data", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
+    "read_all_particle_data", "write_ic_file", "read_ic_file", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]
 
 IC_PLUMMER, IC_HERNQUIST, IC_MIX = 0, 1, 2
@@ -224,6 +224,20 @@ class GpuContext:
         check(lib.ns_gpu_autotune(self._h, ctypes.byref(p), int(steps), ctypes.byref(res)))
         return {"best": res.best.as_dict(), "tried": [res.tried[i].as_dict() for i in range(res.ntried)]}
 
+    def generate_ic(self, kind: int, seed: int, rmax: float = 100.0, lowl_frac: float = 0.0,
+                    lowl_scale: float = 1e-3) -> None:
+        """Draw the seeded initial conditions of generate_ic on the device, as the state."""
+        check(lib.ns_gpu_generate_ic(self._h, int(kind), ctypes.c_uint64(seed), float(rmax), float(lowl_frac),
+                                     float(lowl_scale)))
+
+    def upload_ic(self, records, conv: float = 1.0) -> None:
+        """Upload initial-conditions records (npts x 5: r, |v|, ell, id, mu) with
+        the reference's conversion v_r = |v| mu conv, ell = ell conv."""
+        rec = np.ascontiguousarray(records, np.float64)
+        if rec.shape != (self.npts, 5):
+            raise ValueError(f"records must be ({self.npts}, 5)")
+        check(lib.ns_gpu_upload_ic(self._h, rec.ctypes.data, float(conv)))
+
     def snapshot_records(self) -> np.ndarray:
         """all_particle_data records of the current state in id order
         (rank, float32 r, v_r, ell), as nsphere.c:6456-6471 fills them."""
@@ -295,6 +309,23 @@ class ApdWriter:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def write_ic_file(path: str, records) -> None:
+    """The reference's initial-conditions file (nsphere.c:2386-2416)."""
+    rec = np.ascontiguousarray(records, np.float64)
+    if rec.ndim != 2 or rec.shape[1] != 5:
+        raise ValueError("records must be (npts, 5)")
+    check(lib.ns_ic_file_write(str(path).encode(), rec.shape[0], rec.ctypes.data))
+
+
+def read_ic_file(path: str) -> np.ndarray:
+    """Records (npts x 5) of an initial-conditions file (nsphere.c:2445-2487)."""
+    n = ctypes.c_int32()
+    check(lib.ns_ic_file_read(str(path).encode(), ctypes.byref(n), None))
+    rec = np.empty((n.value, 5), np.float64)
+    check(lib.ns_ic_file_read(str(path).encode(), ctypes.byref(n), rec.ctypes.data))
+    return rec
 
 
 def read_all_particle_data(path: str, npts: int, snap: int) -> np.ndarray:

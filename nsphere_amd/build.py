@@ -66,7 +66,7 @@ def build(verbose: bool = False) -> str:
             objs.append(obj)
         for src in HOST_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", *inc, "-c",
+            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unknown-pragmas", *inc, "-c",
                   os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)
         tmp_lib = LIB + ".tmp"

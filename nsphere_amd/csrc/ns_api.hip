@@ -720,6 +720,60 @@ int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const dou
     return NS_OK;
 }
 
+// The state just written into the float64 buffers of index 0: convert for
+// the float32 form and mark it uploaded, as ns_gpu_upload_soa does.
+static int state_written(ns_gpu_ctx *c)
+{
+    c->cur = 0;
+    if (c->n > 0 && c->f32)
+        if (int rc = state_to_f32(c, 0))
+            return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->keys_valid = false;
+    c->levels_valid = false;
+    return NS_OK;
+}
+
+int ns_gpu_generate_ic(ns_gpu_ctx *c, int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale)
+{
+    if (int rc = check_ctx(c, "ns_gpu_generate_ic"))
+        return rc;
+    if (kind < 0 || kind > 2 || !(rmax > 0.0) || c->n > (int64_t)UINT32_MAX)
+        return set_error(NS_ERR_ARG, "ns_gpu_generate_ic: bad kind/rmax");
+    DeviceGuard g(c->device);
+    HIP_TRY(ns::launch_generate_ic(kind, seed, rmax, lowl_frac, lowl_scale, c->n, c->r[0], c->v[0], c->l[0], c->id[0],
+                                   c->stream));
+    return state_written(c);
+}
+
+int ns_gpu_upload_ic(ns_gpu_ctx *c, const double *records, double conv)
+{
+    if (int rc = check_ctx(c, "ns_gpu_upload_ic"))
+        return rc;
+    if (c->n > 0 && !records)
+        return set_error(NS_ERR_ARG, "ns_gpu_upload_ic: null records");
+    for (int64_t i = 0; i < c->n; i++) {
+        const double x = records[5 * i + 3];
+        if (!(x >= 0.0 && x < 4294967296.0))
+            return set_error(NS_ERR_ARG, "ns_gpu_upload_ic: id %g at %lld is not a uint32", x, (long long)i);
+    }
+    if (c->n == 0)
+        return state_written(c);
+    DeviceGuard g(c->device);
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, (size_t)c->n * 5 * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, records, (size_t)c->n * 5 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = ns::launch_ic_records(d, c->n, conv, c->r[0], c->v[0], c->l[0], c->id[0], c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_upload_ic: %s", hipGetErrorString(e));
+    return state_written(c);
+}
+
 int ns_gpu_upload(ns_gpu_ctx *c, double *const particles[5])
 {
     if (int rc = check_ctx(c, "ns_gpu_upload"))

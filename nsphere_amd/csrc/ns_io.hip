@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
+#include "ns_ic_sampler.h"
 #include "ns_kernels.h"
 
 namespace ns {
@@ -59,7 +60,62 @@ __global__ void k_gather_ids(const S *__restrict__ r, const S *__restrict__ v, c
     out[4 * j + 3] = ok ? (double)p : -1.0;
 }
 
+// Seeded initial conditions drawn on the device (ns_ic_sampler.h, the same
+// draws as the host generator).
+__global__ __launch_bounds__(kIoThreads) void k_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac,
+                                                            double lowl_scale, int64_t n, double *__restrict__ r,
+                                                            double *__restrict__ v, double *__restrict__ l,
+                                                            uint32_t *__restrict__ id)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    ic::sample_particle(kind, seed, (uint64_t)i, rmax, lowl_frac, lowl_scale, r[i], v[i], l[i]);
+    id[i] = (uint32_t)i;
+}
+
+// An initial-conditions file's records (r, |v|, ell, id, mu per particle,
+// read_initial_conditions, nsphere.c:2445-2487) into the state, with the
+// reference's conversion after reading (nsphere.c:4908-4920):
+// v_r = (|v| mu) kmsec_to_kpcmyr, ell = ell kmsec_to_kpcmyr, in that order.
+__global__ __launch_bounds__(kIoThreads) void k_ic_records(const double *__restrict__ rec, int64_t n, double conv,
+                                                           double *__restrict__ r, double *__restrict__ v,
+                                                           double *__restrict__ l, uint32_t *__restrict__ id)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double *p = rec + 5 * i;
+    r[i] = p[0];
+    double vr = p[1];
+    vr *= p[4];
+    vr *= conv;
+    v[i] = vr;
+    l[i] = p[2] * conv;
+    id[i] = (uint32_t)p[3];
+}
+
 } // namespace
+
+hipError_t launch_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale, int64_t n,
+                              double *r, double *v, double *l, uint32_t *id, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_generate_ic, dim3((unsigned)((n + kIoThreads - 1) / kIoThreads)), dim3(kIoThreads), 0,
+                       stream, kind, seed, rmax, lowl_frac, lowl_scale, n, r, v, l, id);
+    return hipGetLastError();
+}
+
+hipError_t launch_ic_records(const double *rec, int64_t n, double conv, double *r, double *v, double *l,
+                             uint32_t *id, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ic_records, dim3((unsigned)((n + kIoThreads - 1) / kIoThreads)), dim3(kIoThreads), 0,
+                       stream, rec, n, conv, r, v, l, id);
+    return hipGetLastError();
+}
 
 hipError_t launch_apd_records(const void *r, const void *v, const void *l, bool f32, const uint32_t *id, int64_t n,
                               void *rec, hipStream_t stream)

@@ -104,6 +104,12 @@ hipError_t launch_apd_records(const void *r, const void *v, const void *l, bool 
                               void *rec, hipStream_t stream);
 hipError_t launch_gather_ids(const void *r, const void *v, const void *l, bool f32, const int32_t *inv, int64_t n,
                              const uint32_t *ids, int k, double *out, hipStream_t stream);
+// Initial conditions (ns_io.hip): drawn on the device, or converted from an
+// initial-conditions file's records (5 doubles per particle).
+hipError_t launch_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale, int64_t n,
+                              double *r, double *v, double *l, uint32_t *id, hipStream_t stream);
+hipError_t launch_ic_records(const double *rec, int64_t n, double conv, double *r, double *v, double *l,
+                             uint32_t *id, hipStream_t stream);
 int energy_blocks(int64_t n);
 // mass (rank order via perm, may be null: equal masses m) and menc (may be
 // null: M_enc from the rank) select the per-particle-mass energy; seg_off /
