@@ -312,6 +312,28 @@ def measure_snapshots(ns, precision: str, state, p, every: int, steps: int, work
     return out
 
 
+def measure_potential(ns, precision: str, state, reps: int = 20) -> dict:
+    """Dynamic potential (ns_gpu_potential) of the benchmark state: the three
+    scan kernels timed together with HIP events; algorithmic bytes = two reads
+    of r and one write of psi (float64)."""
+    n = state[0].size
+    p = ns.default_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
+        ctx.upload(*state)
+        ctx.potential(p)
+        ctx.set_option(ns.NS_OPT_PROFILE, 1)
+        ctx.reset_kernel_stats()
+        for _ in range(reps):
+            ctx.potential(p)
+        launches, ms = ctx.kernel_stats().get("potential", (0, 0.0))
+    avg = ms / max(launches, 1) * 1e-3
+    bpp = 2 * (4 if precision == "f32" else 8) + 8
+    return {"calls": launches, "avg_us": round(avg * 1e6, 2), "bytes_per_particle": bpp,
+            "GBps": round(bpp * n / avg / 1e9, 1) if avg > 0 else None,
+            "note": "includes the copy of psi to the host only outside the timed kernels"}
+
+
 def traffic_for(run: dict, n: int, kernel: str):
     return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
 
@@ -375,6 +397,8 @@ def main() -> int:
                     help="also time the snapshot path: an all_particle_data snapshot every K steps "
                          "(ns_apd_writer, written to --snapshot-path) against the same steps without")
     ap.add_argument("--snapshot-path", default="/dev/null")
+    ap.add_argument("--potential", action="store_true",
+                    help="also time the dynamic potential (ns_gpu_potential) on the benchmark state")
     ap.add_argument("--ic-on-gpu", action="store_true",
                     help="draw the initial conditions on the device (ns_gpu_generate_ic) instead of the host")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
@@ -472,6 +496,8 @@ def main() -> int:
     if learned is not None:
         out["config"]["learned_binning"] = True
         out["learned_binning"] = learned
+    if args.potential:
+        out["potential"] = measure_potential(ns, args.precision, state)
     if args.snapshot_every:
         print("[bench] timing the snapshot path", file=sys.stderr, flush=True)
         out["snapshot"] = measure_snapshots(ns, args.precision, state, p, args.snapshot_every,

@@ -256,6 +256,16 @@ int ns_apd_writer_write(ns_apd_writer *w);
  * (may be NULL) receives the number of snapshots in the file's whole blocks. */
 int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept);
 
+/* Dynamic potential, method A (nsphere.c:7698-7738), over the current radii
+ * in rank order (as after ns_gpu_sort_particles or a step's sort):
+ *   psi[i] = G ((i + 1) dM / r_i + dM sum_{j > i} 1 / r_j) vel_conv_sq,
+ *   psi0   = G dM (sum_j 1 / r_j) vel_conv_sq   (the value at r = 0),
+ * with dM = halo_mass / npts and G, vel_conv_sq, halo_mass from p.  The sum
+ * is a device suffix scan in a fixed order (the same result on every run);
+ * it differs from the reference's sequential sum only in rounding.
+ * psi: npts doubles (may be NULL); psi0 may be NULL. */
+int ns_gpu_potential(ns_gpu_ctx *ctx, const ns_step_params *p, double *psi, double *psi0);
+
 /* ---- initial conditions (nsphere.c:2386-2487, 4908-4920) -----------------
  *
  * Initial conditions drawn on the device straight into the context's state:

@@ -238,6 +238,14 @@ class GpuContext:
             raise ValueError(f"records must be ({self.npts}, 5)")
         check(lib.ns_gpu_upload_ic(self._h, rec.ctypes.data, float(conv)))
 
+    def potential(self, params: StepParams):
+        """Dynamic potential, method A (nsphere.c:7698-7738), over the current
+        radii in the current order taken as rank order: (psi per rank, psi(0))."""
+        psi = np.empty(self.npts, np.float64)
+        psi0 = ctypes.c_double()
+        check(lib.ns_gpu_potential(self._h, ctypes.byref(params), psi.ctypes.data, ctypes.byref(psi0)))
+        return psi, psi0.value
+
     def snapshot_records(self) -> np.ndarray:
         """all_particle_data records of the current state in id order
         (rank, float32 r, v_r, ell), as nsphere.c:6456-6471 fills them."""

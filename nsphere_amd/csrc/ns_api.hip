@@ -100,7 +100,7 @@ const char *kernel_name(int id)
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
                                          "integrate_deep", "mass_scan", "mlp_binning",
-                                         "snapshot_records"};
+                                         "snapshot_records", "potential"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -917,6 +917,50 @@ int ns_gpu_snapshot_records(ns_gpu_ctx *c, ns_apd_record *host_out)
     (void)hipFree(d);
     if (e != hipSuccess)
         return set_error(NS_ERR_HIP, "ns_gpu_snapshot_records: %s", hipGetErrorString(e));
+    return NS_OK;
+}
+
+int ns_gpu_potential(ns_gpu_ctx *c, const ns_step_params *p, double *psi, double *psi0)
+{
+    if (int rc = check_ctx(c, "ns_gpu_potential"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_potential"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_potential: no state uploaded");
+    if (c->n == 0) {
+        if (psi0)
+            *psi0 = 0.0;
+        return NS_OK;
+    }
+    DeviceGuard g(c->device);
+    const double dm = p->halo_mass / (double)c->n;  // deltaM, nsphere.c:5179
+    double *d_psi = nullptr, *work = nullptr;
+    const int64_t nw = ns::potential_work_doubles(c->n);
+    HIP_TRY(hipMalloc((void **)&work, (size_t)nw * sizeof(double)));
+    hipError_t e = psi ? hipMalloc((void **)&d_psi, (size_t)c->n * sizeof(double)) : hipSuccess;
+    double total = 0.0;
+    if (e == hipSuccess) {
+        const int s = c->cur;
+        c->prof.begin(ns::K_POTENTIAL);
+        // without psi only the tile and total passes run
+        e = ns::launch_potential(c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s], c->f32, c->n, dm, p->G,
+                                 p->vel_conv_sq, d_psi, work, c->stream);
+        c->prof.end(ns::K_POTENTIAL);
+    }
+    if (e == hipSuccess && psi)
+        e = hipMemcpyAsync(psi, d_psi, (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&total, work + nw - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (d_psi)
+        (void)hipFree(d_psi);
+    (void)hipFree(work);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_potential: %s", hipGetErrorString(e));
+    if (psi0)
+        *psi0 = p->G * dm * total * p->vel_conv_sq;  // PsiA(0), nsphere.c:7733-7738
     return NS_OK;
 }
 

@@ -168,6 +168,7 @@ enum KernelId : int {
     K_MASS_SCAN,
     K_MLP,
     K_APD,
+    K_POTENTIAL,
     K_COUNT
 };
 const char *kernel_name(int id);

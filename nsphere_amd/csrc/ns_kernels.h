@@ -104,6 +104,11 @@ hipError_t launch_apd_records(const void *r, const void *v, const void *l, bool 
                               void *rec, hipStream_t stream);
 hipError_t launch_gather_ids(const void *r, const void *v, const void *l, bool f32, const int32_t *inv, int64_t n,
                              const uint32_t *ids, int k, double *out, hipStream_t stream);
+// Dynamic potential (ns_scan.hip): psi per rank from the suffix sum of 1/r;
+// work holds potential_work_doubles(n) doubles, the last the sum of all 1/r.
+hipError_t launch_potential(const void *r, bool f32, int64_t n, double dm, double g, double vcs, double *psi,
+                            double *work, hipStream_t stream);
+int64_t potential_work_doubles(int64_t n);
 // Initial conditions (ns_io.hip): drawn on the device, or converted from an
 // initial-conditions file's records (5 doubles per particle).
 hipError_t launch_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale, int64_t n,

@@ -253,6 +253,136 @@ __global__ __launch_bounds__(1024) void k_scan_begin(uint32_t *ctl, uint64_t *wo
     }
 }
 
+// ---- dynamic potential (PsiA, nsphere.c:7698-7738) -------------------------
+//
+// psi_i = G ((i + 1) dM / r_i + dM sum_{j > i} 1 / r_j) vel_conv_sq over the
+// radii in rank order, psi0 = G dM sum_j (1 / r_j) vel_conv_sq: a suffix sum
+// of 1/r, done as reduce-then-scan over tiles of kScanTile in a fixed order,
+// so that the result is the same on every run.
+
+// Exclusive suffix over a block's threads (the sum of x over the threads
+// above this one) and the block total, by wave shuffles and the wave totals,
+// in a fixed order.  s_w: WAVES doubles.
+template <int WAVES>
+__device__ inline double block_suffix_exclusive(double x, double *s_w, double &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double inc = x;  // sum over lanes >= lane
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_down(inc, off, 64);
+        if (lane + off < 64)
+            inc += y;
+    }
+    const double up = __shfl_down(inc, 1, 64);
+    const double excl_wave = lane < 63 ? up : 0.0;
+    if (lane == 0)
+        s_w[wave] = inc;
+    __syncthreads();
+    double after = 0.0;
+    total = 0.0;
+    for (int w = WAVES - 1; w >= 0; w--) {
+        const double t = s_w[w];
+        if (w > wave)
+            after += t;
+        total += t;
+    }
+    __syncthreads();  // s_w may be reused
+    return after + excl_wave;
+}
+
+// The tile's sum of 1/r: coalesced reads (thread t takes items t, t + 256,
+// ...), then the block's fixed-order total.
+template <typename S>
+__global__ __launch_bounds__(kScanThreads) void k_psi_tiles(const S *__restrict__ r, int64_t n,
+                                                            double *__restrict__ tile_sum)
+{
+    __shared__ double s_w[kScanWaves];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = base + (int64_t)k * kScanThreads + tid;
+        if (i < n)
+            acc += 1.0 / (double)r[i];
+    }
+    double total;
+    (void)block_suffix_exclusive<kScanWaves>(acc, s_w, total);
+    if (tid == 0)
+        tile_sum[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive suffix over the tile sums (in place) and the total.
+__global__ __launch_bounds__(1024) void k_psi_top(double *__restrict__ tile_sum, int64_t tiles,
+                                                  double *__restrict__ total)
+{
+    __shared__ double s_w[16];
+    const int tid = threadIdx.x;
+    const int64_t per = (tiles + 1023) / 1024;
+    const int64_t lo = (int64_t)tid * per, hi = lo + per < tiles ? lo + per : tiles;
+    double acc = 0.0;
+    for (int64_t t = hi - 1; t >= lo; t--)
+        acc += tile_sum[t];
+    double all;
+    double run = block_suffix_exclusive<16>(acc, s_w, all);
+    if (tid == 0)
+        *total = all;
+    for (int64_t t = hi - 1; t >= lo; t--) {
+        const double c = tile_sum[t];
+        tile_sum[t] = run;
+        run += c;
+    }
+}
+
+// Each tile again: its threads' partials suffix-scanned across the block,
+// then each thread walks its items from the last, so item i sees
+// E_i = sum_{j > i} 1 / r_j; psi_i in the reference's operation order.
+template <typename S>
+__global__ __launch_bounds__(kScanThreads) void k_psi_apply(const S *__restrict__ r, int64_t n,
+                                                            const double *__restrict__ tile_after, double dm,
+                                                            double g, double vcs, double *__restrict__ psi)
+{
+    __shared__ double s_w[kScanWaves];
+    // radii, then psi, of the tile; one pad slot per 16 so that the blocked
+    // access (stride kScanItems) does not fall on one bank
+    __shared__ double s_r[kScanTile + kScanTile / 16];
+    auto at = [](int j) { return j + (j >> 4); };
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    // coalesced in, then each thread takes kScanItems consecutive items
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = base + (int64_t)k * kScanThreads + tid;
+        s_r[at(k * kScanThreads + tid)] = i < n ? (double)r[i] : 1.0;
+    }
+    __syncthreads();
+    const int own = tid * kScanItems;
+    double inv[kScanItems];
+    double acc = 0.0;
+    for (int k = kScanItems - 1; k >= 0; k--) {
+        inv[k] = base + own + k < n ? 1.0 / s_r[at(own + k)] : 0.0;
+        acc += inv[k];
+    }
+    double total;
+    double after = tile_after[blockIdx.x] + block_suffix_exclusive<kScanWaves>(acc, s_w, total);
+    for (int k = kScanItems - 1; k >= 0; k--) {
+        const int64_t i = base + own + k;
+        const double M_r = (double)(i + 1) * dm;
+        const double outer = dm * after;  // 0 for the last rank
+        double v = g * (M_r / s_r[at(own + k)] + outer);
+        v *= vcs;
+        s_r[at(own + k)] = v;  // only this thread reads or writes its own slots here
+        after += inv[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = base + (int64_t)k * kScanThreads + tid;
+        if (i < n)
+            psi[i] = s_r[at(k * kScanThreads + tid)];
+    }
+}
+
 } // namespace
 
 int64_t mass_scan_words(int64_t n) { return 2 * ((n + kScanTile - 1) / kScanTile) + 2; }
@@ -275,6 +405,33 @@ hipError_t launch_mass_scan(MassScan &w, const M *mass_in, const uint32_t *perm,
 
 template hipError_t launch_mass_scan<double>(MassScan &, const double *, const uint32_t *, double *, double *,
                                              int64_t, hipStream_t);
+hipError_t launch_potential(const void *r, bool f32, int64_t n, double dm, double g, double vcs, double *psi,
+                            double *work, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    double *tile_sum = work, *total = work + tiles;
+    if (f32)
+        hipLaunchKernelGGL(k_psi_tiles<float>, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream,
+                           static_cast<const float *>(r), n, tile_sum);
+    else
+        hipLaunchKernelGGL(k_psi_tiles<double>, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream,
+                           static_cast<const double *>(r), n, tile_sum);
+    hipLaunchKernelGGL(k_psi_top, dim3(1), dim3(1024), 0, stream, tile_sum, tiles, total);
+    if (!psi)  // the total alone
+        return hipGetLastError();
+    if (f32)
+        hipLaunchKernelGGL(k_psi_apply<float>, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream,
+                           static_cast<const float *>(r), n, tile_sum, dm, g, vcs, psi);
+    else
+        hipLaunchKernelGGL(k_psi_apply<double>, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream,
+                           static_cast<const double *>(r), n, tile_sum, dm, g, vcs, psi);
+    return hipGetLastError();
+}
+
+int64_t potential_work_doubles(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
 template hipError_t launch_mass_scan<float>(MassScan &, const float *, const uint32_t *, float *, double *, int64_t,
                                             hipStream_t);
 
