@@ -334,6 +334,27 @@ def measure_potential(ns, precision: str, state, reps: int = 20) -> dict:
             "note": "includes the copy of psi to the host only outside the timed kernels"}
 
 
+def measure_density(ns, precision: str, state, reps: int = 3) -> dict:
+    """The snapshot density estimate (ns_gpu_density) of the benchmark state,
+    timed end to end with HIP events (device kernels plus the two host
+    tridiagonal solves between them)."""
+    n = state[0].size
+    p = ns.default_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
+        ctx.upload(*state)
+        _, info = ctx.density(p)
+        ctx.set_option(ns.NS_OPT_PROFILE, 1)
+        ctx.reset_kernel_stats()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.density(p)
+        wall = (time.perf_counter() - t0) / reps
+        launches, ms = ctx.kernel_stats().get("density", (0, 0.0))
+    return {"calls": launches, "avg_ms": round(ms / max(launches, 1), 3), "wall_ms_per_call": round(wall * 1e3, 3),
+            "info": info}
+
+
 def traffic_for(run: dict, n: int, kernel: str):
     return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
 
@@ -397,6 +418,8 @@ def main() -> int:
                     help="also time the snapshot path: an all_particle_data snapshot every K steps "
                          "(ns_apd_writer, written to --snapshot-path) against the same steps without")
     ap.add_argument("--snapshot-path", default="/dev/null")
+    ap.add_argument("--density", action="store_true",
+                    help="also time the snapshot density estimate (ns_gpu_density) on the benchmark state")
     ap.add_argument("--potential", action="store_true",
                     help="also time the dynamic potential (ns_gpu_potential) on the benchmark state")
     ap.add_argument("--ic-on-gpu", action="store_true",
@@ -498,6 +521,8 @@ def main() -> int:
         out["learned_binning"] = learned
     if args.potential:
         out["potential"] = measure_potential(ns, args.precision, state)
+    if args.density:
+        out["density"] = measure_density(ns, args.precision, state)
     if args.snapshot_every:
         print("[bench] timing the snapshot path", file=sys.stderr, flush=True)
         out["snapshot"] = measure_snapshots(ns, args.precision, state, p, args.snapshot_every,

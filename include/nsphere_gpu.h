@@ -266,6 +266,30 @@ int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept);
  * psi: npts doubles (may be NULL); psi0 may be NULL. */
 int ns_gpu_potential(ns_gpu_ctx *ctx, const ns_step_params *p, double *psi, double *psi0);
 
+/* Density estimate of the current radii in rank order (as after
+ * ns_gpu_sort_particles), the reference's snapshot density
+ * (nsphere.c:7236-7686 with fft_gaussian_convolution, 2901-3060): enclosed
+ * mass (i + 1) dM (dM = halo_mass / npts) over the strictly increasing radii,
+ * decimated to min(ceil(npts / 100), 30000) points, a natural cubic spline
+ * of it on 131072 points uniform in log10 r, dM/dr by central differences
+ * (floored at 1e-10), smoothed by the reference's one-sided Gaussian of width
+ * 0.08 (npts / 1e4)^-0.4 in log10 r as a zero-padded linear convolution, and
+ * a natural cubic spline of the result evaluated at every radius.  The
+ * convolution is summed directly over the taps where the Gaussian exceeds
+ * 1e-300 of its peak: the reference's FFT gives the same values up to its
+ * round-off, about 1e-16 of the largest density.  density: npts doubles.
+ * NS_ERR_ARG where the reference skips the snapshot (fewer than three
+ * decimated radii, or a degenerate range). */
+typedef struct ns_density_info {
+    int64_t kept;       /* strictly increasing radii                     */
+    int32_t decimated;  /* spline nodes of M(r)                          */
+    int32_t grid;       /* log-spaced grid points (131072)               */
+    int32_t taps;       /* convolution taps summed                       */
+    double sigma;       /* smoothing width in log10 r                    */
+    double r_min, r_max;/* grid range                                    */
+} ns_density_info;
+int ns_gpu_density(ns_gpu_ctx *ctx, const ns_step_params *p, double *density, ns_density_info *info);
+
 /* ---- initial conditions (nsphere.c:2386-2487, 4908-4920) -----------------
  *
  * Initial conditions drawn on the device straight into the context's state:

@@ -246,6 +246,16 @@ class GpuContext:
         check(lib.ns_gpu_potential(self._h, ctypes.byref(params), psi.ctypes.data, ctypes.byref(psi0)))
         return psi, psi0.value
 
+    def density(self, params: StepParams):
+        """The reference's snapshot density estimate (nsphere.c:7236-7686) at
+        the current radii in the current order taken as rank order: (density
+        per rank, info dict)."""
+        from ._lib import DensityInfo
+        out = np.empty(self.npts, np.float64)
+        info = DensityInfo()
+        check(lib.ns_gpu_density(self._h, ctypes.byref(params), out.ctypes.data, ctypes.byref(info)))
+        return out, {k: getattr(info, k) for k, _ in DensityInfo._fields_}
+
     def snapshot_records(self) -> np.ndarray:
         """all_particle_data records of the current state in id order
         (rank, float32 r, v_r, ell), as nsphere.c:6456-6471 fills them."""

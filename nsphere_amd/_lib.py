@@ -60,6 +60,12 @@ class StepParams(ctypes.Structure):
         return p
 
 
+class DensityInfo(ctypes.Structure):
+    _fields_ = [("kept", ctypes.c_int64), ("decimated", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("taps", ctypes.c_int32), ("sigma", ctypes.c_double), ("r_min", ctypes.c_double),
+                ("r_max", ctypes.c_double)]
+
+
 class TuneConfig(ctypes.Structure):
     """Mirror of ns_tune_config."""
 
@@ -124,6 +130,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_set_mlp": (i32, [vp, vp]),
         "ns_gpu_snapshot_records": (i32, [vp, vp]),
         "ns_gpu_potential": (i32, [vp, c.POINTER(StepParams), vp, c.POINTER(c.c_double)]),
+        "ns_gpu_density": (i32, [vp, c.POINTER(StepParams), vp, c.POINTER(DensityInfo)]),
         "ns_gpu_generate_ic": (i32, [vp, i32, c.c_uint64, c.c_double, c.c_double, c.c_double]),
         "ns_gpu_upload_ic": (i32, [vp, vp, c.c_double]),
         "ns_ic_file_write": (i32, [c.c_char_p, i32, vp]),

@@ -100,7 +100,7 @@ const char *kernel_name(int id)
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
                                          "integrate_deep", "mass_scan", "mlp_binning",
-                                         "snapshot_records", "potential"};
+                                         "snapshot_records", "potential", "density"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -961,6 +961,39 @@ int ns_gpu_potential(ns_gpu_ctx *c, const ns_step_params *p, double *psi, double
         return set_error(NS_ERR_HIP, "ns_gpu_potential: %s", hipGetErrorString(e));
     if (psi0)
         *psi0 = p->G * dm * total * p->vel_conv_sq;  // PsiA(0), nsphere.c:7733-7738
+    return NS_OK;
+}
+
+int ns_gpu_density(ns_gpu_ctx *c, const ns_step_params *p, double *density, ns_density_info *info)
+{
+    if (int rc = check_ctx(c, "ns_gpu_density"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_density"))
+        return rc;
+    if (!density && c->n > 0)
+        return set_error(NS_ERR_ARG, "ns_gpu_density: null output");
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_density: no state uploaded");
+    if (c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, (size_t)c->n * sizeof(double)));
+    const int s = c->cur;
+    c->prof.begin(ns::K_DENSITY);
+    hipError_t e = ns::density_estimate(c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s], c->f32, c->n,
+                                        p->halo_mass, d, info, c->stream);
+    c->prof.end(ns::K_DENSITY);
+    if (e == hipErrorInvalidValue) {
+        (void)hipFree(d);
+        return set_error(NS_ERR_ARG, "ns_gpu_density: fewer than three distinct decimated radii or a degenerate "
+                                     "range (the reference skips such a snapshot)");
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(density, d, (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_density: %s", hipGetErrorString(e));
     return NS_OK;
 }
 

@@ -169,6 +169,7 @@ enum KernelId : int {
     K_MLP,
     K_APD,
     K_POTENTIAL,
+    K_DENSITY,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "nsphere_gpu.h"
+
 namespace ns {
 
 // Everything the fused step kernel needs, passed by value.  State arrays are
@@ -109,6 +111,10 @@ hipError_t launch_gather_ids(const void *r, const void *v, const void *l, bool f
 hipError_t launch_potential(const void *r, bool f32, int64_t n, double dm, double g, double vcs, double *psi,
                             double *work, hipStream_t stream);
 int64_t potential_work_doubles(int64_t n);
+// Density estimate (ns_density.hip) of radii in rank order: out[n] doubles.
+hipError_t density_estimate(const void *r_dev, bool f32, int64_t n, double halo_mass, double *out,
+                            ns_density_info *info, hipStream_t stream);
+int64_t density_grid_size();
 // Initial conditions (ns_io.hip): drawn on the device, or converted from an
 // initial-conditions file's records (5 doubles per particle).
 hipError_t launch_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale, int64_t n,
