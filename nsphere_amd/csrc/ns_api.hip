@@ -48,6 +48,7 @@ struct ns_gpu_ctx {
     // learned binning: the network and the rank order by predicted level
     const ns_mlp *mlp = nullptr;
     uint32_t *order = nullptr;
+    uint32_t *id_order = nullptr;  // NS_OPT_TIEBREAK=1: the id order of the sort's first stage
     // ensemble of halos (nseg > 0): halo h holds ranks [seg_off[h], seg_off[h+1])
     int nseg = 0;
     int64_t *seg_off = nullptr;
@@ -245,7 +246,7 @@ void free_all(ns_gpu_ctx *c)
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
-                    c->order};
+                    c->order, c->id_order};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -416,6 +417,16 @@ bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
 // Sort r[cur] into c->perm (keys from the integrate epilogue when valid).
 int sort_perm(ns_gpu_ctx *c)
 {
+    if (c->tiebreak) {  // (radius, id): ids first, radius keys gathered through their order
+        const int s = c->cur;
+        c->prof.begin(ns::K_TIEFIX);
+        const hipError_t e = ns::radix_sort_perm_ids(c->sw, c->n, c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s],
+                                                     c->f32, c->id[s], c->id_order, c->perm, c->stream, nullptr);
+        c->prof.end(ns::K_TIEFIX);
+        HIP_TRY(e);
+        c->keys_valid = false;
+        return NS_OK;
+    }
     if (!c->keys_valid) {
         c->prof.begin(ns::K_KEYS);
         if (c->f32)
@@ -623,8 +634,14 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
     case NS_OPT_TIEBREAK:
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_TIEBREAK takes 0 or 1");
-        if (value == 1)
-            return set_error(NS_ERR_ARG, "NS_OPT_TIEBREAK=1 (id tiebreak) is not available in this build");
+        if (value == 1 && !c->id_order) {
+            DeviceGuard g(c->device);
+            hipError_t e = dalloc(&c->id_order, c->n);
+            if (e != hipSuccess)
+                return set_error(NS_ERR_NOMEM, "NS_OPT_TIEBREAK: %s", hipGetErrorString(e));
+        }
+        if (c->tiebreak != (int)value)
+            release_graph(c);
         c->tiebreak = (int)value;
         return NS_OK;
     case NS_OPT_PROFILE:

@@ -147,6 +147,11 @@ struct Profiler;
 // One stable pass on the low byte of 32-bit keys in keys[0], ignoring any
 // segments (reduce-then-scan only): the order of a small integer label.
 hipError_t radix_sort_perm_u8(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream);
+// Stable sort on (radius key, id): the ids first, then the radius keys of r
+// (float32 or float64) gathered through the id order; order: n scratch
+// entries (no alias of the sort's buffers or of perm_out).
+hipError_t radix_sort_perm_ids(SortWork &w, int64_t n, const void *r, bool f32, const uint32_t *id,
+                               uint32_t *order, uint32_t *perm_out, hipStream_t stream, Profiler *prof);
 hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_out, hipStream_t stream,
                            Profiler *prof);
 

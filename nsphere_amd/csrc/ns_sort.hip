@@ -111,6 +111,28 @@ __global__ void k_radius_keys(const S *__restrict__ r, typename KeyOf<S>::type *
         keys[i] = radius_key(r[i]);
 }
 
+// Id tiebreak: the ids as 32-bit keys, then the radius keys gathered through
+// the id order (with that order as the first pass's payload).
+__global__ void k_id_keys(const uint32_t *__restrict__ id, uint32_t *__restrict__ keys, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        keys[i] = id[i];
+}
+
+template <typename S>
+__global__ void k_radius_keys_gather(const S *__restrict__ r, const uint32_t *__restrict__ order,
+                                     typename KeyOf<S>::type *__restrict__ keys, uint32_t *__restrict__ vals,
+                                     int64_t n)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        const uint32_t p = order[j];
+        keys[j] = radius_key(r[p]);
+        vals[j] = p;
+    }
+}
+
 __global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -619,8 +641,15 @@ hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int 
 
 template <typename KeyT, int THREADS, int ITEMS>
 hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream, Profiler *prof)
+                           hipStream_t stream, Profiler *prof, bool have_vals)
 {
+    // have_vals: the first pass reads its payload from vals[src] instead of
+    // generating the input positions
+    if (have_vals && pass == 0) {
+        if (passes == 1)
+            return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream, prof);
+        return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream, prof);
+    }
     if (passes == 1)
         return launch_pass<KeyT, THREADS, ITEMS, true, true>(w, src, vals_out, n, pass, stream, prof);
     if (pass == 0)
@@ -632,16 +661,16 @@ hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, 
 
 template <typename KeyT>
 hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream, Profiler *prof)
+                           hipStream_t stream, Profiler *prof, bool have_vals)
 {
     const SortConfig c = w.config;
     if (c.threads == 512 && c.items == 16)
-        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream, prof);
+        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
     if (c.threads == 256 && c.items == 24)
-        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof);
+        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
     if (c.threads == 512 && c.items == 12)
-        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof);
-    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof);
+        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
 }
 
 // (Re)builds the device tile table of the segments for tile size `tile`:
@@ -687,7 +716,7 @@ hipError_t seg_table_build(SegTable &t, int tile, int64_t n)
 
 template <typename KeyT>
 hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof,
-                      int kPasses = (int)sizeof(KeyT), bool use_segments = true)
+                      int kPasses = (int)sizeof(KeyT), bool use_segments = true, bool have_vals = false)
 {
     hipError_t e;
     // the segment table is set aside for an unsegmented sort of this call
@@ -748,7 +777,7 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
     int src = 0;
     for (int p = 0; p < kPasses; p++) {
         uint32_t *vout = (p == kPasses - 1) ? perm_out : w.vals[src ^ 1];
-        if ((e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream, prof)) != hipSuccess)
+        if ((e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream, prof, have_vals)) != hipSuccess)
             return e;
         src ^= 1;
     }
@@ -801,6 +830,34 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
     if (key_bits == 32)
         return sort_typed<uint32_t>(w, n, perm_out, stream, prof);
     return sort_typed<uint64_t>(w, n, perm_out, stream, prof);
+}
+
+hipError_t radix_sort_perm_ids(SortWork &w, int64_t n, const void *r, bool f32, const uint32_t *id,
+                               uint32_t *order, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipError_t e;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    // 1. the ids, stably (a permutation within each segment)
+    hipLaunchKernelGGL(k_id_keys, grid, dim3(256), 0, stream, id, reinterpret_cast<uint32_t *>(w.keys[0]), n);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if ((e = sort_typed<uint32_t>(w, n, order, stream, prof)) != hipSuccess)
+        return e;
+    // 2. the radius keys in id order, that order as the payload; a stable sort
+    //    on them leaves equal radii in id order
+    if (f32)
+        hipLaunchKernelGGL(k_radius_keys_gather<float>, grid, dim3(256), 0, stream, static_cast<const float *>(r),
+                           order, reinterpret_cast<uint32_t *>(w.keys[0]), w.vals[0], n);
+    else
+        hipLaunchKernelGGL(k_radius_keys_gather<double>, grid, dim3(256), 0, stream, static_cast<const double *>(r),
+                           order, w.keys[0], w.vals[0], n);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if (f32)
+        return sort_typed<uint32_t>(w, n, perm_out, stream, prof, 4, true, true);
+    return sort_typed<uint64_t>(w, n, perm_out, stream, prof, 8, true, true);
 }
 
 hipError_t radix_sort_perm_u8(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream)
