@@ -355,6 +355,21 @@ def measure_density(ns, precision: str, state, reps: int = 3) -> dict:
             "info": info}
 
 
+def measure_rank_files(ns, precision: str, state, path: str = "/dev/null") -> dict:
+    """The two Rank_Mass_Rad_VRad files of the benchmark state (ns_gpu_rank_files),
+    wall time per call with the rows written to `path`."""
+    n = state[0].size
+    p = ns.default_params()
+    with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
+        ctx.upload(*state)
+        ctx.rank_files(p, path, path)
+        t0 = time.perf_counter()
+        ctx.rank_files(p, path, path)
+        wall = time.perf_counter() - t0
+    return {"wall_ms": round(wall * 1e3, 2), "rows": 2 * n, "bytes": 60 * n, "path": path}
+
+
 def traffic_for(run: dict, n: int, kernel: str):
     return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
 
@@ -418,6 +433,8 @@ def main() -> int:
                     help="also time the snapshot path: an all_particle_data snapshot every K steps "
                          "(ns_apd_writer, written to --snapshot-path) against the same steps without")
     ap.add_argument("--snapshot-path", default="/dev/null")
+    ap.add_argument("--rank-files", action="store_true",
+                    help="also time the Rank_Mass_Rad_VRad files (ns_gpu_rank_files, rows to /dev/null)")
     ap.add_argument("--density", action="store_true",
                     help="also time the snapshot density estimate (ns_gpu_density) on the benchmark state")
     ap.add_argument("--potential", action="store_true",
@@ -523,6 +540,8 @@ def main() -> int:
         out["potential"] = measure_potential(ns, args.precision, state)
     if args.density:
         out["density"] = measure_density(ns, args.precision, state)
+    if args.rank_files:
+        out["rank_files"] = measure_rank_files(ns, args.precision, state)
     if args.snapshot_every:
         print("[bench] timing the snapshot path", file=sys.stderr, flush=True)
         out["snapshot"] = measure_snapshots(ns, args.precision, state, p, args.snapshot_every,

@@ -290,6 +290,20 @@ typedef struct ns_density_info {
 } ns_density_info;
 int ns_gpu_density(ns_gpu_ctx *ctx, const ns_step_params *p, double *density, ns_density_info *info);
 
+/* The Rank_Mass_Rad_VRad files of the current state taken as a snapshot
+ * (nsphere.c:7043-7231, 7698-7850, 7911-7989), written as fprintf_bin writes
+ * them (int32 for the rank, float32 for every other column):
+ *   unsorted_path, a row per id:  rank, (rank + 1) dM, r, v_r, PsiA, E, ell
+ *   sorted_path, by float32 radius (ties in id order):
+ *                                 i, (i + 1) dM, r, v_r, PsiA, E, ell, density
+ * with r, v_r, ell rounded to float32 as the snapshot stores them, rank the
+ * particle's position in the rank-ordered arrays, PsiA the method-A
+ * potential on the reference's linear spline through (0, PsiA0) and the
+ * sorted radii forced strictly increasing by steps of 1e-5, E = PsiA -
+ * ell^2 / (2 r^2) - v_r^2 / 2 and density the estimate of ns_gpu_density.
+ * Either path may be NULL.  Not with halos (NS_ERR_STATE). */
+int ns_gpu_rank_files(ns_gpu_ctx *ctx, const ns_step_params *p, const char *unsorted_path, const char *sorted_path);
+
 /* ---- initial conditions (nsphere.c:2386-2487, 4908-4920) -----------------
  *
  * Initial conditions drawn on the device straight into the context's state:

@@ -256,6 +256,11 @@ class GpuContext:
         check(lib.ns_gpu_density(self._h, ctypes.byref(params), out.ctypes.data, ctypes.byref(info)))
         return out, {k: getattr(info, k) for k, _ in DensityInfo._fields_}
 
+    def rank_files(self, params: StepParams, unsorted_path=None, sorted_path=None) -> None:
+        """The reference's Rank_Mass_Rad_VRad files of the current state."""
+        enc = (lambda p: None if p is None else str(p).encode())  # noqa: E731
+        check(lib.ns_gpu_rank_files(self._h, ctypes.byref(params), enc(unsorted_path), enc(sorted_path)))
+
     def snapshot_records(self) -> np.ndarray:
         """all_particle_data records of the current state in id order
         (rank, float32 r, v_r, ell), as nsphere.c:6456-6471 fills them."""

@@ -131,6 +131,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_snapshot_records": (i32, [vp, vp]),
         "ns_gpu_potential": (i32, [vp, c.POINTER(StepParams), vp, c.POINTER(c.c_double)]),
         "ns_gpu_density": (i32, [vp, c.POINTER(StepParams), vp, c.POINTER(DensityInfo)]),
+        "ns_gpu_rank_files": (i32, [vp, c.POINTER(StepParams), c.c_char_p, c.c_char_p]),
         "ns_gpu_generate_ic": (i32, [vp, i32, c.c_uint64, c.c_double, c.c_double, c.c_double]),
         "ns_gpu_upload_ic": (i32, [vp, vp, c.c_double]),
         "ns_ic_file_write": (i32, [c.c_char_p, i32, vp]),

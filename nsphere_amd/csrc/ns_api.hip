@@ -101,7 +101,8 @@ const char *kernel_name(int id)
                                          "gather_state", "integrate", "inverse_map", "energy",
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
                                          "integrate_deep", "mass_scan", "mlp_binning",
-                                         "snapshot_records", "potential", "density"};
+                                         "snapshot_records", "potential", "density",
+                                         "rank_files"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -978,6 +979,36 @@ int ns_gpu_potential(ns_gpu_ctx *c, const ns_step_params *p, double *psi, double
         return set_error(NS_ERR_HIP, "ns_gpu_potential: %s", hipGetErrorString(e));
     if (psi0)
         *psi0 = p->G * dm * total * p->vel_conv_sq;  // PsiA(0), nsphere.c:7733-7738
+    return NS_OK;
+}
+
+int ns_gpu_rank_files(ns_gpu_ctx *c, const ns_step_params *p, const char *unsorted_path, const char *sorted_path)
+{
+    if (int rc = check_ctx(c, "ns_gpu_rank_files"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_rank_files"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_rank_files: no state uploaded");
+    if (c->nseg > 0)
+        return set_error(NS_ERR_STATE, "ns_gpu_rank_files: one halo per context (the files rank a whole snapshot)");
+    DeviceGuard g(c->device);
+    const int s = c->cur;
+    int io_failed = 0;
+    c->prof.begin(ns::K_RANKFILES);
+    const hipError_t e = ns::rank_files(
+        c->sw, c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s], c->f32 ? (const void *)c->vf[s] : (const void *)c->v[s],
+        c->f32 ? (const void *)c->lf[s] : (const void *)c->l[s], c->f32, c->id[s], c->n, p->G, p->vel_conv_sq,
+        p->halo_mass, unsorted_path, sorted_path, c->stream, &io_failed);
+    c->prof.end(ns::K_RANKFILES);
+    c->keys_valid = false;  // the radius sort of the records used the key buffers
+    if (io_failed)
+        return set_error(NS_ERR_IO, "ns_gpu_rank_files: cannot write %s / %s", unsorted_path ? unsorted_path : "-",
+                         sorted_path ? sorted_path : "-");
+    if (e == hipErrorInvalidValue)
+        return set_error(NS_ERR_ARG, "ns_gpu_rank_files: the density estimate has fewer than three distinct radii");
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_rank_files: %s", hipGetErrorString(e));
     return NS_OK;
 }
 

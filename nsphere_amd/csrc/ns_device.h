@@ -175,6 +175,7 @@ enum KernelId : int {
     K_APD,
     K_POTENTIAL,
     K_DENSITY,
+    K_RANKFILES,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -115,6 +115,12 @@ int64_t potential_work_doubles(int64_t n);
 hipError_t density_estimate(const void *r_dev, bool f32, int64_t n, double halo_mass, double *out,
                             ns_density_info *info, hipStream_t stream);
 int64_t density_grid_size();
+struct SortWork;
+// Rank_Mass_Rad_VRad files of the current state (ns_rank.hip); io_failed is
+// set when a file cannot be written.
+hipError_t rank_files(SortWork &sw, const void *r, const void *v, const void *l, bool f32, const uint32_t *id,
+                      int64_t n, double G, double vcs, double halo_mass, const char *unsorted_path,
+                      const char *sorted_path, hipStream_t stream, int *io_failed);
 // Initial conditions (ns_io.hip): drawn on the device, or converted from an
 // initial-conditions file's records (5 doubles per particle).
 hipError_t launch_generate_ic(int kind, uint64_t seed, double rmax, double lowl_frac, double lowl_scale, int64_t n,
