@@ -253,7 +253,9 @@ typedef struct ns_apd_writer ns_apd_writer;
 int ns_apd_writer_open(ns_gpu_ctx *ctx, const char *path, int block_size, ns_apd_writer **out);
 int ns_apd_writer_write(ns_apd_writer *w);
 /* Writes what is pending, cuts a partial block, frees the writer; kept
- * (may be NULL) receives the number of snapshots in the file's whole blocks. */
+ * (may be NULL) receives the number of snapshots in the file's whole blocks.
+ * The context must outlive every ns_apd_writer_write; closing needs only the
+ * device, so it may also follow the context's destruction. */
 int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept);
 
 /* Dynamic potential, method A (nsphere.c:7698-7738), over the current radii

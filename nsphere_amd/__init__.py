@@ -333,6 +333,12 @@ class ApdWriter:
     def __exit__(self, *exc):
         self.close()
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def write_ic_file(path: str, records) -> None:
     """The reference's initial-conditions file (nsphere.c:2386-2416)."""

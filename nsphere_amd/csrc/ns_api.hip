@@ -918,6 +918,8 @@ int ns_gpu_snapshot_records(ns_gpu_ctx *c, ns_apd_record *host_out)
         return set_error(NS_ERR_ARG, "ns_gpu_snapshot_records: null output");
     if (!c->uploaded)
         return set_error(NS_ERR_STATE, "ns_gpu_snapshot_records: no state uploaded");
+    if (c->n > (int64_t)INT32_MAX)
+        return set_error(NS_ERR_ARG, "ns_gpu_snapshot_records: the record's rank is an int32");
     if (c->n == 0)
         return NS_OK;
     DeviceGuard g(c->device);
@@ -992,6 +994,8 @@ int ns_gpu_rank_files(ns_gpu_ctx *c, const ns_step_params *p, const char *unsort
         return set_error(NS_ERR_STATE, "ns_gpu_rank_files: no state uploaded");
     if (c->nseg > 0)
         return set_error(NS_ERR_STATE, "ns_gpu_rank_files: one halo per context (the files rank a whole snapshot)");
+    if (c->n > (int64_t)INT32_MAX)
+        return set_error(NS_ERR_ARG, "ns_gpu_rank_files: the rows' rank is an int32");
     DeviceGuard g(c->device);
     const int s = c->cur;
     int io_failed = 0;
@@ -1661,7 +1665,8 @@ int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[2])
 // ------------------------------------------------------ snapshot writer
 
 struct ns_apd_writer {
-    ns_gpu_ctx *ctx = nullptr;
+    ns_gpu_ctx *ctx = nullptr;  // used by writes only: closing needs no context
+    int device = 0;
     FILE *f = nullptr;
     int block_size = 1;
     int64_t n = 0;
@@ -1723,10 +1728,13 @@ int ns_apd_writer_open(ns_gpu_ctx *c, const char *path, int block_size, ns_apd_w
     *out = nullptr;
     if (!c->uploaded)
         return set_error(NS_ERR_STATE, "ns_apd_writer_open: no state uploaded");
+    if (c->n > (int64_t)INT32_MAX)
+        return set_error(NS_ERR_ARG, "ns_apd_writer_open: the record's rank is an int32");
     ns_apd_writer *w = new (std::nothrow) ns_apd_writer;
     if (!w)
         return set_error(NS_ERR_NOMEM, "ns_apd_writer_open");
     w->ctx = c;
+    w->device = c->device;
     w->block_size = block_size;
     w->n = c->n;
     w->f = fopen(path, "ab");
@@ -1795,7 +1803,7 @@ int ns_apd_writer_close(ns_apd_writer *w, int64_t *kept)
         return set_error(NS_ERR_ARG, "ns_apd_writer_close: null writer");
     int rc = NS_OK;
     {
-        DeviceGuard g(w->ctx->device);
+        DeviceGuard g(w->device);
         // oldest first: the slot not written last
         const int first = w->slot;
         rc = apd_flush_slot(w, first);
