@@ -67,7 +67,13 @@ MEASURED_TRAFFIC = {("f32", 12_000_000): {"sort_scatter": 171.2e6, "sort_upsweep
                     ("f64", 12_000_000): {"sort_scatter": 275.9e6, "sort_upsweep": 98.1e6,
                                           "integrate_bulk": 835.9e6, "integrate_deep": 16.1e6}}
 
-WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep"}
+WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
+            "integrate_late": "late"}
+# Kernel families timed with HIP events inside the timed region (two events
+# per launch): the integrate kernels and the sort's scatter, which the
+# rooflines use.  The other families are timed over a short run after it.
+ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", "sort_scatter"]
+AFTER_STEPS = 20
 
 
 
@@ -150,11 +156,12 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int, halos=None) -> d
 
 
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
-            mass=None, halos=None, graph=False, autotune=0, mlp=None) -> dict:
-    """Time `steps` steps after `warmup`.  Per-kernel HIP-event timing runs over
-    the timed region, unless graph: then the timed region replays captured
-    steps (no per-kernel events) and the kernel table comes from 10 profiled
-    steps after it."""
+            mass=None, halos=None, graph=False, autotune=0, mlp=None, overlap=True) -> dict:
+    """Time `steps` steps after `warmup`.  HIP-event timing of the roofline
+    kernel families (ROOFLINE_KERNELS) runs over the timed region, unless
+    graph: then the timed region replays captured steps (no per-kernel
+    events).  Every other family is timed over AFTER_STEPS profiled steps
+    after the timed region."""
     n = state[0].size
     ctx = ns.GpuContext(n)
     chunk = 10
@@ -163,6 +170,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
+        ctx.set_option(ns.NS_OPT_OVERLAP, int(overlap))
         if halos is not None:
             ctx.set_halos(halos)
         ctx.upload(*state)
@@ -177,6 +185,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_GRAPH, int(graph))
         ctx.cap_hits(reset=True)
         ctx.set_option(ns.NS_OPT_PROFILE, 0 if graph else 1)
+        ctx.profile_kernels(ROOFLINE_KERNELS)
         ctx.reset_kernel_stats()
         ctx.sync()
         t0 = time.perf_counter()
@@ -200,20 +209,34 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
             ctx.sync()
         stats = ctx.kernel_stats()
         evals = ctx.force_evals()
+        # every kernel family, over a short run after the timed region
+        ctx.set_option(ns.NS_OPT_PROFILE, 1)
+        ctx.profile_kernels(None)
+        ctx.reset_kernel_stats()
+        ctx.run(p, AFTER_STEPS)
+        ctx.sync()
+        after = ctx.kernel_stats()
+        after_evals = ctx.force_evals()
         ctx.set_option(ns.NS_OPT_PROFILE, 0)
     finally:
         ctx.close()
-    kernels = {}
-    for name, (launches, ms) in stats.items():
-        avg_ms = ms / max(launches, 1)
-        d = {"launches": launches, "total_ms": round(ms, 4), "avg_us": round(avg_ms * 1e3, 3)}
-        if name in bpp:
-            d["algorithmic_GBps"] = round(bpp[name] * n / (avg_ms * 1e-3) / 1e9, 1)
-        if name in WORK_ROW and ms > 0:
-            ev = evals[WORK_ROW[name]]
-            d["force_evals_per_launch"] = ev / max(launches, 1)
-            d["fp64_TFLOPs"] = round(FLOPS_PER_FORCE_EVAL * ev / (ms * 1e-3) / 1e12, 4)
-        kernels[name] = d
+
+    def table(st, ev_rows):
+        out = {}
+        for name, (launches, ms) in st.items():
+            avg_ms = ms / max(launches, 1)
+            d = {"launches": launches, "total_ms": round(ms, 4), "avg_us": round(avg_ms * 1e3, 3)}
+            if name in bpp:
+                d["algorithmic_GBps"] = round(bpp[name] * n / (avg_ms * 1e-3) / 1e9, 1)
+            if name in WORK_ROW and ms > 0:
+                ev = ev_rows[WORK_ROW[name]]
+                d["force_evals_per_launch"] = ev / max(launches, 1)
+                d["fp64_TFLOPs"] = round(FLOPS_PER_FORCE_EVAL * ev / (ms * 1e-3) / 1e12, 4)
+            out[name] = d
+        return out
+
+    kernels = table(stats, evals)
+    kernels_after = table(after, after_evals)
     lev = lv & ns.NS_LVL_MASK
     return {
         "precision": precision,
@@ -224,6 +247,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         "stats": stats,
         "evals": evals,
         "kernels": kernels,
+        "kernels_after": kernels_after,
         "E0": K0 + W0,
         "E1": K1 + W1,
         "cap_hits": hits,
@@ -427,6 +451,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="every kernel on one stream (NS_OPT_OVERLAP=0): the same results, for comparison")
     ap.add_argument("--graph", action="store_true",
                     help="time the chosen form with captured HIP graphs (NS_OPT_GRAPH)")
     ap.add_argument("--mlp", type=int, default=0, metavar="N_LOG",
@@ -491,10 +517,10 @@ def main() -> int:
     if args.mlp:
         mlp, learned = train_level_predictor(ns, kind, args.mlp, rmax, lowl, args.seed, p, args.budget)
     chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass, halos,
-                     args.graph, args.autotune, mlp)
+                     args.graph, args.autotune, mlp, overlap=not args.no_overlap)
     if mlp is not None:  # the same form without the network
         without = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass,
-                          halos, args.graph, args.autotune)
+                          halos, args.graph, args.autotune, overlap=not args.no_overlap)
         learned.update({"value_without": without["value"], "ms_per_step_without": without["ms_per_step"],
                         "gain": chosen["value"] / without["value"],
                         "kernels_without": without["kernels"],
@@ -521,7 +547,7 @@ def main() -> int:
         "config": {"workload": args.workload, "n": n, "dt": p.dt, "form": args.form,
                    "state": args.precision, "lowl_frac": lowl, "lowl_scale": 1e-3, "seed": args.seed,
                    "masses": args.masses, "halos": None if halos is None else int(len(halos)),
-                   "graph": args.graph},
+                   "graph": args.graph, "overlap": not args.no_overlap and args.form != "plain"},
         "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
@@ -531,6 +557,8 @@ def main() -> int:
                           "frac": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9 / PEAK_HBM_GBS, 4)},
         "dominant_kernel": dom,
         "kernels": chosen["kernels"],
+        "kernels_after": {"steps": AFTER_STEPS, "note": "every kernel family, timed over the steps after the "
+                                                        "timed region", **chosen["kernels_after"]},
         "dE_over_E": (chosen["E1"] - chosen["E0"]) / abs(chosen["E0"]),
         "levels": chosen["levels"],
         "work_cap": {"force_evals_per_particle_step": args.budget, "capped_particle_steps": chosen["cap_hits"]},

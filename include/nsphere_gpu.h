@@ -76,7 +76,12 @@ void ns_step_params_unit(ns_step_params *p);
 enum {
     NS_OPT_TIEBREAK = 1,   /* 0: stable on input order, exactly as quadsort
                               (default); 1: exact ties broken by particle id    */
-    NS_OPT_PROFILE = 2,    /* 1: time every kernel with HIP events             */
+    NS_OPT_PROFILE = 2,    /* 1: time every kernel with HIP events (those of
+                              NS_OPT_PROFILE_KERNELS) and count the force
+                              evaluations                                     */
+    NS_OPT_PROFILE_KERNELS = 12, /* bit k set: time kernel family k (the index
+                              of its name in ns_gpu_kernel_stats); default
+                              all.  Each timed launch costs two events.       */
     NS_OPT_INTEGRATOR = 3, /* 0 plain: one lane per particle, levels in turn,
                                  strict IEEE order of the reference (default);
                               1 binned: leapfrog levels 0-3 per lane; deeper
@@ -107,12 +112,23 @@ enum {
                               keys (4 sort passes), float64 arithmetic in the
                               integrator and the energy.  Switching converts
                               the current state (round to nearest / exact). */
-    NS_OPT_BUDGET = 4      /* force evaluations allowed per particle and step
+    NS_OPT_BUDGET = 4,     /* force evaluations allowed per particle and step
                               (default 2^26); a particle that needs more is
                               stopped, flagged NS_LVL_NOCONV and counted
                               (ns_gpu_cap_hits).  Not in the reference, whose
                               Levi-Civita tau steps converge only to first
                               order and can run 2^28 subdivisions each       */
+    NS_OPT_OVERLAP = 11    /* binned forms, ns_gpu_run of two or more steps:
+                              1 (default): particles whose previous step was
+                              slow (flagged NS_LVL_NOCONV or at level 8 or
+                              deeper; up to 1024 per step) are
+                              integrated on a second stream while the main
+                              stream integrates the rest and sorts the next
+                              step's radii; a merge then puts them at their
+                              sorted ranks.  0: every kernel on the context's
+                              stream.  Same result.  Not taken with graphs,
+                              tiebreak by id, ensembles, per-particle masses
+                              or learned binning.                            */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;
@@ -150,7 +166,9 @@ int ns_gpu_integrate(ns_gpu_ctx *ctx, const ns_step_params *p);
 /* One method-5 step: sort, then integrate at the new ranks.  Asynchronous:
  * returns once the work is queued on the context's stream. */
 int ns_gpu_step(ns_gpu_ctx *ctx, const ns_step_params *p);
-/* nsteps steps back to back, queued without host synchronisation. */
+/* nsteps steps back to back, queued without host synchronisation (with
+ * NS_OPT_OVERLAP the context's stream waits for the second stream before the
+ * call returns, so the context's stream orders all later work). */
 int ns_gpu_run(ns_gpu_ctx *ctx, const ns_step_params *p, int64_t nsteps);
 int ns_gpu_sync(ns_gpu_ctx *ctx);
 /* Particle-steps stopped by the work cap (NS_OPT_BUDGET) since the context
@@ -374,9 +392,10 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);
 /* Force evaluations (the work units of NS_OPT_BUDGET: the reference's
  * sequential count) spent while NS_OPT_PROFILE was on, since the last
  * ns_gpu_kernel_stats_reset: out[0] by the plain kernel or the binned forms'
- * bulk kernel, out[1] by their wave-per-particle kernel.  Synchronises the
- * context's stream. */
-int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[2]);
+ * bulk kernel, out[1] by their wave-per-particle kernel on the context's
+ * stream (deep), out[2] by the late kernel of overlapped runs.  Synchronises
+ * the context's stream. */
+int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[3]);
 
 /* Seeded initial conditions in G = M = a = 1 units (host, deterministic per
  * particle through a Philox-4x32-10 counter stream keyed by seed):

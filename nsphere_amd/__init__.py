@@ -20,16 +20,17 @@ import ctypes
 import numpy as np
 
 from ._lib import (FORM_BINNED, FORM_BINNED_FAST, FORM_PLAIN, NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV,
-                   NS_OPT_BUDGET, NS_OPT_INTEGRATOR, NS_OPT_PRECISION, NS_OPT_PROFILE,
-                   NS_OPT_GRAPH, NS_OPT_SORT_ALG, NS_OPT_SORT_TILE, NS_OPT_TIEBREAK, SORT_ONESWEEP, SORT_REDUCE_SCAN,
-                   NsError, StepParams, check, lib)
+                   NS_OPT_BUDGET, NS_OPT_INTEGRATOR, NS_OPT_PRECISION, NS_OPT_PROFILE, NS_OPT_PROFILE_KERNELS,
+                   NS_OPT_GRAPH, NS_OPT_OVERLAP, NS_OPT_SORT_ALG, NS_OPT_SORT_TILE, NS_OPT_TIEBREAK,
+                   SORT_ONESWEEP, SORT_REDUCE_SCAN, NsError, StepParams, check, lib)
 
 __all__ = [
     "GpuContext", "StepParams", "NsError", "default_params", "unit_params", "generate_ic",
     "sort_particles", "method5_step", "philox4x32_10",
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
-    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "Mlp", "level_samples", "ApdWriter", "APD_RECORD",
+    "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "NS_OPT_OVERLAP", "NS_OPT_PROFILE_KERNELS",
+    "Mlp", "level_samples", "ApdWriter", "APD_RECORD",
     "read_all_particle_data", "write_ic_file", "read_ic_file", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]
 
@@ -297,9 +298,29 @@ class GpuContext:
 
     def force_evals(self) -> dict:
         """Force evaluations spent while profiling, per integrate kernel."""
-        out = (ctypes.c_uint64 * 2)()
+        out = (ctypes.c_uint64 * 3)()
         check(lib.ns_gpu_force_evals(self._h, out))
-        return {"bulk_or_plain": out[0], "deep": out[1]}
+        return {"bulk_or_plain": out[0], "deep": out[1], "late": out[2]}
+
+    def kernel_families(self) -> list:
+        """Kernel family names in id order (bit k of NS_OPT_PROFILE_KERNELS)."""
+        n = ctypes.c_int()
+        from ._lib import NS_MAX_KERNEL_STATS as cap
+        names = (ctypes.c_char_p * cap)()
+        check(lib.ns_gpu_kernel_stats(self._h, ctypes.byref(n), names, None, None))
+        return [names[i].decode() for i in range(n.value)]
+
+    def profile_kernels(self, names) -> None:
+        """Time only these kernel families while profiling (None: all)."""
+        if names is None:
+            mask = 0xFFFFFFFF
+        else:
+            fam = self.kernel_families()
+            unknown = set(names) - set(fam)
+            if unknown:
+                raise ValueError(f"unknown kernel families {sorted(unknown)}")
+            mask = sum(1 << fam.index(x) for x in names)
+        self.set_option(NS_OPT_PROFILE_KERNELS, mask)
 
 
 APD_RECORD = np.dtype([("rank", "<i4"), ("r", "<f4"), ("v", "<f4"), ("ell", "<f4")])

@@ -56,6 +56,12 @@ struct ns_gpu_ctx {
     unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
     int64_t work_stride = 0;
     uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
+    // NS_OPT_OVERLAP: merge buffers, the second stream and its fork / join
+    // events (allocated on first use)
+    int overlap = 1;
+    ns::OverlapBufs ov = {};
+    hipStream_t side = nullptr;
+    hipEvent_t ev_merged = nullptr, ev_late = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool levels_valid = false;
@@ -102,7 +108,7 @@ const char *kernel_name(int id)
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
                                          "integrate_deep", "mass_scan", "mlp_binning",
                                          "snapshot_records", "potential", "density",
-                                         "rank_files"};
+                                         "rank_files", "integrate_late", "overlap_merge"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -116,24 +122,24 @@ hipEvent_t Profiler::take()
     return e;
 }
 
-void Profiler::begin(int kid)
+void Profiler::begin(int kid, hipStream_t s)
 {
-    if (!on)
+    if (!on || !((mask >> kid) & 1u))
         return;
     open_ = take();
     open_kid_ = kid;
     if (open_)
-        (void)hipEventRecord(open_, stream);
+        (void)hipEventRecord(open_, s ? s : stream);
 }
 
-void Profiler::end(int kid)
+void Profiler::end(int kid, hipStream_t s)
 {
     if (!on || !open_ || open_kid_ != kid)
         return;
     hipEvent_t b = take();
     if (!b)
         return;
-    (void)hipEventRecord(b, stream);
+    (void)hipEventRecord(b, s ? s : stream);
     if (npend_ == cap_) {
         int nc = cap_ ? 2 * cap_ : 256;
         Pending *np = (Pending *)realloc(pend_, (size_t)nc * sizeof(Pending));
@@ -145,7 +151,7 @@ void Profiler::end(int kid)
     pend_[npend_++] = Pending{kid, open_, b};
     open_ = nullptr;
     if (npend_ >= 4096)
-        fold();
+        (void)fold();
 }
 
 hipError_t Profiler::fold()
@@ -182,7 +188,7 @@ hipError_t Profiler::fold()
 
 void Profiler::reset()
 {
-    fold();
+    (void)fold();
     for (int k = 0; k < K_COUNT; k++) {
         launches[k] = 0;
         ms[k] = 0.0;
@@ -191,7 +197,7 @@ void Profiler::reset()
 
 void Profiler::release()
 {
-    fold();
+    (void)fold();
     for (int i = 0; i < npool_; i++)
         (void)hipEventDestroy(pool_[i]);
     free(pool_);
@@ -247,10 +253,38 @@ void free_all(ns_gpu_ctx *c)
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
-                    c->order, c->id_order};
+                    c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.def_pos, c->ov.ins_lo, c->ov.ctl};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
+    for (hipEvent_t ev : {c->ev_merged, c->ev_late})
+        if (ev)
+            (void)hipEventDestroy(ev);
+    if (c->side)
+        (void)hipStreamDestroy(c->side);
+}
+
+// Buffers, second stream and events of an overlapped run, on first use.
+int ensure_overlap(ns_gpu_ctx *c)
+{
+    hipError_t e = hipSuccess;
+    if (!c->ov.perm)
+        e = dalloc(&c->ov.perm, c->n);
+    uint32_t **small[] = {&c->ov.slow, &c->ov.def_pos, &c->ov.ins_lo};
+    for (uint32_t **b : small)
+        if (e == hipSuccess && !*b)
+            e = dalloc(b, ns::kSlowCap);
+    if (e == hipSuccess && !c->ov.ctl)
+        e = dalloc(&c->ov.ctl, 4);
+    if (e == hipSuccess && !c->side)
+        e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    for (hipEvent_t *ev : {&c->ev_merged, &c->ev_late})
+        if (e == hipSuccess && !*ev)
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e != hipSuccess)
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "overlap buffers (npts=%lld): %s",
+                         (long long)c->n, hipGetErrorString(e));
+    return NS_OK;
 }
 
 // Float32 state buffers of the mixed-precision form, allocated on first use.
@@ -442,10 +476,10 @@ int sort_perm(ns_gpu_ctx *c)
     return NS_OK;
 }
 
-int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
+// Arguments of the integrate kernels for the state of buffer cur: gather
+// (through perm) into buffer cur^1, or in place.
+ns::StepArgs state_args(ns_gpu_ctx *c, const ns_step_params *p, bool gather, const uint32_t *perm)
 {
-    if (int rc = mass_scan(c, gather ? c->perm : nullptr, gather))
-        return rc;
     ns::StepArgs a = make_args(c, p);
     // gather: previous order in buffer cur -> new rank order in cur^1;
     // otherwise in place (each thread reads and writes only its own index).
@@ -466,10 +500,19 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
         a.l1 = gather ? c->l[d] : nullptr;
     }
     a.id0 = c->id[s];
-    a.perm = gather ? c->perm : nullptr;
+    a.perm = gather ? perm : nullptr;
     a.id1 = gather ? c->id[d] : nullptr;
     a.keys = c->sw.keys[0];
     a.lvl = c->lvl;
+    return a;
+}
+
+int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
+{
+    if (int rc = mass_scan(c, gather ? c->perm : nullptr, gather))
+        return rc;
+    ns::StepArgs a = state_args(c, p, gather, c->perm);
+    const int d = gather ? (c->cur ^ 1) : c->cur;
     if (c->mlp && c->integrator != ns::FORM_PLAIN) {
         // learned binning: predicted levels (as 32-bit labels in the key
         // buffer, free until the integrator writes the next radius keys),
@@ -485,6 +528,73 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
     c->cur = d;
     c->keys_valid = true;
     c->levels_valid = true;
+    return NS_OK;
+}
+
+// Whether ns_gpu_run overlaps the late particles with the next sort.
+bool overlap_applies(const ns_gpu_ctx *c, int64_t nsteps)
+{
+    return c->overlap && nsteps >= 2 && c->integrator != ns::FORM_PLAIN && !c->tiebreak && c->nseg == 0 &&
+           !c->has_mass && !c->mlp && c->n <= (int64_t)ns::kIndexMask;
+}
+
+// One step of an overlapped run: the sort (late keys = sentinel), the wait
+// for the previous step's late kernel, the merge, then the late kernel on the
+// second stream beside the bulk and deep kernels on the context's stream.
+int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, bool first)
+{
+    if (int rc = sort_perm(c))
+        return rc;
+    if (!first)
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_late, 0));
+    const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
+    c->prof.begin(ns::K_MERGE);
+    HIP_TRY(ns::overlap_merge(c->ov, c->perm, r, c->f32, c->n, c->stream));
+    c->prof.end(ns::K_MERGE);
+    HIP_TRY(hipEventRecord(c->ev_merged, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_merged, 0));
+    ns::StepArgs a = state_args(c, p, true, c->ov.perm);
+    a.slow_list = c->ov.slow;
+    a.slow_count = c->ov.ctl;  // ctl[0]
+    ns::StepArgs late = a;
+    late.keys = nullptr;  // the sentinel stays until the next merge
+    c->prof.begin(ns::K_INT_LATE, c->side);
+    HIP_TRY(ns::launch_late(late, c->integrator, c->f32, c->ov.def_pos, c->ov.ctl + 1, c->side));
+    c->prof.end(ns::K_INT_LATE, c->side);
+    HIP_TRY(hipEventRecord(c->ev_late, c->side));
+    HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
+    c->cur ^= 1;
+    c->keys_valid = true;
+    c->levels_valid = true;
+    return NS_OK;
+}
+
+// nsteps >= 2 overlapped steps; the context's stream then waits for the
+// second stream and the last step's late ranks get their true keys.
+int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
+{
+    if (int rc = ensure_overlap(c))
+        return rc;
+    // the slow list of the last step, from its level records (whatever form
+    // ran it): the first step defers the chains that step found slow
+    HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->n, c->stream));
+    int rc = NS_OK;
+    int64_t done = 0;
+    for (; done < nsteps && rc == NS_OK; done++)
+        rc = overlapped_step(c, p, done == 0);
+    if (done > 0) {
+        // join even after a failed step, so the second stream is never left
+        // running past the call
+        const hipError_t e = hipStreamWaitEvent(c->stream, c->ev_late, 0);
+        if (rc == NS_OK && e != hipSuccess)
+            rc = set_error(NS_ERR_HIP, "ns_gpu_run: join of the second stream: %s", hipGetErrorString(e));
+    }
+    if (rc != NS_OK) {
+        c->keys_valid = false;  // late ranks may still hold the key sentinel
+        return rc;
+    }
+    const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
+    HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->stream));
     return NS_OK;
 }
 
@@ -601,6 +711,8 @@ int ns_gpu_ctx_destroy(ns_gpu_ctx *c)
         return NS_OK;
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->side)
+        (void)hipStreamSynchronize(c->side);
     c->prof.release();
     release_graph(c);
     free_all(c);
@@ -647,6 +759,16 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         return NS_OK;
     case NS_OPT_PROFILE:
         c->prof.on = value != 0;
+        return NS_OK;
+    case NS_OPT_PROFILE_KERNELS:
+        if (value < 0 || value > 0xFFFFFFFFll)
+            return set_error(NS_ERR_ARG, "NS_OPT_PROFILE_KERNELS takes a 32-bit mask of kernel families");
+        c->prof.mask = (uint32_t)value;
+        return NS_OK;
+    case NS_OPT_OVERLAP:
+        if (value != 0 && value != 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_OVERLAP takes 0 or 1");
+        c->overlap = (int)value;
         return NS_OK;
     case NS_OPT_INTEGRATOR:
         if (value < ns::FORM_PLAIN || value > ns::FORM_BINNED_FAST)
@@ -1127,6 +1249,12 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
     if (nsteps < 0)
         return set_error(NS_ERR_ARG, "ns_gpu_run: nsteps < 0");
     const bool graph = c && c->use_graph && !c->prof.on && c->sw.config.alg == ns::kSortReduceScan && nsteps >= 3;
+    if (!graph && c && c->uploaded && overlap_applies(c, nsteps)) {
+        if (int rc = check_params(p, "ns_gpu_run"))
+            return rc;
+        DeviceGuard g(c->device);
+        return run_overlapped(c, p, nsteps);
+    }
     if (!graph) {
         for (int64_t s = 0; s < nsteps; s++)
             if (int rc = ns_gpu_step(c, p))
@@ -1182,6 +1310,8 @@ int ns_gpu_sync(ns_gpu_ctx *c)
         return rc;
     DeviceGuard g(c->device);
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->side)
+        HIP_TRY(hipStreamSynchronize(c->side));
     return NS_OK;
 }
 
@@ -1632,7 +1762,7 @@ int ns_gpu_enclosed_mass(ns_gpu_ctx *c, double *host_out)
     return NS_OK;
 }
 
-int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[2])
+int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[3])
 {
     if (int rc = check_ctx(c, "ns_gpu_force_evals"))
         return rc;
@@ -1650,6 +1780,7 @@ int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[2])
         free(h);
         return set_error(NS_ERR_HIP, "ns_gpu_force_evals: %s", hipGetErrorString(e));
     }
+    static_assert(ns::kWorkKernels == 3, "ns_gpu_force_evals reports three rows");
     for (int k = 0; k < ns::kWorkKernels; k++) {
         uint64_t s = 0;
         for (int64_t j = 0; j < c->work_stride; j++)

@@ -176,20 +176,24 @@ enum KernelId : int {
     K_POTENTIAL,
     K_DENSITY,
     K_RANKFILES,
+    K_INT_LATE,
+    K_MERGE,
     K_COUNT
 };
 const char *kernel_name(int id);
 
 // Per-kernel-family timing with HIP events on the launching stream.  begin()
-// and end() bracket one launch; fold() waits for recorded pairs and adds
-// their durations.  Disabled (no events) unless `on`.
+// and end() bracket one launch (on `stream`, or on s when given); fold()
+// waits for recorded pairs and adds their durations.  Disabled (no events)
+// unless `on`, and only for the families whose bit (1 << id) is in mask.
 struct Profiler {
     bool on = false;
+    uint32_t mask = ~0u;
     hipStream_t stream = nullptr;
     int64_t launches[K_COUNT] = {};
     double ms[K_COUNT] = {};
-    void begin(int kid);
-    void end(int kid);
+    void begin(int kid, hipStream_t s = nullptr);
+    void end(int kid, hipStream_t s = nullptr);
     hipError_t fold();
     void reset();
     void release();

@@ -83,8 +83,11 @@ __device__ inline uint32_t wave_ticket(uint32_t *ticket)
     return __shfl(q, 0, 64);
 }
 
+// slow_tag: what the slow list records for a slow result (the rank, or the
+// late kernel's slot | kDeferBit).
 template <typename S>
-__device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit)
+__device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit,
+                                    uint32_t slow_tag)
 {
     const S rs = (S)r;
     static_cast<S *>(a.r1)[i] = rs;
@@ -95,12 +98,30 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
         a.lvl[i] = (uint8_t)rec;
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
+    if (a.slow_list && slow_record(rec)) {
+        const uint32_t k = atomicAdd(a.slow_count, 1u);
+        if (k < (uint32_t)kSlowCap)
+            a.slow_list[k] = slow_tag;
+    }
+}
+
+template <typename S>
+__device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit)
+{
+    write_result<S>(a, i, r, v, rec, hit, (uint32_t)i);
+}
+
+// Previous position of rank i (perm entries may carry kDeferBit).
+template <bool GATHER>
+__device__ inline int64_t source_of(const StepArgs &a, int64_t i)
+{
+    return GATHER ? (int64_t)(a.perm[i] & kIndexMask) : i;
 }
 
 template <bool GATHER, typename S>
 __device__ inline void load_particle(const StepArgs &a, int64_t i, double &r, double &v, double &ell)
 {
-    const int64_t src = GATHER ? (int64_t)a.perm[i] : i;
+    const int64_t src = source_of<GATHER>(a, i);
     r = (double)static_cast<const S *>(a.r0)[src];
     v = (double)static_cast<const S *>(a.v0)[src];
     ell = (double)static_cast<const S *>(a.l0)[src];
@@ -111,7 +132,7 @@ __device__ inline void carry_ell_id(const StepArgs &a, int64_t i, double ell)
 {
     if (GATHER) {
         static_cast<S *>(a.l1)[i] = (S)ell;
-        a.id1[i] = a.id0[a.perm[i]];
+        a.id1[i] = a.id0[source_of<GATHER>(a, i)];
     }
 }
 
@@ -149,7 +170,14 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
     const int64_t i = (a.order && valid) ? (int64_t)a.order[t] : t;
     bool to_lc = false, to_lf = false;
     unsigned long long spent = 0;
-    if (valid) {
+    if (valid && GATHER && a.slow_list && (a.perm[i] & kDeferBit)) {
+        // integrated by the late kernel on the side stream
+        const int64_t src = source_of<GATHER>(a, i);
+        static_cast<S *>(a.l1)[i] = static_cast<const S *>(a.l0)[src];
+        a.id1[i] = a.id0[src];
+        if (a.keys)
+            static_cast<typename KeyOf<S>::type *>(a.keys)[i] = ~(typename KeyOf<S>::type)0;
+    } else if (valid) {
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
         carry_ell_id<GATHER, S>(a, i, ell);
@@ -200,6 +228,45 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
         }
     }
     add_work(a, 1, spent);
+}
+
+// The late kernel of an overlapped run: one wave per deferred rank (block
+// j takes def_pos[j]), the whole of its step as the bulk and deep kernels
+// would run it -- a Levi-Civita particle on the wave from the start, a
+// leapfrog one lane-per-particle up to cap_N and then on the wave -- so the
+// results are those of the other kernels bit for bit.  Its key stays the
+// sentinel the bulk kernel wrote; the merge of the next step reads r.
+template <bool FAST, typename S>
+__global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_t *__restrict__ def_pos,
+                                                       const uint32_t *__restrict__ count)
+{
+    const uint32_t j = blockIdx.x;
+    if (j >= *count)
+        return;
+    const int64_t i = def_pos[j];
+    double r, v, ell;
+    load_particle<true, S>(a, i, r, v, ell);
+    const RankMass rm = rank_mass(a, i);
+    const Phys ph = make_phys(a, i, rm, ell);
+    Budget bud{a.budget, false};
+    double r_new, v_new;
+    unsigned long long spent;
+    int rec;
+    if (takes_levi_civita(a, rm, r, ell)) {
+        rec = adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud);
+        spent = (unsigned long long)(a.budget - bud.left);
+    } else {
+        rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_N);
+        spent = (unsigned long long)(a.budget - bud.left);
+        if (rec == kEscalate) {
+            bud = Budget{a.budget, false};
+            rec = adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
+            spent += (unsigned long long)(a.budget - bud.left);
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit);
+    add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -372,6 +439,27 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
         return hipSuccess;
     return f32 ? launch_typed<float>(a, gather, form, stream, prof)
                : launch_typed<double>(a, gather, form, stream, prof);
+}
+
+hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *count,
+                       hipStream_t stream)
+{
+    if (a.n <= 0)
+        return hipSuccess;
+    const bool fast = form == FORM_BINNED_FAST;
+    const dim3 grid((unsigned)kSlowCap), block(64);
+    if (f32) {
+        if (fast)
+            hipLaunchKernelGGL((k_integrate_late<true, float>), grid, block, 0, stream, a, def_pos, count);
+        else
+            hipLaunchKernelGGL((k_integrate_late<false, float>), grid, block, 0, stream, a, def_pos, count);
+    } else {
+        if (fast)
+            hipLaunchKernelGGL((k_integrate_late<true, double>), grid, block, 0, stream, a, def_pos, count);
+        else
+            hipLaunchKernelGGL((k_integrate_late<false, double>), grid, block, 0, stream, a, def_pos, count);
+    }
+    return hipGetLastError();
 }
 
 template <typename S>

@@ -47,13 +47,34 @@ struct StepArgs {
     // profiling only (else null): force evaluations spent, one slot per
     // wave of a launch (slot = global wave index; each wave adds its own with
     // a plain read-modify-write, launches being stream-ordered), in rows of
-    // work_stride slots: row 0 plain or bulk, 1 deep
+    // work_stride slots: row 0 plain or bulk, 1 deep, 2 late (side stream)
     unsigned long long *work;
     int64_t work_stride;
+    // overlapped run (else null): every slow result -- flagged NS_LVL_NOCONV
+    // or at level kSlowLevel or deeper -- appends its rank (the late kernel:
+    // its slot | kDeferBit) to slow_list, up to kSlowCap entries (slow_count
+    // keeps counting past them).  In that mode a perm entry with kDeferBit
+    // set marks a rank that the late kernel integrates on the side stream:
+    // the bulk kernel only carries its ell and id and writes the key sentinel
+    // (all ones, above every radius key).
+    uint32_t *slow_list;
+    uint32_t *slow_count;
 };
 
-constexpr int kWorkKernels = 2;
+constexpr int kWorkKernels = 3;
 constexpr int kQueueCounters = 3;
+constexpr uint32_t kDeferBit = 0x80000000u;
+constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
+constexpr int kSlowCap = 1024;
+// A chain that reached level 8 (2^9 or more sequential micro-steps) or hit
+// the cap is likely to be long again in the next step: a particle plunging
+// towards the centre climbs through the deep leapfrog levels before it turns
+// into a capped Levi-Civita chain.
+constexpr int kSlowLevel = 8;
+__host__ __device__ inline bool slow_record(int rec)
+{
+    return (rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel;
+}
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
 
@@ -96,6 +117,32 @@ struct Profiler;
 // K_INT_BULK and K_INT_DEEP (prof may be null).
 hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, hipStream_t stream,
                             Profiler *prof);
+
+// Overlapped run (ns_overlap.hip).  Device control words of ctl:
+//   [0] slow_count (slow results of this step, all kernels)
+//   [1] deferred ranks of the coming step (entries of def_pos)
+//   [2] ranks deferred in the step just sorted (sorted last, by key sentinel)
+struct OverlapBufs {
+    uint32_t *perm;      // n: the merged order, kDeferBit on deferred ranks
+    uint32_t *slow;      // kSlowCap: StepArgs::slow_list
+    uint32_t *def_pos;   // kSlowCap: deferred ranks
+    uint32_t *ins_lo;    // kSlowCap: insertion points of the late ranks, ascending
+    uint32_t *ctl;       // 4 words
+};
+// Clears the control words; with lvl (the level records of the current
+// order) the slow list is seeded from them, else left empty.
+hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hipStream_t stream);
+// After the sort of r (state type S, keys of deferred ranks = sentinel) into
+// sorted: the merged order in o.perm, the deferred ranks of the coming step
+// (the previous step's slow list at their new ranks) in def_pos / ctl[1].
+hipError_t overlap_merge(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
+                         hipStream_t stream);
+// The late kernel: every rank of def_pos integrated on one wave (a.keys
+// unused), on the side stream.
+hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *count,
+                       hipStream_t stream);
+// Radius keys of the deferred ranks of the last step (replacing the sentinel).
+hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, hipStream_t stream);
 template <typename S>
 hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
                                S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);
