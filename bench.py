@@ -164,7 +164,9 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
     after the timed region."""
     n = state[0].size
     ctx = ns.GpuContext(n)
-    chunk = 10
+    # one host sync (and progress line) per chunk: each ns_gpu_run call ends
+    # by joining its streams, so chunks are long enough to keep that rare
+    chunk = 100
     bpp = bytes_per_particle(4 if precision == "f32" else 8)
     try:
         ctx.set_option(ns.NS_OPT_PRECISION, PRECISIONS[precision])
@@ -193,9 +195,9 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         while done < steps:
             k = min(chunk, steps - done)
             ctx.run(p, k)
-            ctx.sync()  # progress line every `chunk` steps (one host sync per chunk)
+            ctx.sync()
             done += k
-            if done % 50 == 0 or done == steps:
+            if done % 100 == 0 or done == steps:
                 print(f"[bench] {form}: {done}/{steps} steps, {time.perf_counter() - t0:.2f}s",
                       file=sys.stderr, flush=True)
         el = time.perf_counter() - t0

@@ -108,7 +108,8 @@ const char *kernel_name(int id)
                                          "sort_tiefix", "sort_upsweep", "sort_scan", "integrate_bulk",
                                          "integrate_deep", "mass_scan", "mlp_binning",
                                          "snapshot_records", "potential", "density",
-                                         "rank_files", "integrate_late", "overlap_merge"};
+                                         "rank_files", "integrate_late", "overlap_plan",
+                                         "overlap_place"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -253,7 +254,8 @@ void free_all(ns_gpu_ctx *c)
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
-                    c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.def_pos, c->ov.ins_lo, c->ov.ctl};
+                    c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.slow_rec, c->ov.def_pos, c->ov.def_src, c->ov.ins_lo,
+                    c->ov.found, c->ov.ctl};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -270,10 +272,12 @@ int ensure_overlap(ns_gpu_ctx *c)
     hipError_t e = hipSuccess;
     if (!c->ov.perm)
         e = dalloc(&c->ov.perm, c->n);
-    uint32_t **small[] = {&c->ov.slow, &c->ov.def_pos, &c->ov.ins_lo};
+    uint32_t **small[] = {&c->ov.slow, &c->ov.slow_rec, &c->ov.def_pos, &c->ov.def_src, &c->ov.ins_lo};
     for (uint32_t **b : small)
         if (e == hipSuccess && !*b)
             e = dalloc(b, ns::kSlowCap);
+    if (e == hipSuccess && !c->ov.found)
+        e = dalloc(&c->ov.found, 2 * ns::kSlowCap);
     if (e == hipSuccess && !c->ov.ctl)
         e = dalloc(&c->ov.ctl, 4);
     if (e == hipSuccess && !c->side)
@@ -548,20 +552,26 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, bool first)
     if (!first)
         HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_late, 0));
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
+    // the plan is all the late kernel needs; the placement of the early
+    // elements runs beside it, before the bulk kernel
     c->prof.begin(ns::K_MERGE);
-    HIP_TRY(ns::overlap_merge(c->ov, c->perm, r, c->f32, c->n, c->stream));
+    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->n, c->stream));
     c->prof.end(ns::K_MERGE);
     HIP_TRY(hipEventRecord(c->ev_merged, c->stream));
     HIP_TRY(hipStreamWaitEvent(c->side, c->ev_merged, 0));
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
     a.slow_list = c->ov.slow;
+    a.slow_rec = c->ov.slow_rec;
     a.slow_count = c->ov.ctl;  // ctl[0]
     ns::StepArgs late = a;
     late.keys = nullptr;  // the sentinel stays until the next merge
     c->prof.begin(ns::K_INT_LATE, c->side);
-    HIP_TRY(ns::launch_late(late, c->integrator, c->f32, c->ov.def_pos, c->ov.ctl + 1, c->side));
+    HIP_TRY(ns::launch_late(late, c->integrator, c->f32, c->ov.def_pos, c->ov.def_src, c->ov.ctl + 1, c->side));
     c->prof.end(ns::K_INT_LATE, c->side);
     HIP_TRY(hipEventRecord(c->ev_late, c->side));
+    c->prof.begin(ns::K_PLACE);
+    HIP_TRY(ns::overlap_place(c->ov, c->perm, c->n, c->stream));
+    c->prof.end(ns::K_PLACE);
     HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
     c->cur ^= 1;
     c->keys_valid = true;

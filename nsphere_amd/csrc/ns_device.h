@@ -178,6 +178,7 @@ enum KernelId : int {
     K_RANKFILES,
     K_INT_LATE,
     K_MERGE,
+    K_PLACE,
     K_COUNT
 };
 const char *kernel_name(int id);

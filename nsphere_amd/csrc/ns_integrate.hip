@@ -98,10 +98,12 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
         a.lvl[i] = (uint8_t)rec;
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
-    if (a.slow_list && slow_record(rec)) {
+    if (a.slow_list && slow_record(rec, i)) {
         const uint32_t k = atomicAdd(a.slow_count, 1u);
-        if (k < (uint32_t)kSlowCap)
+        if (k < (uint32_t)kSlowCap) {
             a.slow_list[k] = slow_tag;
+            a.slow_rec[k] = (uint32_t)rec;
+        }
     }
 }
 
@@ -231,21 +233,24 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 }
 
 // The late kernel of an overlapped run: one wave per deferred rank (block
-// j takes def_pos[j]), the whole of its step as the bulk and deep kernels
-// would run it -- a Levi-Civita particle on the wave from the start, a
-// leapfrog one lane-per-particle up to cap_N and then on the wave -- so the
-// results are those of the other kernels bit for bit.  Its key stays the
+// j takes rank def_pos[j] from position def_src[j] of the previous order),
+// the whole of its step as the bulk and deep kernels would run it -- a
+// Levi-Civita particle on the wave from the start, a leapfrog one
+// lane-per-particle up to cap_N and then on the wave -- so the results are
+// those of the other kernels bit for bit.  Its key stays the
 // sentinel the bulk kernel wrote; the merge of the next step reads r.
 template <bool FAST, typename S>
 __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_t *__restrict__ def_pos,
+                                                       const uint32_t *__restrict__ def_src,
                                                        const uint32_t *__restrict__ count)
 {
     const uint32_t j = blockIdx.x;
     if (j >= *count)
         return;
-    const int64_t i = def_pos[j];
-    double r, v, ell;
-    load_particle<true, S>(a, i, r, v, ell);
+    const int64_t i = def_pos[j], src = def_src[j];
+    const double r = (double)static_cast<const S *>(a.r0)[src];
+    const double v = (double)static_cast<const S *>(a.v0)[src];
+    const double ell = (double)static_cast<const S *>(a.l0)[src];
     const RankMass rm = rank_mass(a, i);
     const Phys ph = make_phys(a, i, rm, ell);
     Budget bud{a.budget, false};
@@ -441,8 +446,8 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
                : launch_typed<double>(a, gather, form, stream, prof);
 }
 
-hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *count,
-                       hipStream_t stream)
+hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *def_src,
+                       const uint32_t *count, hipStream_t stream)
 {
     if (a.n <= 0)
         return hipSuccess;
@@ -450,14 +455,15 @@ hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *de
     const dim3 grid((unsigned)kSlowCap), block(64);
     if (f32) {
         if (fast)
-            hipLaunchKernelGGL((k_integrate_late<true, float>), grid, block, 0, stream, a, def_pos, count);
+            hipLaunchKernelGGL((k_integrate_late<true, float>), grid, block, 0, stream, a, def_pos, def_src, count);
         else
-            hipLaunchKernelGGL((k_integrate_late<false, float>), grid, block, 0, stream, a, def_pos, count);
+            hipLaunchKernelGGL((k_integrate_late<false, float>), grid, block, 0, stream, a, def_pos, def_src, count);
     } else {
         if (fast)
-            hipLaunchKernelGGL((k_integrate_late<true, double>), grid, block, 0, stream, a, def_pos, count);
+            hipLaunchKernelGGL((k_integrate_late<true, double>), grid, block, 0, stream, a, def_pos, def_src, count);
         else
-            hipLaunchKernelGGL((k_integrate_late<false, double>), grid, block, 0, stream, a, def_pos, count);
+            hipLaunchKernelGGL((k_integrate_late<false, double>), grid, block, 0, stream, a, def_pos, def_src,
+                               count);
     }
     return hipGetLastError();
 }

@@ -50,14 +50,15 @@ struct StepArgs {
     // work_stride slots: row 0 plain or bulk, 1 deep, 2 late (side stream)
     unsigned long long *work;
     int64_t work_stride;
-    // overlapped run (else null): every slow result -- flagged NS_LVL_NOCONV
-    // or at level kSlowLevel or deeper -- appends its rank (the late kernel:
-    // its slot | kDeferBit) to slow_list, up to kSlowCap entries (slow_count
-    // keeps counting past them).  In that mode a perm entry with kDeferBit
+    // overlapped run (else null): every slow result (slow_record) appends
+    // its rank (the late kernel: its slot | kDeferBit) to slow_list and its
+    // level record to slow_rec, up to kSlowCap entries (slow_count keeps
+    // counting past them).  In that mode a perm entry with kDeferBit
     // set marks a rank that the late kernel integrates on the side stream:
     // the bulk kernel only carries its ell and id and writes the key sentinel
     // (all ones, above every radius key).
     uint32_t *slow_list;
+    uint32_t *slow_rec;
     uint32_t *slow_count;
 };
 
@@ -66,14 +67,18 @@ constexpr int kQueueCounters = 3;
 constexpr uint32_t kDeferBit = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 constexpr int kSlowCap = 1024;
-// A chain that reached level 8 (2^9 or more sequential micro-steps) or hit
-// the cap is likely to be long again in the next step: a particle plunging
-// towards the centre climbs through the deep leapfrog levels before it turns
-// into a capped Levi-Civita chain.
-constexpr int kSlowLevel = 8;
-__host__ __device__ inline bool slow_record(int rec)
+// A particle is likely to need a long chain in the next step when its chain
+// hit the cap or reached level 6 (2^7 or more sequential micro-steps) -- a
+// particle plunging towards the centre climbs through the deep leapfrog
+// levels before it turns into a capped Levi-Civita chain -- or when it sits
+// among the innermost kCoreRanks ranks, where the plunges end (on the
+// Plummer benchmark these catch every chain of level 12 or more, or capped,
+// of 200 steps).
+constexpr int kSlowLevel = 6;
+constexpr int64_t kCoreRanks = 64;
+__host__ __device__ inline bool slow_record(int rec, int64_t rank)
 {
-    return (rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel;
+    return (rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel || rank < kCoreRanks;
 }
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
@@ -125,22 +130,31 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
 struct OverlapBufs {
     uint32_t *perm;      // n: the merged order, kDeferBit on deferred ranks
     uint32_t *slow;      // kSlowCap: StepArgs::slow_list
-    uint32_t *def_pos;   // kSlowCap: deferred ranks
+    uint32_t *slow_rec;  // kSlowCap: the level record of each slow-list entry
+    uint32_t *def_pos;   // kSlowCap: deferred ranks, costliest first
+    uint32_t *def_src;   // kSlowCap: their positions in the previous order
     uint32_t *ins_lo;    // kSlowCap: insertion points of the late ranks, ascending
+    uint32_t *found;     // 2 kSlowCap: search results of the plan
     uint32_t *ctl;       // 4 words
 };
 // Clears the control words; with lvl (the level records of the current
 // order) the slow list is seeded from them, else left empty.
 hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hipStream_t stream);
 // After the sort of r (state type S, keys of deferred ranks = sentinel) into
-// sorted: the merged order in o.perm, the deferred ranks of the coming step
-// (the previous step's slow list at their new ranks) in def_pos / ctl[1].
-hipError_t overlap_merge(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
-                         hipStream_t stream);
-// The late kernel: every rank of def_pos integrated on one wave (a.keys
-// unused), on the side stream.
-hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *count,
-                       hipStream_t stream);
+// sorted, in two launches.  overlap_plan: the deferred ranks of the coming
+// step (the previous step's slow list at their new ranks) in def_pos /
+// def_src / ctl[1], and the late ranks' merged positions in perm -- all the
+// late kernel reads.  overlap_place: the early elements at their merged
+// positions in perm and the deferred ranks flagged -- what the bulk kernel
+// reads.
+hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
+                        hipStream_t stream);
+hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, int64_t n, hipStream_t stream);
+// The late kernel: rank def_pos[j], gathered from position def_src[j] of the
+// previous order, integrated on one wave per j < *count (a.keys unused), on
+// the side stream.
+hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *def_src,
+                       const uint32_t *count, hipStream_t stream);
 // Radius keys of the deferred ranks of the last step (replacing the sentinel).
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, hipStream_t stream);
 template <typename S>

@@ -15,12 +15,10 @@
 //     merged order is then the stable sort of all the keys, bit for bit;
 //   * an early element at sorted position e moves up by the number of late
 //     particles inserted at or before e.
-// Which particles are deferred: those whose previous step was slow -- ended
-// flagged NS_LVL_NOCONV (the chains that hit the cap, which persist from step
-// to step) or reached level kSlowLevel (a particle plunging towards the
-// centre climbs through the deep leapfrog levels before its chain is
-// capped) -- found through the slow list the integrate kernels append to.
-// The choice affects only the time, never the results.
+// Which particles are deferred: those whose previous step was slow
+// (slow_record: capped, deep, or among the innermost ranks; the capped chains
+// persist from step to step), found through the slow list the integrate
+// kernels append to.  The choice affects only the time, never the results.
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
@@ -31,6 +29,7 @@ namespace ns {
 namespace {
 
 constexpr int kPlanThreads = 1024;  // == kSlowCap: one thread per entry
+constexpr int kSearchThreads = 256;
 constexpr int kShiftThreads = 256;
 constexpr int kShiftItems = 4;
 static_assert(kPlanThreads == kSlowCap, "one plan thread per slow-list entry");
@@ -46,10 +45,12 @@ __global__ void k_overlap_reset(uint32_t *__restrict__ ctl)
 __global__ void k_overlap_seed(OverlapBufs o, const uint8_t *__restrict__ lvl, uint32_t n)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n && slow_record(lvl[t])) {
+    if (t < n && slow_record(lvl[t], t)) {
         const uint32_t k = atomicAdd(o.ctl, 1u);
-        if (k < (uint32_t)kSlowCap)
+        if (k < (uint32_t)kSlowCap) {
             o.slow[k] = t;
+            o.slow_rec[k] = lvl[t];
+        }
     }
 }
 
@@ -59,88 +60,157 @@ __device__ inline typename KeyOf<S>::type key_at(const S *__restrict__ r, int64_
     return radius_key(r[p]);
 }
 
-// Early elements of the sort (positions [0, E)) strictly below (k, t) in
-// (key, old rank) order: the sort output is ascending in that order.
+// Whether the early element at sorted position e is below (k, t) in (key,
+// old rank) order.
 template <typename S>
-__device__ inline uint32_t count_below(const uint32_t *__restrict__ sorted, const S *__restrict__ r, uint32_t E,
-                                       typename KeyOf<S>::type k, uint32_t t)
+__device__ inline bool below(const uint32_t *__restrict__ sorted, const S *__restrict__ r, uint32_t e,
+                             typename KeyOf<S>::type k, uint32_t t)
 {
-    uint32_t lo = 0, hi = E;
-    while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        const uint32_t p = sorted[mid];
-        const typename KeyOf<S>::type km = key_at(r, p);
-        if (km < k || (km == k && p < t))
-            lo = mid + 1;
-        else
-            hi = mid;
+    const uint32_t p = sorted[e];
+    const typename KeyOf<S>::type km = key_at(r, p);
+    return km < k || (km == k && p < t);
+}
+
+// Early elements (sorted positions [0, E)) below (k, t), found by one wave:
+// each round its 64 lanes probe 64 evenly spaced positions of the range that
+// holds the answer (the probes below form a prefix, being sorted), so that
+// 1.2e7 elements take four rounds of two dependent loads.  Wave-uniform.
+template <typename S>
+__device__ inline uint32_t wave_count_below(const uint32_t *__restrict__ sorted, const S *__restrict__ r, uint32_t E,
+                                            typename KeyOf<S>::type k, uint32_t t)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t lo = 0, hi = E;  // the count lies in [lo, hi]
+    while (hi > lo) {
+        const uint32_t stride = (hi - lo + 63) / 64;
+        const uint32_t p = lo + lane * stride;
+        const bool b = p < hi && below(sorted, r, p, k, t);
+        const uint32_t nb = (uint32_t)__popcll(__ballot(b));
+        if (nb == 0) {
+            hi = lo;
+        } else {  // between the last probe below and the next probe (or hi)
+            const uint32_t next = lo + nb * stride;
+            lo = lo + (nb - 1) * stride + 1;
+            hi = next < hi ? next : hi;
+        }
     }
     return lo;
 }
 
-// One block.  ctl[1] late particles (old ranks def_pos[j]) were sorted last;
-// the previous step's slow list (ctl[0] entries) names the ranks to defer in
-// the coming step.  Writes the late particles' merged positions into perm,
-// their insertion points (ascending) into ins_lo, the coming step's deferred
-// ranks into def_pos / ctl[1] and the late count into ctl[2]; clears ctl[0].
+// One wave per query: wave w < kSlowCap is late slot w (its insertion
+// point), wave kSlowCap + q is slow-list entry q when it is an early element
+// (its sorted position).  Results in found[w].
 template <typename S>
-__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, const uint32_t *__restrict__ sorted,
-                                                               const S *__restrict__ r, uint32_t n)
+__global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o, const uint32_t *__restrict__ sorted,
+                                                                   const S *__restrict__ r, uint32_t n)
+{
+    const uint32_t w = blockIdx.x * (kSearchThreads / 64) + (threadIdx.x >> 6);
+    uint32_t K = o.ctl[1];
+    if (K > (uint32_t)kSlowCap)
+        K = kSlowCap;
+    if (K > n)
+        K = n;
+    uint32_t m = o.ctl[0];
+    if (m > (uint32_t)kSlowCap)
+        m = kSlowCap;
+    uint32_t t;
+    if (w < (uint32_t)kSlowCap) {
+        if (w >= K)
+            return;
+        t = o.def_pos[w];
+    } else {
+        const uint32_t q = w - (uint32_t)kSlowCap;
+        if (q >= m)
+            return;
+        t = o.slow[q];
+        if (t & kDeferBit)
+            return;
+    }
+    const uint32_t c = wave_count_below(sorted, r, n - K, key_at(r, t), t);
+    if ((threadIdx.x & 63) == 0)
+        o.found[w] = c;
+}
+
+// Cost order of the deferred ranks: capped chains first, then by level.
+__device__ inline uint32_t slow_cost(uint32_t rec)
+{
+    return (rec & NS_LVL_NOCONV) ? 64u : (rec & NS_LVL_MASK);
+}
+
+// One block, after k_overlap_search.  ctl[1] late particles (old ranks
+// def_pos[j]) were sorted last; the previous step's slow list (ctl[0]
+// entries) names the ranks to defer in the coming step.  Writes the late particles' merged positions into perm,
+// their insertion points (ascending) into ins_lo, the coming step's deferred
+// ranks (costliest first) and their old positions into def_pos / def_src,
+// their count into ctl[1] and the late count into ctl[2]; clears ctl[0].
+template <typename S>
+__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, const S *__restrict__ r, uint32_t n)
 {
     using KeyT = typename KeyOf<S>::type;
     __shared__ KeyT s_key[kSlowCap];
     __shared__ uint32_t s_old[kSlowCap];
     __shared__ uint32_t s_pos[kSlowCap];   // merged position of late slot j
     __shared__ uint32_t s_lo[kSlowCap];    // insertion points, ascending
+    __shared__ uint32_t s_cost[kSlowCap];  // cost of slow entry q
     const uint32_t tid = threadIdx.x;
     uint32_t K = o.ctl[1];
     if (K > (uint32_t)kSlowCap)
         K = kSlowCap;
     if (K > n)
         K = n;
-    const uint32_t E = n - K;
     uint32_t m = o.ctl[0];
     if (m > (uint32_t)kSlowCap)
         m = kSlowCap;
+    uint32_t v = 0;
     if (tid < K) {
         const uint32_t t = o.def_pos[tid];
         s_old[tid] = t;
         s_key[tid] = key_at(r, t);
     }
+    if (tid < m) {
+        v = o.slow[tid];
+        s_cost[tid] = slow_cost(o.slow_rec[tid]);
+    }
     __syncthreads();
-    if (tid < K) {
-        const KeyT k = s_key[tid];
-        const uint32_t t = s_old[tid];
-        const uint32_t lo = count_below(sorted, r, E, k, t);
+    const bool late_q = tid < K, early_q = tid < m && !(v & kDeferBit);
+    const KeyT k1 = late_q ? s_key[tid] : KeyT(0);
+    const uint32_t t1 = late_q ? s_old[tid] : 0u;
+    const uint32_t b1 = late_q ? o.found[tid] : 0u;                  // insertion point
+    const uint32_t b2 = early_q ? o.found[kSlowCap + tid] : 0u;      // sorted position
+    if (late_q) {
+        const uint32_t lo = b1;
         uint32_t rank = 0;  // late particles below this one
         for (uint32_t q = 0; q < K; q++)
-            rank += (s_key[q] < k || (s_key[q] == k && s_old[q] < t)) ? 1u : 0u;
+            rank += (s_key[q] < k1 || (s_key[q] == k1 && s_old[q] < t1)) ? 1u : 0u;
         s_pos[tid] = lo + rank;
         s_lo[rank] = lo;
         o.ins_lo[rank] = lo;
-        o.perm[lo + rank] = t;
+        o.perm[lo + rank] = t1;
     }
     __syncthreads();
     if (tid < m) {
-        const uint32_t v = o.slow[tid];
-        uint32_t pos;
-        if (v & kDeferBit) {
-            pos = s_pos[v & kIndexMask];  // a late particle of this merge
-        } else {
-            // an early element: its sorted position, moved past the late
-            // particles inserted at or before it
-            const uint32_t e = count_below(sorted, r, E, key_at(r, v), v);
-            uint32_t lo = 0, hi = K;  // late particles with insertion point <= e
+        uint32_t pos, src;
+        if (v & kDeferBit) {  // a late particle of this merge
+            pos = s_pos[v & kIndexMask];
+            src = s_old[v & kIndexMask];
+        } else {  // an early element, moved past the late particles inserted at or before it
+            uint32_t lo = 0, hi = K;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (s_lo[mid] <= e)
+                if (s_lo[mid] <= b2)
                     lo = mid + 1;
                 else
                     hi = mid;
             }
-            pos = e + lo;
+            pos = b2 + lo;
+            src = v;
         }
-        o.def_pos[tid] = pos;
+        const uint32_t c = s_cost[tid];
+        uint32_t slot = 0;  // costliest first, ties in list order
+        for (uint32_t q = 0; q < m; q++)
+            slot += (s_cost[q] > c || (s_cost[q] == c && q < tid)) ? 1u : 0u;
+        o.def_pos[slot] = pos;
+        o.def_src[slot] = src;
     }
     __syncthreads();
     if (tid == 0) {
@@ -199,18 +269,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_fix_keys(OverlapBufs o
 }
 
 template <typename S>
-hipError_t merge_typed(const OverlapBufs &o, const uint32_t *sorted, const S *r, int64_t n, hipStream_t stream)
+hipError_t plan_typed(const OverlapBufs &o, const uint32_t *sorted, const S *r, int64_t n, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_overlap_plan<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, sorted, r, (uint32_t)n);
+    hipLaunchKernelGGL(k_overlap_search<S>, dim3(2 * kSlowCap / (kSearchThreads / 64)), dim3(kSearchThreads), 0,
+                       stream, o, sorted, r, (uint32_t)n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    const int64_t per_block = (int64_t)kShiftThreads * kShiftItems;
-    hipLaunchKernelGGL(k_overlap_shift, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(kShiftThreads), 0,
-                       stream, o, sorted, (uint32_t)n);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
-    hipLaunchKernelGGL(k_overlap_flag, dim3(1), dim3(kPlanThreads), 0, stream, o);
+    hipLaunchKernelGGL(k_overlap_plan<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, r, (uint32_t)n);
     return hipGetLastError();
 }
 
@@ -227,15 +293,29 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hi
     return hipGetLastError();
 }
 
-hipError_t overlap_merge(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
-                         hipStream_t stream)
+hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
+                        hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
     if (n > (int64_t)kIndexMask)
         return hipErrorInvalidValue;
-    return f32 ? merge_typed<float>(o, sorted, static_cast<const float *>(r), n, stream)
-               : merge_typed<double>(o, sorted, static_cast<const double *>(r), n, stream);
+    return f32 ? plan_typed<float>(o, sorted, static_cast<const float *>(r), n, stream)
+               : plan_typed<double>(o, sorted, static_cast<const double *>(r), n, stream);
+}
+
+hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    const int64_t per_block = (int64_t)kShiftThreads * kShiftItems;
+    hipLaunchKernelGGL(k_overlap_shift, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(kShiftThreads), 0,
+                       stream, o, sorted, (uint32_t)n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_overlap_flag, dim3(1), dim3(kPlanThreads), 0, stream, o);
+    return hipGetLastError();
 }
 
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, hipStream_t stream)

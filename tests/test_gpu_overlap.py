@@ -106,7 +106,8 @@ def test_overlap_counters_and_timing(ns):
     # the late kernel, so only the total is the same
     assert sum(one["evals"].values()) == sum(two["evals"].values())
     assert two["stats"]["integrate_late"][0] == 5
-    assert two["stats"]["overlap_merge"][0] == 5
+    assert two["stats"]["overlap_plan"][0] == 5
+    assert two["stats"]["overlap_place"][0] == 5
     assert "integrate_late" not in one["stats"]
 
 
@@ -199,4 +200,4 @@ def test_profile_kernel_mask(ns):
         ctx.profile_kernels(None)
         ctx.reset_kernel_stats()
         ctx.run(ns.unit_params(), 2)
-        assert {"integrate_bulk", "integrate_deep", "overlap_merge"} <= set(ctx.kernel_stats())
+        assert {"integrate_bulk", "integrate_deep", "overlap_plan", "overlap_place"} <= set(ctx.kernel_stats())
