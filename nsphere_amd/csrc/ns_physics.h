@@ -416,6 +416,9 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
 // g inv2 = hd dtau forceLC(rho); the physical time is accumulated as
 // T = sum (rho + rho')^2, t = t_in + dtau T / 4 at the end.  The closing
 // half-kick of one micro-step and the opening one of the next are one kick.
+// inv2 = (1/rho)^2 from the refined reciprocal of rho: nine operations per
+// micro-step, the same count as the reciprocal of rho^2, but measured 5 %
+// faster on one wave (23.3 against 24.6 ns per micro-step).
 struct LcFast {
     double hd, lh, gh;  // dtau/2, hd dtau ell^2, hd dtau gm
     double rho, u, u_prev, T;
@@ -429,8 +432,8 @@ struct LcFast {
         gh = (ph.idg ? 0.0 : ph.gm) * sc;
         rho = rho_in;
         T = 0.0;
-        const double q = rho * rho;
-        inv2 = fast_rcp(q);
+        const double inv = fast_rcp(rho);
+        inv2 = inv * inv;
         g = fma(lh, inv2, gh);
         u = fma(0.5 * g, inv2, hd * v_in);  // hd (v + hd forceLC(rho))
         u_prev = u;
@@ -438,7 +441,8 @@ struct LcFast {
     __device__ inline void step()
     {
         const double rn = fma(rho, u, rho);
-        inv2 = fast_rcp(rn * rn);
+        const double inv = fast_rcp(rn);
+        inv2 = inv * inv;
         g = fma(lh, inv2, gh);
         const double s = rho + rn;
         T = fma(s, s, T);
