@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o
 }
 
 // Cost order of the deferred ranks: capped chains first, then by level.
+constexpr int kCostBins = 65;
 __device__ inline uint32_t slow_cost(uint32_t rec)
 {
     return (rec & NS_LVL_NOCONV) ? 64u : (rec & NS_LVL_MASK);
@@ -139,19 +140,25 @@ __device__ inline uint32_t slow_cost(uint32_t rec)
 
 // One block, after k_overlap_search.  ctl[1] late particles (old ranks
 // def_pos[j]) were sorted last; the previous step's slow list (ctl[0]
-// entries) names the ranks to defer in the coming step.  Writes the late particles' merged positions into perm,
-// their insertion points (ascending) into ins_lo, the coming step's deferred
-// ranks (costliest first) and their old positions into def_pos / def_src,
-// their count into ctl[1] and the late count into ctl[2]; clears ctl[0].
+// entries) names the ranks to defer in the coming step.  Ranks the late
+// particles among themselves by a bitonic sort on (key, old rank) and writes
+// their merged positions into perm and their insertion points (ascending)
+// into ins_lo; orders the coming step's deferred ranks costliest first by a
+// counting sort on the cost (the order within a cost is free: it decides
+// only which wave runs which particle) and writes them with their old
+// positions into def_pos / def_src; their count into ctl[1], the late count
+// into ctl[2]; clears ctl[0].
 template <typename S>
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, const S *__restrict__ r, uint32_t n)
 {
     using KeyT = typename KeyOf<S>::type;
-    __shared__ KeyT s_key[kSlowCap];
-    __shared__ uint32_t s_old[kSlowCap];
-    __shared__ uint32_t s_pos[kSlowCap];   // merged position of late slot j
-    __shared__ uint32_t s_lo[kSlowCap];    // insertion points, ascending
-    __shared__ uint32_t s_cost[kSlowCap];  // cost of slow entry q
+    __shared__ KeyT s_key[kSlowCap];        // late keys, sorted in place
+    __shared__ uint32_t s_old[kSlowCap];    // their old ranks, sorted alongside
+    __shared__ uint32_t s_slot[kSlowCap];   // their late slots, sorted alongside
+    __shared__ uint32_t s_t[kSlowCap];      // old rank of late slot j
+    __shared__ uint32_t s_pos[kSlowCap];    // merged position of late slot j
+    __shared__ uint32_t s_lo[kSlowCap];     // insertion points, ascending
+    __shared__ uint32_t s_bin[kCostBins];   // cost histogram, then the first slot of each cost
     const uint32_t tid = threadIdx.x;
     uint32_t K = o.ctl[1];
     if (K > (uint32_t)kSlowCap)
@@ -161,54 +168,86 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, co
     uint32_t m = o.ctl[0];
     if (m > (uint32_t)kSlowCap)
         m = kSlowCap;
-    uint32_t v = 0;
+    // late particles, padded above every real (key, rank) to a power of two
+    uint32_t P = 1;
+    while (P < K)
+        P <<= 1;
     if (tid < K) {
         const uint32_t t = o.def_pos[tid];
-        s_old[tid] = t;
         s_key[tid] = key_at(r, t);
+        s_old[tid] = t;
+        s_t[tid] = t;
+    } else {
+        s_key[tid] = ~KeyT(0);
+        s_old[tid] = ~0u;
     }
+    s_slot[tid] = tid;
+    if (tid < (uint32_t)kCostBins)
+        s_bin[tid] = 0;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t l = tid ^ j;
+            if (tid < P && l > tid) {
+                const KeyT ka = s_key[tid], kb = s_key[l];
+                const uint32_t ta = s_old[tid], tb = s_old[l];
+                const bool a_above = ka > kb || (ka == kb && ta > tb);
+                if (a_above == ((tid & k) == 0)) {  // ascending where bit k of tid is clear
+                    s_key[tid] = kb;
+                    s_key[l] = ka;
+                    s_old[tid] = tb;
+                    s_old[l] = ta;
+                    const uint32_t x = s_slot[tid];
+                    s_slot[tid] = s_slot[l];
+                    s_slot[l] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid < K) {  // the tid-th late particle in (key, old rank) order
+        const uint32_t j = s_slot[tid];
+        const uint32_t lo = o.found[j];
+        s_pos[j] = lo + tid;
+        s_lo[tid] = lo;
+        o.ins_lo[tid] = lo;
+        o.perm[lo + tid] = s_old[tid];
+    }
+    uint32_t v = 0, c = 0, within = 0;
     if (tid < m) {
         v = o.slow[tid];
-        s_cost[tid] = slow_cost(o.slow_rec[tid]);
+        c = slow_cost(o.slow_rec[tid]);
+        within = atomicAdd(&s_bin[c], 1u);
     }
     __syncthreads();
-    const bool late_q = tid < K, early_q = tid < m && !(v & kDeferBit);
-    const KeyT k1 = late_q ? s_key[tid] : KeyT(0);
-    const uint32_t t1 = late_q ? s_old[tid] : 0u;
-    const uint32_t b1 = late_q ? o.found[tid] : 0u;                  // insertion point
-    const uint32_t b2 = early_q ? o.found[kSlowCap + tid] : 0u;      // sorted position
-    if (late_q) {
-        const uint32_t lo = b1;
-        uint32_t rank = 0;  // late particles below this one
-        for (uint32_t q = 0; q < K; q++)
-            rank += (s_key[q] < k1 || (s_key[q] == k1 && s_old[q] < t1)) ? 1u : 0u;
-        s_pos[tid] = lo + rank;
-        s_lo[rank] = lo;
-        o.ins_lo[rank] = lo;
-        o.perm[lo + rank] = t1;
+    if (tid == 0) {  // first slot of each cost, costliest first
+        uint32_t acc = 0;
+        for (int b = kCostBins - 1; b >= 0; b--) {
+            const uint32_t x = s_bin[b];
+            s_bin[b] = acc;
+            acc += x;
+        }
     }
     __syncthreads();
     if (tid < m) {
         uint32_t pos, src;
         if (v & kDeferBit) {  // a late particle of this merge
             pos = s_pos[v & kIndexMask];
-            src = s_old[v & kIndexMask];
+            src = s_t[v & kIndexMask];
         } else {  // an early element, moved past the late particles inserted at or before it
+            const uint32_t e = o.found[kSlowCap + tid];
             uint32_t lo = 0, hi = K;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (s_lo[mid] <= b2)
+                if (s_lo[mid] <= e)
                     lo = mid + 1;
                 else
                     hi = mid;
             }
-            pos = b2 + lo;
+            pos = e + lo;
             src = v;
         }
-        const uint32_t c = s_cost[tid];
-        uint32_t slot = 0;  // costliest first, ties in list order
-        for (uint32_t q = 0; q < m; q++)
-            slot += (s_cost[q] > c || (s_cost[q] == c && q < tid)) ? 1u : 0u;
+        const uint32_t slot = s_bin[c] + within;
         o.def_pos[slot] = pos;
         o.def_src[slot] = src;
     }
