@@ -61,7 +61,7 @@ struct ns_gpu_ctx {
     int overlap = 1;
     ns::OverlapBufs ov = {};
     hipStream_t side = nullptr;
-    hipEvent_t ev_merged = nullptr, ev_late = nullptr;
+    hipEvent_t ev_sorted = nullptr, ev_merged = nullptr, ev_late = nullptr;
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool levels_valid = false;
@@ -259,7 +259,7 @@ void free_all(ns_gpu_ctx *c)
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
-    for (hipEvent_t ev : {c->ev_merged, c->ev_late})
+    for (hipEvent_t ev : {c->ev_sorted, c->ev_merged, c->ev_late})
         if (ev)
             (void)hipEventDestroy(ev);
     if (c->side)
@@ -282,7 +282,7 @@ int ensure_overlap(ns_gpu_ctx *c)
         e = dalloc(&c->ov.ctl, 4);
     if (e == hipSuccess && !c->side)
         e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    for (hipEvent_t *ev : {&c->ev_merged, &c->ev_late})
+    for (hipEvent_t *ev : {&c->ev_sorted, &c->ev_merged, &c->ev_late})
         if (e == hipSuccess && !*ev)
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess)
@@ -542,23 +542,23 @@ bool overlap_applies(const ns_gpu_ctx *c, int64_t nsteps)
            !c->has_mass && !c->mlp && c->n <= (int64_t)ns::kIndexMask;
 }
 
-// One step of an overlapped run: the sort (late keys = sentinel), the wait
-// for the previous step's late kernel, the merge, then the late kernel on the
-// second stream beside the bulk and deep kernels on the context's stream.
-int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, bool first)
+// One step of an overlapped run.  Context's stream: the sort (late keys at
+// the sentinel).  Second stream, after that sort and after the previous
+// step's late kernel: the plan, then this step's late kernel -- one chain
+// hands over to the next without a round trip through the context's stream.
+// Context's stream, after the plan: the placement of the early elements and
+// the bulk and deep kernels.
+int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
 {
     if (int rc = sort_perm(c))
         return rc;
-    if (!first)
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_late, 0));
+    HIP_TRY(hipEventRecord(c->ev_sorted, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_sorted, 0));
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
-    // the plan is all the late kernel needs; the placement of the early
-    // elements runs beside it, before the bulk kernel
-    c->prof.begin(ns::K_MERGE);
-    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->n, c->stream));
-    c->prof.end(ns::K_MERGE);
-    HIP_TRY(hipEventRecord(c->ev_merged, c->stream));
-    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_merged, 0));
+    c->prof.begin(ns::K_MERGE, c->side);
+    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->n, c->side));
+    c->prof.end(ns::K_MERGE, c->side);
+    HIP_TRY(hipEventRecord(c->ev_merged, c->side));
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
     a.slow_list = c->ov.slow;
     a.slow_rec = c->ov.slow_rec;
@@ -569,6 +569,7 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, bool first)
     HIP_TRY(ns::launch_late(late, c->integrator, c->f32, c->ov.def_pos, c->ov.def_src, c->ov.ctl + 1, c->side));
     c->prof.end(ns::K_INT_LATE, c->side);
     HIP_TRY(hipEventRecord(c->ev_late, c->side));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_merged, 0));
     c->prof.begin(ns::K_PLACE);
     HIP_TRY(ns::overlap_place(c->ov, c->perm, c->n, c->stream));
     c->prof.end(ns::K_PLACE);
@@ -591,7 +592,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
     int rc = NS_OK;
     int64_t done = 0;
     for (; done < nsteps && rc == NS_OK; done++)
-        rc = overlapped_step(c, p, done == 0);
+        rc = overlapped_step(c, p);
     if (done > 0) {
         // join even after a failed step, so the second stream is never left
         // running past the call
