@@ -91,7 +91,8 @@ enum {
                               2 binned with the fast arithmetic form (one
                                  reciprocal per force, fused multiply-adds)   */
     NS_OPT_SORT_TILE = 6,  /* radix tile: (threads << 16) | keys per thread,
-                              one of 256x16, 256x24, 512x12 (default), 512x16 */
+                              one of 256x16, 256x24, 512x8, 512x12 (default),
+                              512x16, 1024x6                                 */
     NS_OPT_SORT_ALG = 7,   /* radix pass: 0 reduce-then-scan (per pass: tile
                               digit counts, their scan, scatter; default),
                               1 onesweep (one scatter kernel per pass with

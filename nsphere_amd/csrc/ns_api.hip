@@ -797,7 +797,7 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         cfg.items = (int)(value & 0xFFFF);
         if (!ns::sort_config_supported(cfg))
             return set_error(NS_ERR_ARG, "NS_OPT_SORT_TILE: (threads << 16 | items) with threads x items in "
-                                         "{256x16, 256x24, 512x12, 512x16}");
+                                         "{256x16, 256x24, 512x8, 512x12, 512x16, 1024x6}");
         c->sw.config = cfg;
         return NS_OK;
     }
@@ -1553,11 +1553,12 @@ int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_r
     ns_tune_config best = orig;
     best.ms_per_step = 1e300;
     // knob 1: pass algorithm x tile shape
-    static const int kTiles[4][2] = {{256, 16}, {256, 24}, {512, 12}, {512, 16}};
+    static const int kTiles[][2] = {{256, 16}, {256, 24}, {512, 8}, {512, 12}, {512, 16}, {1024, 6}};
+    constexpr int kNumTiles = (int)(sizeof kTiles / sizeof kTiles[0]);
     for (int alg = 0; alg < 2 && !rc; alg++) {
         if (alg == ns::kSortOnesweep && c->nseg > 0)
             continue;  // an ensemble sorts by segments
-        for (int k = 0; k < 4 && !rc; k++) {
+        for (int k = 0; k < kNumTiles && !rc; k++) {
             ns_tune_config t = best;
             t.sort_alg = alg;
             t.sort_threads = kTiles[k][0];

@@ -670,6 +670,10 @@ hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, 
         return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
     if (c.threads == 512 && c.items == 12)
         return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+    if (c.threads == 512 && c.items == 8)
+        return launch_pass_cfg<KeyT, 512, 8>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+    if (c.threads == 1024 && c.items == 6)
+        return launch_pass_cfg<KeyT, 1024, 6>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
     return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
 }
 
@@ -803,7 +807,8 @@ bool sort_config_supported(SortConfig c)
     if (c.alg != kSortReduceScan && c.alg != kSortOnesweep)
         return false;
     return (c.threads == 256 && (c.items == 16 || c.items == 24)) ||
-           (c.threads == 512 && (c.items == 12 || c.items == 16));
+           (c.threads == 512 && (c.items == 8 || c.items == 12 || c.items == 16)) ||
+           (c.threads == 1024 && c.items == 6);
 }
 
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream)

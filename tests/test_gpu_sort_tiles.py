@@ -7,7 +7,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-TILES = [(256, 16), (256, 24), (512, 12), (512, 16)]
+TILES = [(256, 16), (256, 24), (512, 8), (512, 12), (512, 16), (1024, 6)]
 
 
 @pytest.mark.parametrize("alg", [0, 1])
