@@ -128,9 +128,9 @@ enum {
                               stream integrates the rest and sorts the next
                               step's radii; a merge then puts them at their
                               sorted ranks.  0: every kernel on the context's
-                              stream.  Same result.  Not taken with graphs,
-                              tiebreak by id, ensembles, per-particle masses
-                              or learned binning.                            */
+                              stream.  Same result; with halos, per halo.
+                              Not taken with graphs, tiebreak by id,
+                              per-particle masses or learned binning.        */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;

@@ -254,8 +254,8 @@ void free_all(ns_gpu_ctx *c)
                     c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
-                    c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.slow_rec, c->ov.def_pos, c->ov.def_src, c->ov.ins_lo,
-                    c->ov.found, c->ov.ctl};
+                    c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.slow_rec, c->ov.sel, c->ov.def_pos, c->ov.def_src, c->ov.late_h,
+                    c->ov.late_t, c->ov.late_slot, c->ov.ins, c->ov.found, c->ov.ctl};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -272,14 +272,21 @@ int ensure_overlap(ns_gpu_ctx *c)
     hipError_t e = hipSuccess;
     if (!c->ov.perm)
         e = dalloc(&c->ov.perm, c->n);
-    uint32_t **small[] = {&c->ov.slow, &c->ov.slow_rec, &c->ov.def_pos, &c->ov.def_src, &c->ov.ins_lo};
+    uint32_t **lists[] = {&c->ov.slow, &c->ov.slow_rec};
+    for (uint32_t **b : lists)
+        if (e == hipSuccess && !*b)
+            e = dalloc(b, ns::kSlowList + ns::kCoreList);
+    uint32_t **small[] = {&c->ov.sel,    &c->ov.def_pos, &c->ov.def_src,
+                          &c->ov.late_h, &c->ov.late_t,  &c->ov.late_slot};
     for (uint32_t **b : small)
         if (e == hipSuccess && !*b)
             e = dalloc(b, ns::kSlowCap);
+    if (e == hipSuccess && !c->ov.ins)
+        e = dalloc(&c->ov.ins, ns::kSlowCap);
     if (e == hipSuccess && !c->ov.found)
         e = dalloc(&c->ov.found, 2 * ns::kSlowCap);
     if (e == hipSuccess && !c->ov.ctl)
-        e = dalloc(&c->ov.ctl, 4);
+        e = dalloc(&c->ov.ctl, 8);
     if (e == hipSuccess && !c->side)
         e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     for (hipEvent_t *ev : {&c->ev_sorted, &c->ev_merged, &c->ev_late})
@@ -538,7 +545,7 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
 // Whether ns_gpu_run overlaps the late particles with the next sort.
 bool overlap_applies(const ns_gpu_ctx *c, int64_t nsteps)
 {
-    return c->overlap && nsteps >= 2 && c->integrator != ns::FORM_PLAIN && !c->tiebreak && c->nseg == 0 &&
+    return c->overlap && nsteps >= 2 && c->integrator != ns::FORM_PLAIN && !c->tiebreak &&
            !c->has_mass && !c->mlp && c->n <= (int64_t)ns::kIndexMask;
 }
 
@@ -556,13 +563,13 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
     HIP_TRY(hipStreamWaitEvent(c->side, c->ev_sorted, 0));
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
     c->prof.begin(ns::K_MERGE, c->side);
-    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->n, c->side));
+    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->seg_off, c->nseg, c->n, c->side));
     c->prof.end(ns::K_MERGE, c->side);
     HIP_TRY(hipEventRecord(c->ev_merged, c->side));
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
     a.slow_list = c->ov.slow;
     a.slow_rec = c->ov.slow_rec;
-    a.slow_count = c->ov.ctl;  // ctl[0]
+    a.slow_count = c->ov.ctl;  // ctl[0] and ctl[3]
     ns::StepArgs late = a;
     late.keys = nullptr;  // the sentinel stays until the next merge
     c->prof.begin(ns::K_INT_LATE, c->side);
@@ -571,7 +578,7 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
     HIP_TRY(hipEventRecord(c->ev_late, c->side));
     HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_merged, 0));
     c->prof.begin(ns::K_PLACE);
-    HIP_TRY(ns::overlap_place(c->ov, c->perm, c->n, c->stream));
+    HIP_TRY(ns::overlap_place(c->ov, c->perm, c->seg_off, c->nseg, c->n, c->stream));
     c->prof.end(ns::K_PLACE);
     HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
     c->cur ^= 1;
@@ -588,7 +595,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
         return rc;
     // the slow list of the last step, from its level records (whatever form
     // ran it): the first step defers the chains that step found slow
-    HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->n, c->stream));
+    HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->seg_off, c->nseg, c->n, c->stream));
     int rc = NS_OK;
     int64_t done = 0;
     for (; done < nsteps && rc == NS_OK; done++)

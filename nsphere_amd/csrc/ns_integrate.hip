@@ -84,10 +84,10 @@ __device__ inline uint32_t wave_ticket(uint32_t *ticket)
 }
 
 // slow_tag: what the slow list records for a slow result (the rank, or the
-// late kernel's slot | kDeferBit).
+// late kernel's slot | kDeferBit); local: the rank within its halo.
 template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit,
-                                    uint32_t slow_tag)
+                                    uint32_t slow_tag, int64_t local)
 {
     const S rs = (S)r;
     static_cast<S *>(a.r1)[i] = rs;
@@ -98,19 +98,15 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
         a.lvl[i] = (uint8_t)rec;
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
-    if (a.slow_list && slow_record(rec, i)) {
-        const uint32_t k = atomicAdd(a.slow_count, 1u);
-        if (k < (uint32_t)kSlowCap) {
-            a.slow_list[k] = slow_tag;
-            a.slow_rec[k] = (uint32_t)rec;
+    const int cls = a.slow_list ? slow_class(rec, local) : 0;
+    if (cls) {
+        const uint32_t k = atomicAdd(a.slow_count + (cls == 2 ? 0 : 3), 1u);
+        if (k < (uint32_t)(cls == 2 ? kSlowList : kCoreList)) {
+            const uint32_t q = (cls == 2 ? 0u : (uint32_t)kSlowList) + k;
+            a.slow_list[q] = slow_tag;
+            a.slow_rec[q] = (uint32_t)rec;
         }
     }
-}
-
-template <typename S>
-__device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit)
-{
-    write_result<S>(a, i, r, v, rec, hit, (uint32_t)i);
 }
 
 // Previous position of rank i (perm entries may carry kDeferBit).
@@ -156,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
             rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
         else
             rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
-        write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
         spent = (unsigned long long)(a.budget - bud.left);
     }
     add_work(a, 0, spent);
@@ -194,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
             if (rec == kEscalate)
                 to_lf = true;
             else
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
             spent = (unsigned long long)(a.budget - bud.left);
         }
     }
@@ -219,13 +215,14 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
         const int64_t i = lc ? a.q_lc[q] : a.q_lf[q - n_lc];
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
-        const Phys ph = make_phys(a, i, ell);
+        const RankMass rm = rank_mass(a, i);
+        const Phys ph = make_phys(a, i, rm, ell);
         Budget bud{a.budget, false};
         double r_new, v_new;
         const int rec = lc ? adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud)
                            : adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
         if (lane == 0) {
-            write_result<S>(a, i, r_new, v_new, rec, bud.hit);
+            write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
             spent += (unsigned long long)(a.budget - bud.left);  // the wave's particle, once
         }
     }
@@ -270,7 +267,7 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
         }
     }
     if ((threadIdx.x & 63) == 0)
-        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i);
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);
 }
 

@@ -50,13 +50,14 @@ struct StepArgs {
     // work_stride slots: row 0 plain or bulk, 1 deep, 2 late (side stream)
     unsigned long long *work;
     int64_t work_stride;
-    // overlapped run (else null): every slow result (slow_record) appends
-    // its rank (the late kernel: its slot | kDeferBit) to slow_list and its
-    // level record to slow_rec, up to kSlowCap entries (slow_count keeps
-    // counting past them).  In that mode a perm entry with kDeferBit
-    // set marks a rank that the late kernel integrates on the side stream:
-    // the bulk kernel only carries its ell and id and writes the key sentinel
-    // (all ones, above every radius key).
+    // overlapped run (else null): every result slow_class ranks appends its
+    // rank (the late kernel: its slot | kDeferBit) and its level record to
+    // slow_list / slow_rec -- class 2 at [0, kSlowList) counted in
+    // slow_count[0], class 1 at [kSlowList, kSlowList + kCoreList) counted
+    // in slow_count[3] (the counts go on past the capacities).  In that mode
+    // a perm entry with kDeferBit set marks a rank that the late kernel
+    // integrates on the side stream: the bulk kernel only carries its ell and
+    // id and writes the key sentinel (all ones, above every radius key).
     uint32_t *slow_list;
     uint32_t *slow_rec;
     uint32_t *slow_count;
@@ -66,19 +67,24 @@ constexpr int kWorkKernels = 3;
 constexpr int kQueueCounters = 3;
 constexpr uint32_t kDeferBit = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
-constexpr int kSlowCap = 1024;
-// A particle is likely to need a long chain in the next step when its chain
-// hit the cap or reached level 6 (2^7 or more sequential micro-steps) -- a
-// particle plunging towards the centre climbs through the deep leapfrog
-// levels before it turns into a capped Levi-Civita chain -- or when it sits
-// among the innermost kCoreRanks ranks, where the plunges end (on the
-// Plummer benchmark these catch every chain of level 12 or more, or capped,
-// of 200 steps).
+constexpr int kSlowCap = 1024;   // particles deferred per step
+constexpr int kSlowList = 4096;  // class-2 candidates kept per step
+constexpr int kCoreList = 1024;  // class-1 candidates kept per step
+// Which particles are likely to need a long chain in the next step.  Class 2:
+// the chain hit the cap or reached level 6 (2^7 or more sequential
+// micro-steps) -- a particle plunging towards the centre climbs through the
+// deep leapfrog levels before it turns into a capped Levi-Civita chain.
+// Class 1: otherwise, the particle sits among the innermost kCoreRanks ranks
+// of its halo, where the plunges end.  On the Plummer benchmark the two
+// catch every chain of level 12 or more, or capped, of 200 steps; the
+// deferred particles are taken class 2 first, costliest first.
 constexpr int kSlowLevel = 6;
 constexpr int64_t kCoreRanks = 64;
-__host__ __device__ inline bool slow_record(int rec, int64_t rank)
+__host__ __device__ inline int slow_class(int rec, int64_t local_rank)
 {
-    return (rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel || rank < kCoreRanks;
+    if ((rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel)
+        return 2;
+    return local_rank < kCoreRanks ? 1 : 0;
 }
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
@@ -124,22 +130,28 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
                             Profiler *prof);
 
 // Overlapped run (ns_overlap.hip).  Device control words of ctl:
-//   [0] slow_count (slow results of this step, all kernels)
+//   [0], [3] class-2 and class-1 results of this step (all kernels)
 //   [1] deferred ranks of the coming step (entries of def_pos)
 //   [2] ranks deferred in the step just sorted (sorted last, by key sentinel)
+//   [4] slow-list entries selected for deferral (entries of sel)
 struct OverlapBufs {
     uint32_t *perm;      // n: the merged order, kDeferBit on deferred ranks
-    uint32_t *slow;      // kSlowCap: StepArgs::slow_list
-    uint32_t *slow_rec;  // kSlowCap: the level record of each slow-list entry
+    uint32_t *slow;      // kSlowList + kCoreList: StepArgs::slow_list
+    uint32_t *slow_rec;  // kSlowList + kCoreList: the level record of each entry
+    uint32_t *sel;       // kSlowCap: the slow-list entries deferred, costliest first
     uint32_t *def_pos;   // kSlowCap: deferred ranks, costliest first
     uint32_t *def_src;   // kSlowCap: their positions in the previous order
-    uint32_t *ins_lo;    // kSlowCap: insertion points of the late ranks, ascending
+    uint32_t *late_h, *late_t, *late_slot;  // kSlowCap each: the late particles in (halo, key, rank) order
+    uint64_t *ins;       // kSlowCap: (halo << 32 | insertion point) of the late particles, ascending
     uint32_t *found;     // 2 kSlowCap: search results of the plan
-    uint32_t *ctl;       // 4 words
+    uint32_t *ctl;       // 8 words
 };
+// Halos (seg_off, nseg > 0): ranks [seg_off[h], seg_off[h + 1]) are halo h,
+// sorted as a segment of their own; nseg == 0: one array.
 // Clears the control words; with lvl (the level records of the current
 // order) the slow list is seeded from them, else left empty.
-hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hipStream_t stream);
+hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream);
 // After the sort of r (state type S, keys of deferred ranks = sentinel) into
 // sorted, in two launches.  overlap_plan: the deferred ranks of the coming
 // step (the previous step's slow list at their new ranks) in def_pos /
@@ -147,9 +159,10 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hi
 // late kernel reads.  overlap_place: the early elements at their merged
 // positions in perm and the deferred ranks flagged -- what the bulk kernel
 // reads.
-hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
-                        hipStream_t stream);
-hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, int64_t n, hipStream_t stream);
+hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, const int64_t *seg_off,
+                        int nseg, int64_t n, hipStream_t stream);
+hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream);
 // The late kernel: rank def_pos[j], gathered from position def_src[j] of the
 // previous order, integrated on one wave per j < *count (a.keys unused), on
 // the side stream.

@@ -6,19 +6,22 @@
 // An overlapped run integrates those particles on a side stream (the late
 // kernel) and meanwhile sorts the next step's radii on the main stream, with
 // the late particles' keys replaced by a sentinel above every radius key, so
-// that the sort leaves them last, in rank order (the sort is stable).  Once
-// the late kernel is done, the merge inserts them where a stable sort of the
-// true keys puts them:
-//   * a late particle at old rank t with key k goes after every early element
-//     whose (key, old rank) is below (k, t) -- lo, a binary search over the
-//     sorted early elements -- and after the late particles below it; the
-//     merged order is then the stable sort of all the keys, bit for bit;
-//   * an early element at sorted position e moves up by the number of late
-//     particles inserted at or before e.
+// that the sort leaves them last in their halo's segment (the whole array
+// when there are no halos), in rank order (the sort is stable).  Once the
+// late kernel is done, the merge inserts them where a stable sort of the true
+// keys puts them, halo by halo:
+//   * a late particle of halo h at old rank t with key k goes after every
+//     early element of h whose (key, old rank) is below (k, t) -- lo, a
+//     search over h's sorted early elements -- and after the late particles
+//     of h below it; the merged order is then the stable sort of all the
+//     keys, bit for bit;
+//   * an early element of h at local sorted position e moves up by the number
+//     of late particles of h inserted at or before e.
 // Which particles are deferred: those whose previous step was slow
-// (slow_record: capped, deep, or among the innermost ranks; the capped chains
-// persist from step to step), found through the slow list the integrate
-// kernels append to.  The choice affects only the time, never the results.
+// (slow_class: capped or deep, else among the innermost ranks of their halo;
+// the capped chains persist from step to step), found through the slow list
+// the integrate kernels append to.  The choice affects only the time, never
+// the results.
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
@@ -32,25 +35,88 @@ constexpr int kPlanThreads = 1024;  // == kSlowCap: one thread per entry
 constexpr int kSearchThreads = 256;
 constexpr int kShiftThreads = 256;
 constexpr int kShiftItems = 4;
+constexpr int kShiftTile = kShiftThreads * kShiftItems;
+constexpr int kShiftHalos = 64;  // halo bounds of a shift tile held in LDS
 static_assert(kPlanThreads == kSlowCap, "one plan thread per slow-list entry");
+
+// The halos as the merge sees them: halo h holds positions
+// [start(h), start(h + 1)); without halos, one halo of all n.
+struct Halos {
+    const int64_t *off;  // nseg + 1 offsets, or null
+    int32_t nseg;
+    uint32_t n;
+    __device__ inline uint32_t start(uint32_t h) const { return off ? (uint32_t)off[h] : (h ? n : 0u); }
+    __device__ inline uint32_t size(uint32_t h) const { return start(h + 1) - start(h); }
+    __device__ inline uint32_t count() const { return off ? (uint32_t)nseg : 1u; }
+    // halo of position p < n
+    __device__ inline uint32_t of(uint32_t p) const
+    {
+        if (!off)
+            return 0;
+        uint32_t lo = 0, hi = (uint32_t)nseg;  // off[lo] <= p < off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((uint32_t)off[mid] <= p)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+};
+
+__device__ inline uint64_t hl_pack(uint32_t h, uint32_t lo) { return ((uint64_t)h << 32) | lo; }
+
+// Entries of the ascending array a[0, m) that are <= x (upper bound).
+__device__ inline uint32_t count_le(const uint64_t *a, uint32_t m, uint64_t x)
+{
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Entries of the ascending array a[0, m) that are < x (lower bound).
+__device__ inline uint32_t count_lt(const uint64_t *a, uint32_t m, uint64_t x)
+{
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 __global__ void k_overlap_reset(uint32_t *__restrict__ ctl)
 {
-    if (threadIdx.x < 4)
+    if (threadIdx.x < 8)
         ctl[threadIdx.x] = 0;
 }
 
 // The slow list of the step that left the level records lvl (one per
 // position of the current order, so every entry is a distinct position).
-__global__ void k_overlap_seed(OverlapBufs o, const uint8_t *__restrict__ lvl, uint32_t n)
+__global__ void k_overlap_seed(OverlapBufs o, Halos g, const uint8_t *__restrict__ lvl)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n && slow_record(lvl[t], t)) {
-        const uint32_t k = atomicAdd(o.ctl, 1u);
-        if (k < (uint32_t)kSlowCap) {
-            o.slow[k] = t;
-            o.slow_rec[k] = lvl[t];
-        }
+    if (t >= g.n)
+        return;
+    const uint8_t rec = lvl[t];
+    const int cls = slow_class(rec, t - g.start(g.of(t)));
+    if (!cls)
+        return;
+    const uint32_t k = atomicAdd(o.ctl + (cls == 2 ? 0 : 3), 1u);
+    if (k < (uint32_t)(cls == 2 ? kSlowList : kCoreList)) {
+        const uint32_t q = (cls == 2 ? 0u : (uint32_t)kSlowList) + k;
+        o.slow[q] = t;
+        o.slow_rec[q] = rec;
     }
 }
 
@@ -71,20 +137,21 @@ __device__ inline bool below(const uint32_t *__restrict__ sorted, const S *__res
     return km < k || (km == k && p < t);
 }
 
-// Early elements (sorted positions [0, E)) below (k, t), found by one wave:
-// each round its 64 lanes probe 64 evenly spaced positions of the range that
-// holds the answer (the probes below form a prefix, being sorted), so that
-// 1.2e7 elements take four rounds of two dependent loads.  Wave-uniform.
+// Early elements at sorted positions [base, base + E) below (k, t), found by
+// one wave: each round its 64 lanes probe 64 evenly spaced positions of the
+// range that holds the answer (the probes below form a prefix, being
+// sorted), so that 1.2e7 elements take four rounds of two dependent loads.
+// Wave-uniform.
 template <typename S>
-__device__ inline uint32_t wave_count_below(const uint32_t *__restrict__ sorted, const S *__restrict__ r, uint32_t E,
-                                            typename KeyOf<S>::type k, uint32_t t)
+__device__ inline uint32_t wave_count_below(const uint32_t *__restrict__ sorted, const S *__restrict__ r,
+                                            uint32_t base, uint32_t E, typename KeyOf<S>::type k, uint32_t t)
 {
     const uint32_t lane = threadIdx.x & 63;
     uint32_t lo = 0, hi = E;  // the count lies in [lo, hi]
     while (hi > lo) {
         const uint32_t stride = (hi - lo + 63) / 64;
         const uint32_t p = lo + lane * stride;
-        const bool b = p < hi && below(sorted, r, p, k, t);
+        const bool b = p < hi && below(sorted, r, base + p, k, t);
         const uint32_t nb = (uint32_t)__popcll(__ballot(b));
         if (nb == 0) {
             hi = lo;
@@ -97,102 +164,98 @@ __device__ inline uint32_t wave_count_below(const uint32_t *__restrict__ sorted,
     return lo;
 }
 
-// One wave per query: wave w < kSlowCap is late slot w (its insertion
-// point), wave kSlowCap + q is slow-list entry q when it is an early element
-// (its sorted position).  Results in found[w].
-template <typename S>
-__global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o, const uint32_t *__restrict__ sorted,
-                                                                   const S *__restrict__ r, uint32_t n)
+__device__ inline uint32_t late_count(const OverlapBufs &o, uint32_t n)
 {
-    const uint32_t w = blockIdx.x * (kSearchThreads / 64) + (threadIdx.x >> 6);
     uint32_t K = o.ctl[1];
     if (K > (uint32_t)kSlowCap)
         K = kSlowCap;
-    if (K > n)
-        K = n;
-    uint32_t m = o.ctl[0];
-    if (m > (uint32_t)kSlowCap)
-        m = kSlowCap;
-    uint32_t t;
-    if (w < (uint32_t)kSlowCap) {
-        if (w >= K)
-            return;
-        t = o.def_pos[w];
-    } else {
-        const uint32_t q = w - (uint32_t)kSlowCap;
-        if (q >= m)
-            return;
-        t = o.slow[q];
-        if (t & kDeferBit)
-            return;
-    }
-    const uint32_t c = wave_count_below(sorted, r, n - K, key_at(r, t), t);
-    if ((threadIdx.x & 63) == 0)
-        o.found[w] = c;
+    return K > n ? n : K;
 }
 
-// Cost order of the deferred ranks: capped chains first, then by level.
-constexpr int kCostBins = 65;
-__device__ inline uint32_t slow_cost(uint32_t rec)
+// Slow-list entries selected for deferral (k_overlap_order).
+__device__ inline uint32_t selected_count(const OverlapBufs &o)
 {
-    return (rec & NS_LVL_NOCONV) ? 64u : (rec & NS_LVL_MASK);
+    const uint32_t m = o.ctl[4];
+    return m > (uint32_t)kSlowCap ? (uint32_t)kSlowCap : m;
 }
 
-// One block, after k_overlap_search.  ctl[1] late particles (old ranks
-// def_pos[j]) were sorted last; the previous step's slow list (ctl[0]
-// entries) names the ranks to defer in the coming step.  Ranks the late
-// particles among themselves by a bitonic sort on (key, old rank) and writes
-// their merged positions into perm and their insertion points (ascending)
-// into ins_lo; orders the coming step's deferred ranks costliest first by a
-// counting sort on the cost (the order within a cost is free: it decides
-// only which wave runs which particle) and writes them with their old
-// positions into def_pos / def_src; their count into ctl[1], the late count
-// into ctl[2]; clears ctl[0].
+// Cost order of the deferred ranks: class 2 before class 1, each capped
+// chains first, then by level.
+constexpr int kCostBins = 130;
+__device__ inline uint32_t slow_cost(uint32_t rec, bool class2)
+{
+    return (class2 ? 65u : 0u) + ((rec & NS_LVL_NOCONV) ? 64u : (rec & NS_LVL_MASK));
+}
+
+// Late particles of halo h in the (halo, key, rank)-sorted late list
+// late_h[0, K): [first, first + count).
+__device__ inline void late_run(const uint32_t *late_h, uint32_t K, uint32_t h, uint32_t &first, uint32_t &count)
+{
+    uint32_t lo = 0, hi = K;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (late_h[mid] < h)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    first = lo;
+    hi = K;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (late_h[mid] <= h)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    count = lo - first;
+}
+
+// One block.  The ctl[1] late particles (old ranks def_pos[j]) in (halo,
+// key, old rank) order by a bitonic sort in LDS: late_h / late_t /
+// late_slot[r] for the r-th.  And the slow-list entries to defer: the
+// costliest kSlowCap of both classes by a counting sort on the cost (the
+// order within a cost is free: it decides only which wave runs which
+// particle), into sel / ctl[4].
 template <typename S>
-__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, const S *__restrict__ r, uint32_t n)
+__global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, Halos g, const S *__restrict__ r)
 {
     using KeyT = typename KeyOf<S>::type;
-    __shared__ KeyT s_key[kSlowCap];        // late keys, sorted in place
-    __shared__ uint32_t s_old[kSlowCap];    // their old ranks, sorted alongside
-    __shared__ uint32_t s_slot[kSlowCap];   // their late slots, sorted alongside
-    __shared__ uint32_t s_t[kSlowCap];      // old rank of late slot j
-    __shared__ uint32_t s_pos[kSlowCap];    // merged position of late slot j
-    __shared__ uint32_t s_lo[kSlowCap];     // insertion points, ascending
-    __shared__ uint32_t s_bin[kCostBins];   // cost histogram, then the first slot of each cost
+    __shared__ uint32_t s_h[kSlowCap];
+    __shared__ KeyT s_key[kSlowCap];
+    __shared__ uint32_t s_old[kSlowCap];
+    __shared__ uint32_t s_slot[kSlowCap];
+    __shared__ uint32_t s_bin[kCostBins];  // cost histogram, then the first slot of each cost
     const uint32_t tid = threadIdx.x;
-    uint32_t K = o.ctl[1];
-    if (K > (uint32_t)kSlowCap)
-        K = kSlowCap;
-    if (K > n)
-        K = n;
-    uint32_t m = o.ctl[0];
-    if (m > (uint32_t)kSlowCap)
-        m = kSlowCap;
-    // late particles, padded above every real (key, rank) to a power of two
-    uint32_t P = 1;
+    const uint32_t K = late_count(o, g.n);
+    if (tid < (uint32_t)kCostBins)
+        s_bin[tid] = 0;
+    uint32_t P = 1;  // padded above every real entry to a power of two
     while (P < K)
         P <<= 1;
     if (tid < K) {
         const uint32_t t = o.def_pos[tid];
+        s_h[tid] = g.of(t);
         s_key[tid] = key_at(r, t);
         s_old[tid] = t;
-        s_t[tid] = t;
     } else {
+        s_h[tid] = ~0u;
         s_key[tid] = ~KeyT(0);
         s_old[tid] = ~0u;
     }
     s_slot[tid] = tid;
-    if (tid < (uint32_t)kCostBins)
-        s_bin[tid] = 0;
     __syncthreads();
     for (uint32_t k = 2; k <= P; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             const uint32_t l = tid ^ j;
             if (tid < P && l > tid) {
+                const uint32_t ha = s_h[tid], hb = s_h[l];
                 const KeyT ka = s_key[tid], kb = s_key[l];
                 const uint32_t ta = s_old[tid], tb = s_old[l];
-                const bool a_above = ka > kb || (ka == kb && ta > tb);
+                const bool a_above = ha > hb || (ha == hb && (ka > kb || (ka == kb && ta > tb)));
                 if (a_above == ((tid & k) == 0)) {  // ascending where bit k of tid is clear
+                    s_h[tid] = hb;
+                    s_h[l] = ha;
                     s_key[tid] = kb;
                     s_key[l] = ka;
                     s_old[tid] = tb;
@@ -205,19 +268,23 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, co
             __syncthreads();
         }
     }
-    if (tid < K) {  // the tid-th late particle in (key, old rank) order
-        const uint32_t j = s_slot[tid];
-        const uint32_t lo = o.found[j];
-        s_pos[j] = lo + tid;
-        s_lo[tid] = lo;
-        o.ins_lo[tid] = lo;
-        o.perm[lo + tid] = s_old[tid];
+    if (tid < K) {
+        o.late_h[tid] = s_h[tid];
+        o.late_t[tid] = s_old[tid];
+        o.late_slot[tid] = s_slot[tid];
     }
-    uint32_t v = 0, c = 0, within = 0;
-    if (tid < m) {
-        v = o.slow[tid];
-        c = slow_cost(o.slow_rec[tid]);
-        within = atomicAdd(&s_bin[c], 1u);
+    // selection: entry q of class 2 (q < m2) or class 1 (kSlowList + q, q < m1)
+    constexpr int kPer = kSlowList / kPlanThreads;
+    static_assert(kSlowList % kPlanThreads == 0 && kCoreList <= kPlanThreads, "entries per thread");
+    const uint32_t m2 = o.ctl[0] < (uint32_t)kSlowList ? o.ctl[0] : (uint32_t)kSlowList;
+    const uint32_t m1 = o.ctl[3] < (uint32_t)kCoreList ? o.ctl[3] : (uint32_t)kCoreList;
+    uint32_t cost[kPer + 1], within[kPer + 1];
+#pragma unroll
+    for (int u = 0; u <= kPer; u++) {
+        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
+        const bool have = u < kPer ? q < m2 : tid < m1;
+        cost[u] = have ? slow_cost(o.slow_rec[q], u < kPer) : 0u;
+        within[u] = have ? atomicAdd(&s_bin[cost[u]], 1u) : ~0u;
     }
     __syncthreads();
     if (tid == 0) {  // first slot of each cost, costliest first
@@ -229,62 +296,165 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, co
         }
     }
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u <= kPer; u++) {
+        if (within[u] == ~0u)
+            continue;
+        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
+        const uint32_t slot = s_bin[cost[u]] + within[u];
+        if (slot < (uint32_t)kSlowCap)
+            o.sel[slot] = o.slow[q];
+    }
+    if (tid == 0)
+        o.ctl[4] = m2 + m1 < (uint32_t)kSlowCap ? m2 + m1 : (uint32_t)kSlowCap;
+}
+
+// One wave per query: wave w < kSlowCap is the w-th late particle (its
+// insertion point among its halo's early elements), wave kSlowCap + q is
+// selected entry q when it is an early element (its position among its
+// halo's early elements).  Results in found[w].
+template <typename S>
+__global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o, Halos g,
+                                                                   const uint32_t *__restrict__ sorted,
+                                                                   const S *__restrict__ r)
+{
+    const uint32_t w = blockIdx.x * (kSearchThreads / 64) + (threadIdx.x >> 6);
+    const uint32_t K = late_count(o, g.n), m = selected_count(o);
+    uint32_t t, h;
+    if (w < (uint32_t)kSlowCap) {
+        if (w >= K)
+            return;
+        t = o.late_t[w];
+        h = o.late_h[w];
+    } else {
+        const uint32_t q = w - (uint32_t)kSlowCap;
+        if (q >= m)
+            return;
+        t = o.sel[q];
+        if (t & kDeferBit)
+            return;
+        h = g.of(t);
+    }
+    uint32_t first, kh;
+    late_run(o.late_h, K, h, first, kh);
+    const uint32_t c = wave_count_below(sorted, r, g.start(h), g.size(h) - kh, key_at(r, t), t);
+    if ((threadIdx.x & 63) == 0)
+        o.found[w] = c;
+}
+
+// One block, after k_overlap_order and k_overlap_search.  Writes the late
+// particles' merged positions into perm and their (halo, insertion point)
+// pairs, ascending, into ins; the selected entries (the previous step's
+// slow list, costliest first) at their new ranks, with their old positions,
+// into def_pos / def_src; their count into ctl[1], the late count into
+// ctl[2]; clears the class counts ctl[0] and ctl[3].
+__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Halos g)
+{
+    __shared__ uint64_t s_ins[kSlowCap];  // (halo, insertion point), ascending
+    __shared__ uint32_t s_pos[kSlowCap];  // merged position of late slot j
+    __shared__ uint32_t s_t[kSlowCap];    // old rank of late slot j
+    const uint32_t tid = threadIdx.x;
+    const uint32_t K = late_count(o, g.n), m = selected_count(o);
+    if (tid < K) {  // the tid-th late particle in (halo, key, old rank) order
+        const uint32_t h = o.late_h[tid], t = o.late_t[tid], j = o.late_slot[tid];
+        uint32_t first, kh;
+        late_run(o.late_h, K, h, first, kh);
+        const uint32_t lo = o.found[tid];
+        const uint32_t pos = g.start(h) + lo + (tid - first);
+        s_pos[j] = pos;
+        s_t[j] = t;
+        s_ins[tid] = hl_pack(h, lo);
+        o.ins[tid] = hl_pack(h, lo);
+        o.perm[pos] = t;
+    }
+    __syncthreads();
     if (tid < m) {
+        const uint32_t v = o.sel[tid];
         uint32_t pos, src;
         if (v & kDeferBit) {  // a late particle of this merge
             pos = s_pos[v & kIndexMask];
             src = s_t[v & kIndexMask];
-        } else {  // an early element, moved past the late particles inserted at or before it
-            const uint32_t e = o.found[kSlowCap + tid];
-            uint32_t lo = 0, hi = K;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_lo[mid] <= e)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            pos = e + lo;
+        } else {  // an early element, moved past its halo's late particles inserted at or before it
+            const uint32_t h = g.of(v), e = o.found[kSlowCap + tid];
+            pos = g.start(h) + e + count_le(s_ins, K, hl_pack(h, e)) - count_lt(s_ins, K, hl_pack(h, 0));
             src = v;
         }
-        const uint32_t slot = s_bin[c] + within;
-        o.def_pos[slot] = pos;
-        o.def_src[slot] = src;
+        o.def_pos[tid] = pos;
+        o.def_src[tid] = src;
     }
     __syncthreads();
     if (tid == 0) {
         o.ctl[0] = 0;
+        o.ctl[3] = 0;
         o.ctl[1] = m;
         o.ctl[2] = K;
     }
 }
 
-// Early elements to their merged positions: sorted[e] -> perm[e + shift(e)],
-// shift(e) = late particles with insertion point <= e.
-__global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, const uint32_t *__restrict__ sorted,
-                                                                uint32_t n)
+// Early elements to their merged positions.  Sorted position p of halo h
+// with local index e = p - start(h) < size(h) - (late particles of h) moves
+// to p + (late particles of h inserted at or before e); the positions past
+// that are h's late particles, placed by the plan.  Each block takes
+// kShiftTile positions; the bounds of their halos and those halos'
+// insertion pairs are staged in LDS (a tile spanning more than kShiftHalos
+// halos searches the halo bounds in global memory).
+__global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, Halos g,
+                                                                const uint32_t *__restrict__ sorted)
 {
-    __shared__ uint32_t s_lo[kSlowCap];
-    uint32_t K = o.ctl[2];
-    const uint32_t E = n - K;
-    for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads)
-        s_lo[q] = o.ins_lo[q];
+    __shared__ uint64_t s_ins[kSlowCap];
+    __shared__ uint32_t s_start[kShiftHalos + 1];
+    __shared__ uint32_t s_rng[4];  // first halo, halo count, first insertion pair, pair count
+    const uint32_t K = o.ctl[2];
+    const uint32_t b0 = blockIdx.x * (uint32_t)kShiftTile;
+    if (b0 >= g.n)
+        return;
+    const uint32_t b1 = b0 + (uint32_t)kShiftTile < g.n ? b0 + (uint32_t)kShiftTile : g.n;
+    if (threadIdx.x == 0) {
+        const uint32_t h0 = g.of(b0), h1 = g.of(b1 - 1);
+        const uint32_t i0 = count_lt(o.ins, K, hl_pack(h0, 0));
+        const uint32_t i1 = count_le(o.ins, K, hl_pack(h1, ~0u));
+        s_rng[0] = h0;
+        s_rng[1] = h1 - h0 + 1;
+        s_rng[2] = i0;
+        s_rng[3] = i1 - i0;
+    }
     __syncthreads();
-    const uint32_t base = blockIdx.x * (uint32_t)(kShiftThreads * kShiftItems) + threadIdx.x;
+    const uint32_t h0 = s_rng[0], nh = s_rng[1], i0 = s_rng[2], ni = s_rng[3];
+    const bool staged = nh <= (uint32_t)kShiftHalos;
+    for (uint32_t q = threadIdx.x; q < ni; q += kShiftThreads)
+        s_ins[q] = o.ins[i0 + q];
+    if (staged)
+        for (uint32_t q = threadIdx.x; q <= nh; q += kShiftThreads)
+            s_start[q] = g.start(h0 + q);
+    __syncthreads();
 #pragma unroll
     for (int it = 0; it < kShiftItems; it++) {
-        const uint32_t e = base + (uint32_t)it * kShiftThreads;
-        if (e >= E)
+        const uint32_t p = b0 + (uint32_t)it * kShiftThreads + threadIdx.x;
+        if (p >= b1)
             break;
-        uint32_t lo = 0, hi = K;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_lo[mid] <= e)
-                lo = mid + 1;
-            else
-                hi = mid;
+        uint32_t h, hs, he;
+        if (staged) {
+            uint32_t lo = 0, hi = nh;  // s_start[lo] <= p < s_start[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_start[mid] <= p)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            h = h0 + lo;
+            hs = s_start[lo];
+            he = s_start[lo + 1];
+        } else {
+            h = g.of(p);
+            hs = g.start(h);
+            he = g.start(h + 1);
         }
-        o.perm[e + lo] = sorted[e];
+        const uint32_t e = p - hs;
+        const uint32_t before = count_lt(s_ins, ni, hl_pack(h, 0));
+        const uint32_t kh = count_le(s_ins, ni, hl_pack(h, ~0u)) - before;
+        if (e < he - hs - kh)
+            o.perm[p + count_le(s_ins, ni, hl_pack(h, e)) - before] = sorted[p];
     }
 }
 
@@ -307,49 +477,63 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_fix_keys(OverlapBufs o
     }
 }
 
-template <typename S>
-hipError_t plan_typed(const OverlapBufs &o, const uint32_t *sorted, const S *r, int64_t n, hipStream_t stream)
+Halos halos_of(const int64_t *seg_off, int nseg, int64_t n)
 {
-    hipLaunchKernelGGL(k_overlap_search<S>, dim3(2 * kSlowCap / (kSearchThreads / 64)), dim3(kSearchThreads), 0,
-                       stream, o, sorted, r, (uint32_t)n);
+    Halos g;
+    g.off = nseg > 0 ? seg_off : nullptr;
+    g.nseg = nseg;
+    g.n = (uint32_t)n;
+    return g;
+}
+
+template <typename S>
+hipError_t plan_typed(const OverlapBufs &o, const Halos &g, const uint32_t *sorted, const S *r, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_overlap_order<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, g, r);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_overlap_plan<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, r, (uint32_t)n);
+    hipLaunchKernelGGL(k_overlap_search<S>, dim3(2 * kSlowCap / (kSearchThreads / 64)), dim3(kSearchThreads), 0,
+                       stream, o, g, sorted, r);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_overlap_plan, dim3(1), dim3(kPlanThreads), 0, stream, o, g);
     return hipGetLastError();
 }
 
 } // namespace
 
-hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, int64_t n, hipStream_t stream)
+hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream)
 {
     hipLaunchKernelGGL(k_overlap_reset, dim3(1), dim3(64), 0, stream, o.ctl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !lvl || n <= 0)
         return e;
-    hipLaunchKernelGGL(k_overlap_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, o, lvl,
-                       (uint32_t)n);
+    hipLaunchKernelGGL(k_overlap_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, o,
+                       halos_of(seg_off, nseg, n), lvl);
     return hipGetLastError();
 }
 
-hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, int64_t n,
-                        hipStream_t stream)
+hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, const int64_t *seg_off,
+                        int nseg, int64_t n, hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
     if (n > (int64_t)kIndexMask)
         return hipErrorInvalidValue;
-    return f32 ? plan_typed<float>(o, sorted, static_cast<const float *>(r), n, stream)
-               : plan_typed<double>(o, sorted, static_cast<const double *>(r), n, stream);
+    const Halos g = halos_of(seg_off, nseg, n);
+    return f32 ? plan_typed<float>(o, g, sorted, static_cast<const float *>(r), stream)
+               : plan_typed<double>(o, g, sorted, static_cast<const double *>(r), stream);
 }
 
-hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, int64_t n, hipStream_t stream)
+hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
-    const int64_t per_block = (int64_t)kShiftThreads * kShiftItems;
-    hipLaunchKernelGGL(k_overlap_shift, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(kShiftThreads), 0,
-                       stream, o, sorted, (uint32_t)n);
+    hipLaunchKernelGGL(k_overlap_shift, dim3((unsigned)((n + kShiftTile - 1) / kShiftTile)), dim3(kShiftThreads), 0,
+                       stream, o, halos_of(seg_off, nseg, n), sorted);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

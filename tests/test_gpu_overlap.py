@@ -201,3 +201,35 @@ def test_profile_kernel_mask(ns):
         ctx.reset_kernel_stats()
         ctx.run(ns.unit_params(), 2)
         assert {"integrate_bulk", "integrate_deep", "overlap_plan", "overlap_place"} <= set(ctx.kernel_stats())
+
+
+def ensemble_state(ns, sizes, seed, ties=False):
+    parts = [plunging_ic(ns, s, seed + h, ties) if s >= 100 else
+             tuple(x[:s].copy() for x in plunging_ic(ns, 100, seed + h, ties)) for h, s in enumerate(sizes)]
+    base = np.cumsum([0] + list(sizes[:-1]))
+    r = np.concatenate([p[0] for p in parts])
+    v = np.concatenate([p[1] for p in parts])
+    ell = np.concatenate([p[2] for p in parts])
+    ids = np.arange(r.size, dtype=np.uint32)
+    return r, v, ell, ids
+
+
+@pytest.mark.parametrize("budget,ties", [(64, False), (64, True), (2, False)])
+def test_overlap_ensemble_is_bit_identical(ns, budget, ties):
+    # halos sort as segments of their own: the late particles of each halo
+    # come back inside their halo, and a budget of 2 makes every particle slow
+    sizes = [20_000, 1, 5_000, 777, 15_000, 3]
+    state = ensemble_state(ns, sizes, 90, ties)
+    outs = []
+    for overlap in (0, 1):
+        with ns.GpuContext(state[0].size) as ctx:
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+            ctx.set_option(ns.NS_OPT_BUDGET, budget)
+            ctx.set_option(ns.NS_OPT_OVERLAP, overlap)
+            ctx.set_halos(sizes)
+            ctx.upload(*state)
+            ctx.run(ns.unit_params(), 3)
+            ctx.run(ns.unit_params(), 3)
+            outs.append(dict(state=ctx.download(), levels=ctx.levels(), hits=ctx.cap_hits()))
+    assert outs[0]["hits"] > 0
+    assert_same(outs[0], outs[1])
