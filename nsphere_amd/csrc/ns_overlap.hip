@@ -403,6 +403,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
 {
     __shared__ uint64_t s_ins[kSlowCap];
     __shared__ uint32_t s_start[kShiftHalos + 1];
+    __shared__ uint32_t s_before[kShiftHalos], s_kh[kShiftHalos];  // each staged halo's run of pairs
     __shared__ uint32_t s_rng[4];  // first halo, halo count, first insertion pair, pair count
     const uint32_t K = o.ctl[2];
     const uint32_t b0 = blockIdx.x * (uint32_t)kShiftTile;
@@ -427,12 +428,20 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
         for (uint32_t q = threadIdx.x; q <= nh; q += kShiftThreads)
             s_start[q] = g.start(h0 + q);
     __syncthreads();
+    if (staged) {
+        for (uint32_t q = threadIdx.x; q < nh; q += kShiftThreads) {
+            const uint32_t before = count_lt(s_ins, ni, hl_pack(h0 + q, 0));
+            s_before[q] = before;
+            s_kh[q] = count_le(s_ins, ni, hl_pack(h0 + q, ~0u)) - before;
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int it = 0; it < kShiftItems; it++) {
         const uint32_t p = b0 + (uint32_t)it * kShiftThreads + threadIdx.x;
         if (p >= b1)
             break;
-        uint32_t h, hs, he;
+        uint32_t hs, he, before, kh;
         if (staged) {
             uint32_t lo = 0, hi = nh;  // s_start[lo] <= p < s_start[hi]
             while (hi - lo > 1) {
@@ -442,19 +451,30 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
                 else
                     hi = mid;
             }
-            h = h0 + lo;
             hs = s_start[lo];
             he = s_start[lo + 1];
+            before = s_before[lo];
+            kh = s_kh[lo];
         } else {
-            h = g.of(p);
+            const uint32_t h = g.of(p);
             hs = g.start(h);
             he = g.start(h + 1);
+            before = count_lt(s_ins, ni, hl_pack(h, 0));
+            kh = count_le(s_ins, ni, hl_pack(h, ~0u)) - before;
         }
         const uint32_t e = p - hs;
-        const uint32_t before = count_lt(s_ins, ni, hl_pack(h, 0));
-        const uint32_t kh = count_le(s_ins, ni, hl_pack(h, ~0u)) - before;
-        if (e < he - hs - kh)
-            o.perm[p + count_le(s_ins, ni, hl_pack(h, e)) - before] = sorted[p];
+        if (e < he - hs - kh) {
+            // insertion points of this halo at or before e: its run is s_ins[before, before + kh)
+            uint32_t lo = 0, hi = kh;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((uint32_t)s_ins[before + mid] <= e)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            o.perm[p + lo] = sorted[p];
+        }
     }
 }
 
