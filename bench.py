@@ -477,7 +477,8 @@ def main() -> int:
                     help="autotune the chosen form's launch configuration first (STEPS timed steps per trial)")
     ap.add_argument("--budget", type=int, default=1 << 18,
                     help="force evaluations allowed per particle-step, the same for every form and "
-                         "the CPU baseline (the reference has no cap; see DESIGN.md)")
+                         "the CPU baseline (the reference has no cap: its first-order Levi-Civita tau steps "
+                         "can need 2^28 subdivisions on plunging orbits)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
