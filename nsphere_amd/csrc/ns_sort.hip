@@ -395,10 +395,12 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 {
     constexpr int WAVES = THREADS / 64;
     constexpr int TILE = THREADS * ITEMS;
-    // 32-bit keys: keys and payload staged together and written in one sweep
-    // (a slot's digit comes from its staged key); 64-bit keys: keys and their
-    // digits first, then the payload in the key buffer
-    constexpr bool kOneSweep = sizeof(KeyT) == 4;
+    // 32-bit keys, and 64-bit keys in tiles of up to 4096: keys and payload
+    // staged together and written in one sweep (a slot's digit comes from its
+    // staged key); larger 64-bit tiles: keys and their digits first, then the
+    // payload in the key buffer (LDS for both at once would leave one block
+    // per CU)
+    constexpr bool kOneSweep = sizeof(KeyT) == 4 || TILE <= 4096;
     static_assert(THREADS >= kRadix, "one thread per digit");
     __shared__ KeyT s_stage[TILE];                     // keys in digit order
     __shared__ uint32_t s_pay1[kOneSweep ? TILE : 1];  // payload in digit order (one sweep)
