@@ -214,8 +214,9 @@ def ensemble_state(ns, sizes, seed, ties=False):
     return r, v, ell, ids
 
 
-@pytest.mark.parametrize("budget,ties", [(64, False), (64, True), (2, False)])
-def test_overlap_ensemble_is_bit_identical(ns, budget, ties):
+@pytest.mark.parametrize("budget,ties,form,precision",
+                         [(64, False, 2, 0), (64, True, 2, 0), (2, False, 2, 0), (64, False, 2, 1), (64, False, 1, 0)])
+def test_overlap_ensemble_is_bit_identical(ns, budget, ties, form, precision):
     # halos sort as segments of their own: the late particles of each halo
     # come back inside their halo, and a budget of 2 makes every particle slow
     sizes = [20_000, 1, 5_000, 777, 15_000, 3]
@@ -223,7 +224,8 @@ def test_overlap_ensemble_is_bit_identical(ns, budget, ties):
     outs = []
     for overlap in (0, 1):
         with ns.GpuContext(state[0].size) as ctx:
-            ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
+            ctx.set_option(ns.NS_OPT_PRECISION, precision)
             ctx.set_option(ns.NS_OPT_BUDGET, budget)
             ctx.set_option(ns.NS_OPT_OVERLAP, overlap)
             ctx.set_halos(sizes)
