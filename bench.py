@@ -61,11 +61,14 @@ def bytes_per_particle(state_bytes: int) -> dict:
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
 # (N = 1.2e7, float32 state), scaled by the counters' reading for kernels of known traffic
-# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md); sort_scatter is the mean over its 4 passes.
+# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md); sort_scatter is the mean over its 4 passes
+# (8 with float64 keys), integrate_late from the overlapped run
+# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md).
 MEASURED_TRAFFIC = {("f32", 12_000_000): {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6,
                                           "integrate_bulk": 499.3e6, "integrate_deep": 14.0e6},
                     ("f64", 12_000_000): {"sort_scatter": 275.9e6, "sort_upsweep": 98.1e6,
-                                          "integrate_bulk": 835.9e6, "integrate_deep": 16.1e6}}
+                                          "integrate_bulk": 835.9e6, "integrate_deep": 16.1e6,
+                                          "integrate_late": 0.23e6}}
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
             "integrate_late": "late"}
