@@ -77,6 +77,11 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 # rooflines use.  The other families are timed over a short run after it.
 ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", "sort_scatter"]
 AFTER_STEPS = 20
+# One wave alone on an idle MI355X running the fast Levi-Civita micro-step
+# (reciprocal, one Newton step, force, velocity and time sums: nine dependent
+# FP64 operations) in a loop: 23.3 ns per micro-step, the issue time of its
+# instructions.  The late kernel's chains run at that rate at best.
+CHAIN_FLOOR_NS = 23.3
 
 
 
@@ -214,6 +219,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
             ctx.sync()
         stats = ctx.kernel_stats()
         evals = ctx.force_evals()
+        chain = ctx.chain_steps()
         # every kernel family, over a short run after the timed region
         ctx.set_option(ns.NS_OPT_PROFILE, 1)
         ctx.profile_kernels(None)
@@ -251,6 +257,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         "ms_per_step": el / steps * 1e3,
         "stats": stats,
         "evals": evals,
+        "chain": chain,
         "kernels": kernels,
         "kernels_after": kernels_after,
         "E0": K0 + W0,
@@ -424,11 +431,20 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
         ev = run["evals"][WORK_ROW[kernel]] / launches
         flops = FLOPS_PER_FORCE_EVAL * ev
         achieved = flops / avg_s / 1e12
-        return {"kernel": kernel, "bound": "fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
-                "traffic": traffic_for(run, n, kernel),
-                "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
-                "avg_us": round(avg_s * 1e6, 2)}
+        out = {"kernel": kernel, "bound": "fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+               "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+               "traffic": traffic_for(run, n, kernel),
+               "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
+               "avg_us": round(avg_s * 1e6, 2)}
+        if kernel == "integrate_late" and run.get("chain"):
+            # the launch lasts as long as its longest Levi-Civita chain: time
+            # per micro-step of that chain against one wave running the same
+            # micro-step alone on an idle GPU
+            ns_step = ms * 1e6 / run["chain"]
+            out["latency"] = {"chain_micro_steps_per_launch": round(run["chain"] / launches, 1),
+                              "ns_per_micro_step": round(ns_step, 2), "floor_ns": CHAIN_FLOOR_NS,
+                              "frac": round(CHAIN_FLOOR_NS / ns_step, 3)}
+        return out
     return None
 
 

@@ -399,6 +399,13 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *ctx);
  * the context's stream. */
 int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[3]);
 
+/* Sum over the late kernel's launches (overlapped runs, while NS_OPT_PROFILE
+ * was on, since the last ns_gpu_kernel_stats_reset) of each launch's longest
+ * Levi-Civita chain: the micro-steps one wave ran one after another, which
+ * bound the launch's duration.  A measurement aid with no counterpart in the
+ * reference.  Synchronises both of the context's streams. */
+int ns_gpu_chain_steps(ns_gpu_ctx *ctx, uint64_t *out);
+
 /* Seeded initial conditions in G = M = a = 1 units (host, deterministic per
  * particle through a Philox-4x32-10 counter stream keyed by seed):
  * kind 0 Plummer (inverse-CDF radii, Aarseth-Henon-Wielen speeds),

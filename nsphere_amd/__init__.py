@@ -302,6 +302,13 @@ class GpuContext:
         check(lib.ns_gpu_force_evals(self._h, out))
         return {"bulk_or_plain": out[0], "deep": out[1], "late": out[2]}
 
+    def chain_steps(self) -> int:
+        """Sum over the late kernel's launches while profiling of each
+        launch's longest Levi-Civita chain (sequential micro-steps)."""
+        out = ctypes.c_uint64()
+        check(lib.ns_gpu_chain_steps(self._h, ctypes.byref(out)))
+        return out.value
+
     def kernel_families(self) -> list:
         """Kernel family names in id order (bit k of NS_OPT_PROFILE_KERNELS)."""
         n = ctypes.c_int()

@@ -222,6 +222,12 @@ namespace {
                              __LINE__);                                                            \
     } while (0)
 
+// The profiling counters: the work rows and the chain counters after them.
+size_t work_bytes(const ns_gpu_ctx *c)
+{
+    return ((size_t)ns::kWorkKernels * (size_t)c->work_stride + ns::kChainWords) * sizeof(unsigned long long);
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -563,7 +569,8 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
     HIP_TRY(hipStreamWaitEvent(c->side, c->ev_sorted, 0));
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
     c->prof.begin(ns::K_MERGE, c->side);
-    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->seg_off, c->nseg, c->n, c->side));
+    HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->seg_off, c->nseg, c->n,
+                             c->prof.on ? c->work + ns::kWorkKernels * c->work_stride : nullptr, c->side));
     c->prof.end(ns::K_MERGE, c->side);
     HIP_TRY(hipEventRecord(c->ev_merged, c->side));
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
@@ -699,14 +706,14 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->energy_out, 2);
     if (e == hipSuccess) e = dalloc(&c->budget_hits, 1);
     c->work_stride = ns::work_stride_for(npts);
-    if (e == hipSuccess) e = dalloc(&c->work, ns::kWorkKernels * c->work_stride);
+    if (e == hipSuccess) e = dalloc(&c->work, ns::kWorkKernels * c->work_stride + ns::kChainWords);
     if (e == hipSuccess) e = dalloc(&c->q_lf, n);
     if (e == hipSuccess) e = dalloc(&c->q_lc, n);
     if (e == hipSuccess) e = dalloc(&c->q_count, ns::kQueueCounters);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->work, 0, work_bytes(c));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         int code = (e == hipErrorOutOfMemory) ? NS_ERR_NOMEM : NS_ERR_HIP;
@@ -1435,7 +1442,7 @@ int ns_gpu_kernel_stats_reset(ns_gpu_ctx *c)
     DeviceGuard g(c->device);
     c->prof.reset();
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemset(c->work, 0, ns::kWorkKernels * c->work_stride * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->work, 0, work_bytes(c)));
     return NS_OK;
 }
 
@@ -1778,6 +1785,22 @@ int ns_gpu_enclosed_mass(ns_gpu_ctx *c, double *host_out)
     DeviceGuard g(c->device);
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(host_out, c->menc, (size_t)(c->n + 2) * sizeof(double), hipMemcpyDeviceToHost));
+    return NS_OK;
+}
+
+int ns_gpu_chain_steps(ns_gpu_ctx *c, uint64_t *out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_chain_steps"))
+        return rc;
+    if (!out)
+        return set_error(NS_ERR_ARG, "ns_gpu_chain_steps: null output");
+    DeviceGuard g(c->device);
+    unsigned long long h[ns::kChainWords];
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->side)
+        HIP_TRY(hipStreamSynchronize(c->side));
+    HIP_TRY(hipMemcpy(h, c->work + ns::kWorkKernels * c->work_stride, sizeof(h), hipMemcpyDeviceToHost));
+    *out = h[ns::kChainSum] + h[ns::kChainMax];  // the last launch is not folded yet
     return NS_OK;
 }
 

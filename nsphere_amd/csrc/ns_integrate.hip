@@ -268,6 +268,8 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
     }
     if ((threadIdx.x & 63) == 0)
         write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i);
+    if (a.work && (threadIdx.x & 63) == 0 && bud.chain > 0)
+        atomicMax(a.work + kWorkKernels * a.work_stride + kChainMax, (unsigned long long)bud.chain);
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);
 }
 

@@ -47,7 +47,8 @@ struct StepArgs {
     // profiling only (else null): force evaluations spent, one slot per
     // wave of a launch (slot = global wave index; each wave adds its own with
     // a plain read-modify-write, launches being stream-ordered), in rows of
-    // work_stride slots: row 0 plain or bulk, 1 deep, 2 late (side stream)
+    // work_stride slots: row 0 plain or bulk, 1 deep, 2 late (side stream),
+    // then the kChainWords chain counters
     unsigned long long *work;
     int64_t work_stride;
     // overlapped run (else null): every result slow_class ranks appends its
@@ -64,6 +65,13 @@ struct StepArgs {
 };
 
 constexpr int kWorkKernels = 3;
+// After the work rows (profiling only): the longest Levi-Civita chain of the
+// running late launch (micro-steps, atomic max) and the sum of the longest
+// chains of the launches before it, folded in by the plan kernel that
+// precedes each late launch on the side stream.
+constexpr int kChainMax = 0;
+constexpr int kChainSum = 1;
+constexpr int kChainWords = 2;
 constexpr int kQueueCounters = 3;
 constexpr uint32_t kDeferBit = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
@@ -158,9 +166,10 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t
 // def_src / ctl[1], and the late ranks' merged positions in perm -- all the
 // late kernel reads.  overlap_place: the early elements at their merged
 // positions in perm and the deferred ranks flagged -- what the bulk kernel
-// reads.
+// reads.  chain (profiling, else null): the kChainWords counters, whose
+// running maximum overlap_plan folds into the sum.
 hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, const int64_t *seg_off,
-                        int nseg, int64_t n, hipStream_t stream);
+                        int nseg, int64_t n, unsigned long long *chain, hipStream_t stream);
 hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int64_t *seg_off, int nseg, int64_t n,
                          hipStream_t stream);
 // The late kernel: rank def_pos[j], gathered from position def_src[j] of the

@@ -348,8 +348,13 @@ __global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o
 // slow list, costliest first) at their new ranks, with their old positions,
 // into def_pos / def_src; their count into ctl[1], the late count into
 // ctl[2]; clears the class counts ctl[0] and ctl[3].
-__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Halos g)
+__global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Halos g, unsigned long long *chain)
 {
+    // the previous late launch has ended (same stream): fold its longest chain
+    if (chain && threadIdx.x == 0) {
+        chain[kChainSum] += chain[kChainMax];
+        chain[kChainMax] = 0;
+    }
     __shared__ uint64_t s_ins[kSlowCap];  // (halo, insertion point), ascending
     __shared__ uint32_t s_pos[kSlowCap];  // merged position of late slot j
     __shared__ uint32_t s_t[kSlowCap];    // old rank of late slot j
@@ -507,7 +512,8 @@ Halos halos_of(const int64_t *seg_off, int nseg, int64_t n)
 }
 
 template <typename S>
-hipError_t plan_typed(const OverlapBufs &o, const Halos &g, const uint32_t *sorted, const S *r, hipStream_t stream)
+hipError_t plan_typed(const OverlapBufs &o, const Halos &g, const uint32_t *sorted, const S *r,
+                      unsigned long long *chain, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_overlap_order<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, g, r);
     hipError_t e = hipGetLastError();
@@ -517,7 +523,7 @@ hipError_t plan_typed(const OverlapBufs &o, const Halos &g, const uint32_t *sort
                        stream, o, g, sorted, r);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_overlap_plan, dim3(1), dim3(kPlanThreads), 0, stream, o, g);
+    hipLaunchKernelGGL(k_overlap_plan, dim3(1), dim3(kPlanThreads), 0, stream, o, g, chain);
     return hipGetLastError();
 }
 
@@ -536,15 +542,15 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t
 }
 
 hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, const int64_t *seg_off,
-                        int nseg, int64_t n, hipStream_t stream)
+                        int nseg, int64_t n, unsigned long long *chain, hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
     if (n > (int64_t)kIndexMask)
         return hipErrorInvalidValue;
     const Halos g = halos_of(seg_off, nseg, n);
-    return f32 ? plan_typed<float>(o, g, sorted, static_cast<const float *>(r), stream)
-               : plan_typed<double>(o, g, sorted, static_cast<const double *>(r), stream);
+    return f32 ? plan_typed<float>(o, g, sorted, static_cast<const float *>(r), chain, stream)
+               : plan_typed<double>(o, g, sorted, static_cast<const double *>(r), chain, stream);
 }
 
 hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int64_t *seg_off, int nseg, int64_t n,

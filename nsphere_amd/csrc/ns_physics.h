@@ -231,6 +231,7 @@ __device__ inline void repeat_uniform(int64_t n, F &&f)
 struct Budget {
     int64_t left;
     bool hit;
+    int64_t chain = 0;  // micro-steps the wave Levi-Civita form ran one after another
     __device__ inline bool take(int64_t n)
     {
         if (n > left) {
@@ -661,6 +662,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
             rho_out = __shfl(rn, src_c, 64);
             v_out = __shfl(vn, src_c, 64);
             t_out = __shfl(tn, src_c, 64);
+            bud.chain += it;
             return (k == 0 ? 0 : level_of(Nk >> 1)) | NS_LVL_NOCONV;
         }
         run_level(k);
@@ -672,6 +674,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
         const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
         if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
             lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
+            bud.chain += it;
             return level_of(Nk);
         }
     }
@@ -682,6 +685,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
         rho_out = __shfl(rn, kmax, 64);
         v_out = __shfl(vn, kmax, 64);
         t_out = __shfl(tn, kmax, 64);
+        bud.chain += it;
         return level_of(Nf >> 1) | NS_LVL_NOCONV;
     }
     run_level(kmax + 1);
@@ -689,6 +693,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     rho_out = __shfl(rn, kmax + 1, 64);
     v_out = __shfl(vn, kmax + 1, 64);
     t_out = __shfl(tn, kmax + 1, 64);
+    bud.chain += it;
     return level_of(Nf) | NS_LVL_NOCONV;
 }
 

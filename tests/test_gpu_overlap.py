@@ -1,6 +1,7 @@
 """Overlapped runs (NS_OPT_OVERLAP=1): the particles whose previous step was
-slow (not converged, or level 8 or deeper) are integrated on a second stream
-while the next step's radii are sorted, then merged back at their ranks.  The merge must reproduce the
+slow (not converged, or level 6 or deeper; else among a halo's 64 innermost
+ranks) are integrated on a second stream while the rest of the step runs and
+the next step's radii are sorted, then merged back at their ranks.  The merge must reproduce the
 stable sort of the true keys, so every result is bit-identical to the run
 with every kernel on one stream -- including exact radius ties, more deferred
 particles than the 1024 slots, float32 state, and the work counters.
@@ -28,16 +29,18 @@ def assert_bits(a, b, what):
     assert bad.size == 0, f"{what}: {bad.size} differ, first at {bad[0]}: {a[bad[0]]!r} vs {b[bad[0]]!r}"
 
 
-def plunging_ic(ns, n, seed, ties=False):
+def plunging_ic(ns, n, seed, ties=False, ell_range=(1e-6, 1e-4)):
     """Plummer with a 20 % low-ell tail and particles on radial plunges near
-    the centre (deep Levi-Civita chains that hit a small work cap)."""
+    the centre (deep chains that hit a small work cap).  With ell in
+    [5e-3, 2e-2] the plunging particles sit inside r_crit and take the
+    Levi-Civita branch."""
     r, v, ell, ids = ns.generate_ic(0, n, seed, 100.0, 0.2, 1e-3)
     rng = np.random.default_rng(seed)
     k = n // 100
     pick = rng.choice(n, k, replace=False)
     r[pick] = rng.uniform(1e-4, 3e-3, k)
     v[pick] = rng.uniform(-0.6, 0.6, k)
-    ell[pick] = rng.uniform(1e-6, 1e-4, k)
+    ell[pick] = rng.uniform(*ell_range, k)
     if ties:
         r = np.round(r, 3) + 1e-3  # a 1e-3 grid: many exact ties, all positive
     return r, v, ell, ids
@@ -59,6 +62,9 @@ def run(ns, state, overlap, form, precision, budget, steps, profile=False):
         if profile:
             out["evals"] = ctx.force_evals()
             out["stats"] = ctx.kernel_stats()
+            out["chain"] = ctx.chain_steps()
+            ctx.reset_kernel_stats()
+            out["chain_after_reset"] = ctx.chain_steps()
     return out
 
 
@@ -109,6 +115,23 @@ def test_overlap_counters_and_timing(ns):
     assert two["stats"]["overlap_plan"][0] == 5
     assert two["stats"]["overlap_place"][0] == 5
     assert "integrate_late" not in one["stats"]
+
+
+def test_overlap_chain_counter(ns):
+    """The sum over late launches of each launch's longest Levi-Civita chain.
+    A wave's tau step that stops at level K runs 4 * 2^K + 1 micro-steps one
+    after another and is charged at least that many force evaluations, so the
+    sum is bounded by the late kernel's force evaluations and, per launch, by
+    one particle's work cap."""
+    # 157 Levi-Civita particles, all stopped by the cap (checked on the oracle)
+    state = plunging_ic(ns, 30_000, 13, ell_range=(5e-3, 2e-2))
+    budget, steps = 4096, 5
+    one = run(ns, state, 0, 2, 0, budget, steps, profile=True)
+    two = run(ns, state, 1, 2, 0, budget, steps, profile=True)
+    assert one["chain"] == 0
+    assert 0 < two["chain"] <= two["evals"]["late"]
+    assert two["chain"] <= steps * budget
+    assert two["chain_after_reset"] == 0
 
 
 def test_overlap_strict_matches_capped_oracle(ns):
