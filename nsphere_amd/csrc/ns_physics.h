@@ -214,15 +214,23 @@ __device__ inline int64_t uniform_i64(int64_t x)
 template <typename F>
 __device__ inline void repeat_uniform(int64_t n, F &&f)
 {
+    // The trip count is wave-uniform.  It runs down in 32-bit chunks so that
+    // the loop test stays on the scalar unit: there is no 64-bit scalar
+    // less-than compare, and a vector compare would cost the lone wave of a
+    // long chain one more vector issue per pass.
     int64_t m = uniform_i64(n);
-    for (; m >= 4; m -= 4) {
-        f();
-        f();
-        f();
-        f();
+    while (m > 0) {
+        uint32_t c = m > (int64_t)0x40000000 ? 0x40000000u : (uint32_t)m;
+        m -= (int64_t)c;
+        for (; c >= 4; c -= 4) {
+            f();
+            f();
+            f();
+            f();
+        }
+        for (; c > 0; c--)
+            f();
     }
-    for (; m > 0; m--)
-        f();
 }
 
 // Force-evaluation budget per particle and step: a guard against inputs on
