@@ -77,11 +77,12 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 # rooflines use.  The other families are timed over a short run after it.
 ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", "sort_scatter"]
 AFTER_STEPS = 20
-# One wave alone on an idle MI355X running the fast Levi-Civita micro-step
-# (reciprocal, one Newton step, force, velocity and time sums: nine dependent
-# FP64 operations) in a loop: 23.3 ns per micro-step, the issue time of its
-# instructions.  The late kernel's chains run at that rate at best.
-CHAIN_FLOOR_NS = 23.3
+# One wave alone on an idle MI355X running the kernel's fast Levi-Civita
+# micro-step (reciprocal, one Newton step, force, velocity and time sums: nine
+# dependent FP64 operations) in a loop unrolled by four on a 32-bit scalar trip
+# counter: 23.4 ns per micro-step, the issue time of its instructions.  The
+# late kernel's chains run at that rate at best.
+CHAIN_FLOOR_NS = 23.4
 
 
 
