@@ -21,7 +21,8 @@ extern "C" {
 #endif
 
 /* nsphere.c:880 (declaration), 8662-8665: sort every column by radius, in
- * place.  Exact and stable (ties keep their input order, as with quadsort). */
+ * place -- all five, mu (particles[4]) included, as nsphere.c:8613-8647 does.
+ * Exact and stable (ties keep their input order, as with quadsort). */
 void sort_particles(double **particles, int npts);
 
 /* nsphere.c:903 (declaration), 8605-8652.  Every sortAlg value -- "quadsort",
@@ -30,11 +31,15 @@ void sort_particles(double **particles, int npts);
 void sort_particles_with_alg(double **particles, int npts, const char *sortAlg);
 
 /* The method-5 loop body, nsphere.c:5775-5838, which the reference inlines in
- * main(): sort, inverse map (inverse_map may be NULL), then every particle
- * advanced by dt at its frozen rank with doAdaptiveFullLeap or
- * doAdaptiveFullLeviCivita and the constants of nsphere.c:364-373,
- * 5777-5782 (G_CONST, HALO_MASS, VEL_CONV_SQ, tolerances 1e-7, max_subdiv
- * 2^28, Richardson output, N_taumin 10, alpha 0.05). */
+ * main(): sort (all five columns), inverse map (inverse_map may be NULL),
+ * then every particle advanced by dt at its frozen rank with
+ * doAdaptiveFullLeap or doAdaptiveFullLeviCivita and the constants of
+ * nsphere.c:364-373, 5777-5782 (G_CONST, HALO_MASS, VEL_CONV_SQ, tolerances
+ * 1e-7, max_subdiv 2^28, Richardson output, N_taumin 10, alpha 0.05).  No
+ * work cap: a particle refines until it converges or reaches max_subdiv, as
+ * in the reference.  The environment variable NSPHERE_GPU_BUDGET (force
+ * evaluations per particle-step) sets the library's cap instead; each step
+ * that stops particles at it prints their count on stderr. */
 void nsphere_method5_step(double **particles, int npts, double dt, int *inverse_map);
 
 /* Releases the GPU context the calls above keep between calls. */

@@ -114,7 +114,9 @@ enum {
                               integrator and the energy.  Switching converts
                               the current state (round to nearest / exact). */
     NS_OPT_BUDGET = 4,     /* force evaluations allowed per particle and step
-                              (default 2^26); a particle that needs more is
+                              (default 2^26; 0: no cap, the reference's own
+                              behaviour, bounded only by max_subdiv and its
+                              1e8 tau steps); a particle that needs more is
                               stopped, flagged NS_LVL_NOCONV and counted
                               (ns_gpu_cap_hits).  Not in the reference, whose
                               Levi-Civita tau steps converge only to first
@@ -145,7 +147,9 @@ int ns_gpu_set_option(ns_gpu_ctx *ctx, int option, int64_t value);
 int64_t ns_gpu_npts(const ns_gpu_ctx *ctx);
 
 /* particles[0..3] = r, v_r, ell, id (as double), the reference's layout;
- * particles[4] (mu) is neither read nor written. */
+ * particles[4] (mu) is neither read nor written (ns_gpu_permute_column
+ * moves it with a reordering).  An upload keeps per-particle masses by
+ * position: mass k goes with the particle uploaded at position k. */
 int ns_gpu_upload(ns_gpu_ctx *ctx, double *const particles[5]);
 int ns_gpu_download(ns_gpu_ctx *ctx, double *particles[5]);
 /* Typed SoA host arrays (any of the outputs may be NULL). */
@@ -159,6 +163,18 @@ int ns_gpu_device_state(ns_gpu_ctx *ctx, double **r, double **v, double **ell,
 
 /* sort_particles semantics: reorder all columns by radius, in place. */
 int ns_gpu_sort_particles(ns_gpu_ctx *ctx);
+/* The permutation of the last reordering of the state -- ns_gpu_sort_particles,
+ * or the sort of the last step of ns_gpu_step / ns_gpu_run: position k of the
+ * current order holds the particle that sat at position host_out[k] before it.
+ * The reference moves all five columns of particles[5][npts] by this
+ * permutation (nsphere.c:8613-8647).  NS_ERR_STATE before any reordering
+ * since the last upload (or after ns_gpu_autotune).  host_out: npts. */
+int ns_gpu_permutation(ns_gpu_ctx *ctx, uint32_t *host_out);
+/* A caller-held column that the library does not keep (the reference's mu,
+ * particles[4], which sort_particles_with_alg moves with the others,
+ * nsphere.c:8613-8647) reordered on the device by that same permutation:
+ * out[k] = in[perm[k]].  in and out may be the same array (npts doubles). */
+int ns_gpu_permute_column(ns_gpu_ctx *ctx, const double *in, double *out);
 /* inverse_map[id] = rank for the current order (host_out may be NULL to
  * keep it on the device only). */
 int ns_gpu_inverse_map(ns_gpu_ctx *ctx, int32_t *host_out);

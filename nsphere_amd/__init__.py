@@ -11,7 +11,8 @@ libnsphere_gpu.so (include/nsphere_gpu.h):
   host round trips.
 
 Particles use the reference's layout: ``particles`` is a (5, npts) float64
-array holding r, v_r, ell, id (as a double) and mu (ignored).
+array holding r, v_r, ell, id (as a double) and mu (not used by the step, but
+moved with the others by every reordering, as the reference moves it).
 """
 from __future__ import annotations
 
@@ -148,6 +149,21 @@ class GpuContext:
     # -- the loop -------------------------------------------------------
     def sort_particles(self) -> None:
         check(lib.ns_gpu_sort_particles(self._h))
+
+    def permutation(self) -> np.ndarray:
+        """The last reordering: position k now holds what sat at position perm[k]."""
+        out = np.empty(self.npts, np.uint32)
+        check(lib.ns_gpu_permutation(self._h, _ptr(out)))
+        return out
+
+    def permute_column(self, col) -> np.ndarray:
+        """A caller-held float64 column (e.g. mu) reordered by the last reordering, on the device."""
+        a = _c_f64(col)
+        if a.shape != (self.npts,):
+            raise ValueError(f"expected shape ({self.npts},)")
+        out = np.empty(self.npts, np.float64)
+        check(lib.ns_gpu_permute_column(self._h, _ptr(a), _ptr(out)))
+        return out
 
     def inverse_map(self) -> np.ndarray:
         out = np.empty(self.npts, np.int32)
@@ -500,6 +516,7 @@ def sort_particles(particles: np.ndarray, npts: int) -> None:
     ctx.upload_particles(particles)
     ctx.sort_particles()
     ctx.download_particles(particles)
+    particles[4] = ctx.permute_column(particles[4])
 
 
 def method5_step(particles: np.ndarray, npts: int, params: StepParams,
@@ -516,4 +533,5 @@ def method5_step(particles: np.ndarray, npts: int, params: StepParams,
     if inverse_map is not None:
         inverse_map[:] = ctx.inverse_map()
     ctx.download_particles(particles)
+    particles[4] = ctx.permute_column(particles[4])
     return ctx.levels()

@@ -34,6 +34,10 @@ struct ns_gpu_ctx {
     uint8_t *lvl = nullptr;
     uint32_t *perm = nullptr;
     int32_t *inv = nullptr;
+    // the permutation of the last reordering of the state (c->perm, or the
+    // merged order of an overlapped run, whose entries carry kDeferBit);
+    // null after an upload
+    const uint32_t *last_perm = nullptr;
     ns::SortWork sw = {};
     double *energy_partial = nullptr;
     double *energy_out = nullptr;  // device, 2 doubles
@@ -466,14 +470,17 @@ bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
            a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
 }
 
-// Sort r[cur] into c->perm (keys from the integrate epilogue when valid).
-int sort_perm(ns_gpu_ctx *c)
+// Sort r[cur] into out (c->perm unless given; keys from the integrate
+// epilogue when valid).
+int sort_perm(ns_gpu_ctx *c, uint32_t *out = nullptr)
 {
+    if (!out)
+        out = c->perm;
     if (c->tiebreak) {  // (radius, id): ids first, radius keys gathered through their order
         const int s = c->cur;
         c->prof.begin(ns::K_TIEFIX);
         const hipError_t e = ns::radix_sort_perm_ids(c->sw, c->n, c->f32 ? (const void *)c->rf[s] : (const void *)c->r[s],
-                                                     c->f32, c->id[s], c->id_order, c->perm, c->stream, nullptr);
+                                                     c->f32, c->id[s], c->id_order, out, c->stream, nullptr);
         c->prof.end(ns::K_TIEFIX);
         HIP_TRY(e);
         c->keys_valid = false;
@@ -488,7 +495,7 @@ int sort_perm(ns_gpu_ctx *c)
             HIP_TRY(ns::launch_radius_keys(c->r[c->cur], c->sw.keys[0], c->n, c->stream));
         c->prof.end(ns::K_KEYS);
     }
-    HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->f32 ? 32 : 64, c->perm, c->stream, &c->prof));
+    HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->f32 ? 32 : 64, out, c->stream, &c->prof));
     c->keys_valid = false;  // the passes overwrite keys[0]
     return NS_OK;
 }
@@ -542,6 +549,8 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
         a.order = c->order;
     }
     HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
+    if (gather)
+        c->last_perm = c->perm;
     c->cur = d;
     c->keys_valid = true;
     c->levels_valid = true;
@@ -588,6 +597,7 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
     HIP_TRY(ns::overlap_place(c->ov, c->perm, c->seg_off, c->nseg, c->n, c->stream));
     c->prof.end(ns::K_PLACE);
     HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
+    c->last_perm = c->ov.perm;
     c->cur ^= 1;
     c->keys_valid = true;
     c->levels_valid = true;
@@ -801,9 +811,9 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         c->integrator = (int)value;
         return NS_OK;
     case NS_OPT_BUDGET:
-        if (value < 2)
-            return set_error(NS_ERR_ARG, "NS_OPT_BUDGET must be >= 2 force evaluations");
-        c->budget = value;
+        if (value != 0 && value < 2)
+            return set_error(NS_ERR_ARG, "NS_OPT_BUDGET must be 0 (no cap) or >= 2 force evaluations");
+        c->budget = value == 0 ? INT64_MAX : value;
         return NS_OK;
     case NS_OPT_SORT_TILE: {
         ns::SortConfig cfg = c->sw.config;
@@ -860,6 +870,30 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
     }
 }
 
+} // extern "C"
+
+namespace {
+
+// An upload writes the state into buffer 0.  Per-particle masses stay by
+// position (mass k goes with the particle uploaded at position k), so after
+// an odd number of reorderings they move to buffer 0 with it.
+int masses_to_buffer0(ns_gpu_ctx *c)
+{
+    if (!c->has_mass || c->cur == 0 || c->n == 0)
+        return NS_OK;
+    if (c->f32)
+        HIP_TRY(hipMemcpyAsync(c->mf[0], c->mf[c->cur], (size_t)c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                               c->stream));
+    else
+        HIP_TRY(hipMemcpyAsync(c->md[0], c->md[c->cur], (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    return NS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
 int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const double *ell, const uint32_t *id)
 {
     if (int rc = check_ctx(c, "ns_gpu_upload_soa"))
@@ -868,7 +902,10 @@ int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const dou
         return set_error(NS_ERR_ARG, "ns_gpu_upload_soa: null array");
     DeviceGuard g(c->device);
     const size_t nd = (size_t)c->n * sizeof(double), ni = (size_t)c->n * sizeof(uint32_t);
+    if (int rc = masses_to_buffer0(c))
+        return rc;
     c->cur = 0;
+    c->last_perm = nullptr;
     if (c->n > 0) {
         HIP_TRY(hipMemcpyAsync(c->r[0], r, nd, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->v[0], v, nd, hipMemcpyHostToDevice, c->stream));
@@ -889,7 +926,10 @@ int ns_gpu_upload_soa(ns_gpu_ctx *c, const double *r, const double *v, const dou
 // the float32 form and mark it uploaded, as ns_gpu_upload_soa does.
 static int state_written(ns_gpu_ctx *c)
 {
+    if (int rc = masses_to_buffer0(c))
+        return rc;
     c->cur = 0;
+    c->last_perm = nullptr;
     if (c->n > 0 && c->f32)
         if (int rc = state_to_f32(c, 0))
             return rc;
@@ -1033,9 +1073,52 @@ int ns_gpu_sort_particles(ns_gpu_ctx *c)
         HIP_TRY(ns::launch_gather_state<double>(c->r[s], c->v[s], c->l[s], c->id[s], c->perm, c->r[d], c->v[d],
                                                 c->l[d], c->id[d], c->n, c->stream));
     c->prof.end(ns::K_GATHER);
+    c->last_perm = c->perm;
     c->cur = d;
     c->keys_valid = false;
     c->levels_valid = false;
+    return NS_OK;
+}
+
+int ns_gpu_permutation(ns_gpu_ctx *c, uint32_t *host_out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_permutation"))
+        return rc;
+    if (!host_out && c->n > 0)
+        return set_error(NS_ERR_ARG, "ns_gpu_permutation: null output");
+    if (!c->last_perm)
+        return set_error(NS_ERR_STATE, "ns_gpu_permutation: no sort or step since the last upload");
+    if (c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    // vals[0] is the sort's payload scratch, free between sorts
+    HIP_TRY(ns::launch_perm_copy(c->last_perm, c->sw.vals[0], c->n, c->stream));
+    HIP_TRY(hipMemcpyAsync(host_out, c->sw.vals[0], (size_t)c->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int ns_gpu_permute_column(ns_gpu_ctx *c, const double *in, double *out)
+{
+    if (int rc = check_ctx(c, "ns_gpu_permute_column"))
+        return rc;
+    if ((!in || !out) && c->n > 0)
+        return set_error(NS_ERR_ARG, "ns_gpu_permute_column: null column");
+    if (!c->last_perm)
+        return set_error(NS_ERR_STATE, "ns_gpu_permute_column: no sort or step since the last upload");
+    if (c->n == 0)
+        return NS_OK;
+    DeviceGuard g(c->device);
+    // staging: the key ping-pong buffer keys[1] (free between sorts) and the
+    // float64 buffer cur^1, which holds the state before the reordering
+    double *d_in = reinterpret_cast<double *>(c->sw.keys[1]);
+    double *d_out = c->r[c->cur ^ 1];
+    const size_t nd = (size_t)c->n * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(d_in, in, nd, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(ns::launch_gather_column(d_in, c->last_perm, d_out, c->n, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_out, nd, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
 
@@ -1390,21 +1473,24 @@ int ns_gpu_energy(ns_gpu_ctx *c, const ns_step_params *p, double *K, double *W)
     }
     // Ranks of the current radii (after a step the arrays hold the order of
     // that step's sort, not of the new radii).
-    if (int rc = sort_perm(c))
+    // (into a queue buffer, free between steps: c->perm keeps the last
+    // reordering's permutation for ns_gpu_permutation)
+    uint32_t *eperm = c->q_lf;
+    if (int rc = sort_perm(c, eperm))
         return rc;
-    if (int rc = mass_scan(c, c->perm, false))
+    if (int rc = mass_scan(c, eperm, false))
         return rc;
     const double m = p->halo_mass / (double)c->n;
     const double *menc = c->has_mass ? c->menc : nullptr;
     c->prof.begin(ns::K_ENERGY);
     const int s = c->cur;
     if (c->f32)
-        HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], c->perm, c->has_mass ? c->mf[s] : nullptr,
+        HIP_TRY(ns::launch_energy<float>(c->rf[s], c->vf[s], c->lf[s], eperm, c->has_mass ? c->mf[s] : nullptr,
                                          menc, c->seg_off, c->seg_mass, c->nseg, c->n, m, p->vel_conv_sq * p->G,
                                          p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
                                          c->stream));
     else
-        HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], c->perm, c->has_mass ? c->md[s] : nullptr,
+        HIP_TRY(ns::launch_energy<double>(c->r[s], c->v[s], c->l[s], eperm, c->has_mass ? c->md[s] : nullptr,
                                           menc, c->seg_off, c->seg_mass, c->nseg, c->n, m, p->vel_conv_sq * p->G,
                                           p->halo_mass, p->identity_gravity, c->energy_partial, c->energy_out,
                                           c->stream));
@@ -1598,6 +1684,7 @@ int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_r
     }
     // leave the state as it was, with the chosen options
     c->cur = cur0;
+    c->last_perm = nullptr;
     c->keys_valid = false;
     c->levels_valid = false;
     const hipError_t er = snapshot_copy(c, snap, false);

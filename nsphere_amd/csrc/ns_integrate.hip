@@ -295,6 +295,23 @@ __global__ void k_gather_state(const S *__restrict__ r0, const S *__restrict__ v
     }
 }
 
+// The permutation of a reordering without the overlap's defer flags.
+__global__ void k_perm_copy(const uint32_t *__restrict__ perm, uint32_t *__restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = perm[i] & kIndexMask;
+}
+
+// A caller's column reordered as the state was: out[i] = in[perm[i]].
+__global__ void k_gather_column(const double *__restrict__ in, const uint32_t *__restrict__ perm,
+                                double *__restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = in[perm[i] & kIndexMask];
+}
+
 template <typename D, typename S>
 __global__ void k_convert(const S *__restrict__ src, D *__restrict__ dst, int64_t n)
 {
@@ -495,6 +512,24 @@ hipError_t launch_convert(const S *src, D *dst, int64_t n, hipStream_t stream)
 }
 template hipError_t launch_convert<float, double>(const double *, float *, int64_t, hipStream_t);
 template hipError_t launch_convert<double, float>(const float *, double *, int64_t, hipStream_t);
+
+hipError_t launch_perm_copy(const uint32_t *perm, uint32_t *out, int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_perm_copy, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream, perm,
+                       out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_column(const double *in, const uint32_t *perm, double *out, int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_gather_column, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream,
+                       in, perm, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream)
 {

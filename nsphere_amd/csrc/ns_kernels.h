@@ -183,6 +183,9 @@ template <typename S>
 hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
                                S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);
 hipError_t launch_inverse_map(const uint32_t *id, int32_t *inv, int64_t n, hipStream_t stream);
+// out[i] = perm[i] without kDeferBit; out[i] = in[perm[i]] (kDeferBit ignored).
+hipError_t launch_perm_copy(const uint32_t *perm, uint32_t *out, int64_t n, hipStream_t stream);
+hipError_t launch_gather_column(const double *in, const uint32_t *perm, double *out, int64_t n, hipStream_t stream);
 // Snapshot path (ns_io.hip): all_particle_data records in id order (16 bytes
 // each) and a gather of (r, v, ell, position) by id through inv.
 hipError_t launch_apd_records(const void *r, const void *v, const void *l, bool f32, const uint32_t *id, int64_t n,

@@ -4,10 +4,11 @@
  *
  *   dropin_driver <in.bin> <out.bin> <dt> <nsteps>
  *
- * in.bin / out.bin: int64 npts, then npts doubles each of r, v_r, ell, id.
- * Runs the reference's initial exact sort (sort_particles_with_alg with
- * "quadsort", nsphere.c:5339-5342), then nsteps method-5 steps, and writes
- * the particles and the last inverse map (npts int32) out.
+ * in.bin / out.bin: int64 npts, then npts doubles each of r, v_r, ell, id,
+ * mu (particles[0..4]).  Runs the reference's initial exact sort
+ * (sort_particles_with_alg with "quadsort", nsphere.c:5339-5342), then nsteps
+ * method-5 steps, and writes the five columns and the last inverse map (npts
+ * int32) out.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -33,7 +34,7 @@ int main(int argc, char **argv)
         if (!particles[c])
             return 5;
     }
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 5; c++)
         if (fread(particles[c], sizeof(double), (size_t)n, f) != (size_t)n)
             return 6;
     fclose(f);
@@ -50,7 +51,7 @@ int main(int argc, char **argv)
     if (!f)
         return 7;
     fwrite(&n, sizeof n, 1, f);
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 5; c++)
         fwrite(particles[c], sizeof(double), (size_t)n, f);
     fwrite(inverse_map, sizeof(int), (size_t)n, f);
     fclose(f);

@@ -162,3 +162,42 @@ def test_masses_rejected(ns):
                 ctx.set_masses(bad)
         with pytest.raises(ns.NsError):
             ctx.enclosed_mass()  # none set
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_upload_after_odd_steps_keeps_masses_by_position(ns, precision):
+    """After set_masses and an odd number of steps the masses sit in the
+    second state buffer; an upload must carry them to the first (masses stay
+    by position), so the next step equals a fresh context given the same
+    masses in that order."""
+    n = 60_000
+    rng = np.random.default_rng(77)
+    m = rng.uniform(0.5, 1.5, n) / n
+    r, v, ell, ids = ns.generate_ic(0, n, 71, 100.0, 0.05, 1e-3)
+    state2 = ns.generate_ic(0, n, 72, 100.0, 0.05, 1e-3)
+    p = ns.unit_params()
+    with ns.GpuContext(n) as a:
+        a.set_option(ns.NS_OPT_PRECISION, precision)
+        a.set_option(ns.NS_OPT_INTEGRATOR, ns.FORM_BINNED)
+        a.upload(r, v, ell, ids)
+        a.set_masses(m)
+        a.step(p)
+        a.sync()
+        id1 = a.download()[3]
+        a.upload(*state2)
+        a.step(p)
+        a.sync()
+        got = a.download()
+        ka, wa = a.energy(p)
+    with ns.GpuContext(n) as b:
+        b.set_option(ns.NS_OPT_PRECISION, precision)
+        b.set_option(ns.NS_OPT_INTEGRATOR, ns.FORM_BINNED)
+        b.upload(*state2)
+        b.set_masses(m[id1])  # ids were 0..n-1 at set_masses: position k then held mass m[id1[k]]
+        b.step(p)
+        b.sync()
+        want = b.download()
+        kb, wb = b.energy(p)
+    for x, y, what in zip(got, want, ("r", "v", "ell", "id")):
+        assert_bits(x, y, what)
+    assert (ka, wa) == (kb, wb)
