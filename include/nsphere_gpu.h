@@ -175,6 +175,23 @@ int ns_gpu_permutation(ns_gpu_ctx *ctx, uint32_t *host_out);
  * nsphere.c:8613-8647) reordered on the device by that same permutation:
  * out[k] = in[perm[k]].  in and out may be the same array (npts doubles). */
 int ns_gpu_permute_column(ns_gpu_ctx *ctx, const double *in, double *out);
+/* Allocation-free radial sort.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * semantics of ns_gpu_sort_particles (stable on radius, ties in input order,
+ * the same kernels) on arrays the caller owns, without a context: device
+ * arrays r, v, ell (float64) and id of n particles in, their radius order out
+ * in r_out, v_out, ell_out, id_out (which must not overlap any other array).
+ * The workspace is a static device array of the library sized for
+ * NS_SORT_NOALLOC_MAX particles (under 32 MiB, held by every GPU the library
+ * runs on until the process ends); a larger n is refused (NS_ERR_ARG).  All
+ * calls on one GPU share that array, so calls that overlap in time on one
+ * GPU race.  Queued on stream (a hipStream_t, NULL for the null stream) of
+ * the current device; returns without waiting.  Calls no allocator. */
+#define NS_SORT_NOALLOC_MAX (1 << 20)
+int ns_gpu_sort_particles_noalloc(const double *r, const double *v, const double *ell, const uint32_t *id,
+                                  double *r_out, double *v_out, double *ell_out, uint32_t *id_out, int64_t n,
+                                  void *stream);
 /* inverse_map[id] = rank for the current order (host_out may be NULL to
  * keep it on the device only). */
 int ns_gpu_inverse_map(ns_gpu_ctx *ctx, int32_t *host_out);

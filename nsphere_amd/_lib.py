@@ -110,6 +110,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_inverse_map": (i32, [vp, vp]),
         "ns_gpu_permutation": (i32, [vp, vp]),
         "ns_gpu_permute_column": (i32, [vp, vp, vp]),
+        "ns_gpu_sort_particles_noalloc": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
         "ns_gpu_integrate": (i32, [vp, c.POINTER(StepParams)]),
         "ns_gpu_step": (i32, [vp, c.POINTER(StepParams)]),
         "ns_gpu_run": (i32, [vp, c.POINTER(StepParams), i64]),
