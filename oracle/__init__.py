@@ -96,6 +96,10 @@ def lib(fast: bool = False):
         L.orc_micro_leapfrog.argtypes = [i32, i32, dbl, dbl, dbl, dbl, i32, i32, P, pd, pd]
         L.orc_adaptive_full_leap.restype = i32
         L.orc_adaptive_full_leap.argtypes = [i32, i32, dbl, dbl, dbl, dbl, P, pd, pd]
+        L.orc_micro_lc.restype = None
+        L.orc_micro_lc.argtypes = [i32, i32, dbl, dbl, dbl, i64, dbl, P, dbl, pd, pd, pd]
+        L.orc_single_tau_step_lc.restype = i32
+        L.orc_single_tau_step_lc.argtypes = [i32, i32, dbl, dbl, dbl, dbl, P, dbl, pd, pd, pd]
         L.orc_adaptive_full_lc.restype = i32
         L.orc_adaptive_full_lc.argtypes = [i32, i32, dbl, dbl, dbl, dbl, P, pd, pd,
                                            ctypes.POINTER(ctypes.c_int64)]
@@ -239,6 +243,22 @@ def adaptive_lc(i, npts, r, v, ell, dt, p: OrcParams):
     lv = lib().orc_adaptive_full_lc(i, npts, r, v, ell, dt, ctypes.byref(p), ctypes.byref(ro),
                                     ctypes.byref(vo), ctypes.byref(nt))
     return ro.value, vo.value, lv, nt.value
+
+
+def micro_lc(i, npts, rho, v, t, substeps, h_tau, ell, p: OrcParams):
+    """doMicroLeviCivita (nsphere.c:1990-2036): (rho, v, t) after substeps KDK steps in tau."""
+    ro, vo, to = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().orc_micro_lc(i, npts, rho, v, t, substeps, h_tau, ctypes.byref(p), ell, ctypes.byref(ro),
+                       ctypes.byref(vo), ctypes.byref(to))
+    return ro.value, vo.value, to.value
+
+
+def single_tau_lc(i, npts, rho, v, t, h_tau, ell, p: OrcParams):
+    """doSingleTauStepAdaptiveLeviCivita (nsphere.c:2077-2186): (rho, v, t, level record)."""
+    ro, vo, to = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lv = lib().orc_single_tau_step_lc(i, npts, rho, v, t, h_tau, ctypes.byref(p), ell, ctypes.byref(ro),
+                                      ctypes.byref(vo), ctypes.byref(to))
+    return ro.value, vo.value, to.value, lv
 
 
 def micro_leapfrog(i, npts, r, v, ell, h, N, sub, p: OrcParams):

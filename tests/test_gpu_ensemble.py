@@ -69,14 +69,30 @@ def test_segmented_sort_matches_oracle_per_halo(ns, threads, items, precision):
         off += s
 
 
+def f32(a):
+    return a.astype(np.float32).astype(np.float64)
+
+
 @pytest.mark.parametrize("form", [0, 1])
-def test_ensemble_steps_equal_oracle_per_halo(ns, form):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ensemble_steps_equal_oracle_per_halo(ns, form, precision):
+    """Float64 state: the oracle halo by halo.  Float32 state (the mixed
+    precision of the ensemble workload): the oracle on the float32-rounded
+    state with r and v rounded to float32 after every step, bit for bit."""
     sizes = [1000, 23_457, 4096, 51_000, 777, 12_345]
     masses = [1.0, 0.5, 2.0, 1.0, 3.0, 0.25]
     r, v, ell, ids = ensemble(ns, sizes, 500)
+    if precision:
+        r, v, ell = f32(r), f32(v), f32(ell)
+        off = 0
+        for s_ in sizes:  # float32 rounding can reorder a halo's radii: re-sort each halo
+            q = oracle.sort_perm(r[off:off + s_]) + off
+            r[off:off + s_], v[off:off + s_], ell[off:off + s_], ids[off:off + s_] = r[q], v[q], ell[q], ids[q]
+            off += s_
     n = r.size
     p = ns.unit_params()
     with ns.GpuContext(n) as ctx:
+        ctx.set_option(ns.NS_OPT_PRECISION, precision)
         ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
         ctx.set_halos(sizes, masses)
         ctx.upload(r, v, ell, ids)
@@ -94,6 +110,8 @@ def test_ensemble_steps_equal_oracle_per_halo(ns, form):
         po.halo_mass = m
         for _ in range(3):
             lv, _ = oracle.step(hr, hv, hl, hi, po)
+            if precision:
+                hr[:], hv[:] = f32(hr), f32(hv)
         assert_bits(gid[off:off + s], hi + np.uint32(off), f"halo {h} id")
         assert_bits(gr[off:off + s], hr, f"halo {h} r")
         assert_bits(gv[off:off + s], hv, f"halo {h} v")

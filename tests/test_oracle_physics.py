@@ -210,3 +210,122 @@ def test_strict_and_fast_builds_agree():
     oracle.step(*b, p, fast=True)
     assert np.array_equal(a[3], b[3])
     assert np.max(np.abs(a[0] - b[0]) / a[0]) < 1e-6
+
+
+# ---- Levi-Civita branch: its quirks pinned by mathematics ----------------
+#
+# In tau (dt = rho^2 dtau, rho = sqrt r) the motion is
+#   drho/dtau = rho v / 2,  dv/dtau = -G M_i / rho^2 + ell^2 / rho^4,  dt/dtau = rho^2
+# (nsphere.c:1757-1787).  The reference's micro-step drifts rho with rho at
+# the start of the step (nsphere.c:2017), so a pass of M substeps has a
+# first-order error in rho, e(M) ~ K / M.  With no force (identity gravity,
+# ell = 0) a pass has a closed form: v stays v0, rho_k = rho0 q^k with
+# q = 1 + a / 2, a = dtau v0, and t = sum dtau rho_k^2 (1 + a / 4)^2.
+
+
+def lc_pass_closed_form(rho0, v0, h, m):
+    """(rho, t) of a force-free pass of m substeps over h."""
+    dtau = h / m
+    a = dtau * v0
+    q = 1.0 + 0.5 * a
+    rho = rho0 * q ** m
+    t = dtau * rho0 * rho0 * (1.0 + 0.25 * a) ** 2 * (q ** (2 * m) - 1.0) / (q * q - 1.0)
+    return rho, t
+
+
+def exact_tau(rho0, v0, ell, gm, h):
+    def f(_, y):
+        rho, v, _t = y
+        return [0.5 * rho * v, -gm / rho ** 2 + ell * ell / rho ** 4, rho * rho]
+    s = solve_ivp(f, (0.0, h), [rho0, v0, 0.0], method="DOP853", rtol=1e-13, atol=1e-18)
+    return s.y[:, -1]
+
+
+def test_lc_micro_pass_closed_form():
+    rho0, v0, h = 0.3, -0.8, 0.6
+    p = oracle.params(identity_gravity=1)
+    for m in (1, 3, 5, 33, 1000):
+        rho, v, t = oracle.micro_lc(3, 10, rho0, v0, 0.0, m, h, 0.0, p)
+        rx, tx = lc_pass_closed_form(rho0, v0, h, m)
+        assert rho == pytest.approx(rx, rel=1e-13)
+        assert t == pytest.approx(tx, rel=1e-13)
+        assert v == v0
+
+
+def test_lc_single_tau_outputs_closed_form():
+    """Level 0 (tolerance 1): the coarse pass has 3 substeps, the fine 5, and
+    out_type 2 returns 4F - 3C on rho and t (nsphere.c:2120-2157)."""
+    rho0, v0, h = 0.3, -0.8, 0.6
+    C = lc_pass_closed_form(rho0, v0, h, 3)
+    F = lc_pass_closed_form(rho0, v0, h, 5)
+    R = (4.0 * F[0] - 3.0 * C[0], 4.0 * F[1] - 3.0 * C[1])
+    for ot, want in ((0, C), (1, F), (2, R)):
+        p = oracle.params(identity_gravity=1, out_type=ot, radius_tol=1.0, velocity_tol=1.0)
+        rho, v, t, lv = oracle.single_tau_lc(3, 10, rho0, v0, 0.0, h, 0.0, p)
+        assert lv == 0
+        assert rho == pytest.approx(want[0], rel=1e-13)
+        assert t == pytest.approx(want[1], rel=1e-13)
+        assert v == pytest.approx(v0, abs=4 * np.spacing(abs(v0)))
+    # the three candidates are far apart at this step size
+    assert abs(R[0] - F[0]) > 1e-3 * R[0] and abs((4 * F[0] - C[0]) / 3 - R[0]) > 1e-3 * R[0]
+
+
+@pytest.mark.parametrize("r0,v0,ell,i,h", [(0.04, -0.3, 0.02, 100, 0.02), (0.1, -0.5, 0.05, 300, 0.01)])
+def test_lc_single_tau_error_structure_pins_4f_minus_3c(r0, v0, ell, i, h):
+    """Against an independent DOP853 solution of the tau ODE over one tau
+    step: the first-order passes of 3 and 5 substeps err in ratio
+    e_F / e_C = 3/5, and 4F - 3C then errs by -3/5 e_C.  F alone (+3/5), C
+    (1) or the textbook (4F - C)/3 (+7/15) would fail."""
+    npts = 1000
+    rho0 = math.sqrt(r0)
+    outs = {}
+    for ot in (0, 1, 2):
+        p = oracle.params(out_type=ot, radius_tol=1.0, velocity_tol=1.0)
+        outs[ot] = oracle.single_tau_lc(i, npts, rho0, v0, 0.0, h, ell, p)
+        assert outs[ot][3] == 0
+    rx = exact_tau(rho0, v0, ell, gm_rank(i, npts, oracle.params()), h)[0]
+    eC, eF, eR = (outs[k][0] - rx for k in (0, 1, 2))
+    assert abs(eC) > 1e-9 * rho0
+    assert 0.55 < eF / eC < 0.65
+    assert -0.7 < eR / eC < -0.5
+
+
+def test_lc_nonconvergence_falls_back_to_fine_pass_at_doubled_n():
+    """No level converges up to max_subdiv = 4 (N = 1, 2, 4): the result is
+    the fine pass at the next N, 4 * 8 + 1 = 33 substeps (nsphere.c:2168-2185),
+    flagged unconverged at level 3."""
+    rho0, v0, h = 0.3, -0.8, 0.6
+    p = oracle.params(identity_gravity=1, radius_tol=1e-30, velocity_tol=1.0, max_subdiv=4)
+    rho, v, t, lv = oracle.single_tau_lc(3, 10, rho0, v0, 0.0, h, 0.0, p)
+    assert lv == (3 | oracle.LVL_NOCONV)
+    rx, tx = lc_pass_closed_form(rho0, v0, h, 33)
+    assert rho == pytest.approx(rx, rel=1e-13)
+    assert t == pytest.approx(tx, rel=1e-13)
+    assert v == v0
+    r17, _ = lc_pass_closed_form(rho0, v0, h, 17)  # the last tried fine pass is distinguishable
+    assert abs(r17 - rx) > 1e-4 * rx
+
+
+@pytest.mark.parametrize("r0,v0,nt", [(0.01, -0.5, 10), (0.02, 0.7, 10), (0.005, -0.3, 3)])
+def test_lc_overshoot_interpolation_on_a_radial_line(r0, v0, nt):
+    """With no force a radial orbit is the straight line r0 + v0 t.  The tau
+    loop stops at the step that passes t = dt and interpolates rho and v
+    linearly in t (nsphere.c:2294-2308): the result lies on the line to the
+    scheme's tolerance, while the states on either side of dt miss it by
+    about 1e-3."""
+    dt = 1e-3 * (2.0 if v0 > 0 else 1.0)
+    p = oracle.params(identity_gravity=1, dt=dt, N_taumin=nt)
+    r, v, lv, ntau = oracle.adaptive_lc(5, 100, r0, v0, 0.0, dt, p)
+    assert lv & oracle.LVL_LC and ntau >= 2
+    line = r0 + v0 * dt
+    assert abs(r - line) < 1e-5 * line
+    assert v == pytest.approx(v0, abs=4 * np.spacing(abs(v0)))
+    # the tau step that crossed dt, from the state before it
+    rho, vv, t = math.sqrt(r0), v0, 0.0
+    dtau = dt / (2.0 * r0 * nt)
+    while True:
+        rn, vn, tn, _ = oracle.single_tau_lc(5, 100, rho, vv, t, dtau, 0.0, p)
+        if tn > dt:
+            break
+        rho, vv, t = rn, vn, tn
+    assert abs(rho * rho - line) > 1e-4 * line and abs(rn * rn - line) > 1e-4 * line
