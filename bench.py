@@ -31,32 +31,44 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 (vector and matrix) peak
 FLOPS_PER_FORCE_EVAL = 16     # FP64 flops per force evaluation + kick/drift, a divide counted as one
-# canonical HBM bytes per particle-step of the whole step (8-bit digits, uint32
-# payload, keys from the integrate epilogue): float64 state / float32 state
-STEP_BYTES = {"f64": 304, "f32": 124}
+# HBM bytes per particle-step of the whole step.  Canonical (the survey's
+# model: 8-bit digits over keys as wide as the state, so 8 passes of 64-bit
+# keys for float64): float64 304, float32 124.  This build's algorithm: the
+# float64 sort runs 4 passes over the 32-bit high halves of the keys and one
+# fix-up pass over runs of equal high halves (92 B instead of 200 B), so a
+# float64 step moves 196 B.
+CANONICAL_STEP_BYTES = {"f64": 304, "f32": 124}
+STEP_BYTES = {"f64": 196, "f32": 124}
 
 
 def bytes_per_particle(state_bytes: int) -> dict:
     """Algorithmic HBM bytes per particle of each kernel family of one step,
-    for float64 (8) or float32 (4) state: keys as wide as the state, uint32
-    permutation and ids, 8-bit digits (so state_bytes passes)."""
+    for float64 (8) or float32 (4) state, uint32 permutation and ids, 8-bit
+    digits.  Both sorts pass 4 times over 32-bit keys: the float32 radius
+    keys, or the high halves of the float64 keys, whose last pass also writes
+    its keys for the fix-up that orders runs of equal high halves by the low
+    halves (read high half 4 and permutation 4, gather the low half 4, write
+    the permutation 4)."""
     s = float(state_bytes)
-    passes = state_bytes
-    first = s + (s + 4)              # read key; write key + payload
-    middle = 2 * (s + 4)             # read and write key + payload
-    last = (s + 4) + 4               # read key + payload; write the permutation
+    wide = state_bytes == 8
+    first = 4 + (4 + 4)              # read key; write key + payload
+    middle = 2 * (4 + 4)             # read and write key + payload
+    last = (4 + 4) + (8 if wide else 4)  # read key + payload; write (key +) permutation
     # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
     integrate = (4 + 3 * s + 4) + (3 * s + 4 + s + 1)
-    return {
-        "sort_histogram": s,                                     # read the key (onesweep)
-        "sort_upsweep": s,                                       # read the key (reduce-then-scan)
-        "sort_scatter": (first + (passes - 2) * middle + last) / passes,
+    out = {
+        "sort_histogram": 4,                                     # read the key (onesweep)
+        "sort_upsweep": 4,                                       # read the key (reduce-then-scan)
+        "sort_scatter": (first + 2 * middle + last) / 4,
         "integrate": integrate,                                  # plain form
         "integrate_bulk": integrate,                             # binned forms: every particle's gather + write
-        "radius_keys": 2 * s,
+        "radius_keys": s + s,                                    # read r; write the key
         # read perm 4 and the mass by it; write the mass in rank order and M_enc (float64)
         "mass_scan": 4 + 2 * s + 8,
     }
+    if wide:
+        out["sort_fixup"] = 16
+    return out
 
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
@@ -75,7 +87,8 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 # Kernel families timed with HIP events inside the timed region (two events
 # per launch): the integrate kernels and the sort's scatter, which the
 # rooflines use.  The other families are timed over a short run after it.
-ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", "sort_scatter"]
+SORT_KERNELS = ["sort_upsweep", "sort_scan", "sort_scatter", "sort_fixup"]
+ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", *SORT_KERNELS]
 AFTER_STEPS = 20
 # One wave alone on an idle MI355X running the kernel's fast Levi-Civita
 # micro-step (reciprocal, one Newton step, force, velocity and time sums: nine
@@ -449,6 +462,32 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
     return None
 
 
+def sort_roofline(run: dict, n: int) -> dict | None:
+    """The whole radius sort of one step -- tile counts, their scan, the
+    scatter passes and (float64 keys) the run fix-up -- as one HBM-bound unit:
+    the algorithmic bytes of all its launches over their summed durations
+    (the scan moves a few MB of tile counts; counted as zero)."""
+    stats = run["stats"]
+    if "sort_scatter" not in stats:
+        return None
+    steps = run["stats"].get("sort_fixup", (None,))[0] or stats["sort_scatter"][0] / 4
+    total_ms, total_bytes, parts = 0.0, 0.0, {}
+    for k in SORT_KERNELS:
+        if k not in stats:
+            continue
+        launches, ms = stats[k]
+        b = run["bpp"].get(k, 0.0) * n * launches
+        total_ms += ms
+        total_bytes += b
+        parts[k] = {"launches": launches, "avg_us": round(ms / launches * 1e3, 2),
+                    "bytes_per_particle": run["bpp"].get(k, 0.0)}
+    achieved = total_bytes / (total_ms * 1e-3) / 1e9
+    return {"kernel": "sort (all passes)", "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "us_per_sort": round(total_ms / steps * 1e3, 1),
+            "bytes_per_particle_per_sort": round(total_bytes / n / steps, 1), "kernels": parts}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -573,8 +612,10 @@ def main() -> int:
                    "graph": args.graph, "overlap": not args.no_overlap and args.form != "plain"},
         "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
-        "roofline_sort": roofline_for(chosen, n, "sort_scatter"),
+        "roofline_sort": sort_roofline(chosen, n),
+        "roofline_scatter": roofline_for(chosen, n, "sort_scatter"),
         "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
+                          "canonical_bytes_per_particle_step": CANONICAL_STEP_BYTES[args.precision],
                           "achieved": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9 / PEAK_HBM_GBS, 4)},

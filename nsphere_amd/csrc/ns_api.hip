@@ -113,7 +113,7 @@ const char *kernel_name(int id)
                                          "integrate_deep", "mass_scan", "mlp_binning",
                                          "snapshot_records", "potential", "density",
                                          "rank_files", "integrate_late", "overlap_plan",
-                                         "overlap_place"};
+                                         "overlap_place", "sort_fixup"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -629,7 +629,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
         return rc;
     }
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
-    HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->stream));
+    HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->stream));
     return NS_OK;
 }
 
@@ -723,6 +723,7 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemset(c->budget_hits, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->sw.counters, 0, 64 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->work, 0, work_bytes(c));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {

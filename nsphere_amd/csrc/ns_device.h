@@ -68,6 +68,23 @@ __host__ __device__ inline uint32_t radius_key(float r)
     return (u >> 31) ? ~u : (u | 0x80000000u);
 }
 
+// Radius keys of n particles in a key buffer.  A 32-bit key (float32 radii)
+// is stored at [i].  A 64-bit key (float64 radii) is stored as two 32-bit
+// planes, the high half at [i] and the low half at [n + i] (32-bit words):
+// the radix passes sort the high plane as 32-bit keys and the runs of equal
+// high halves are then ordered by the low plane, which the passes leave
+// untouched.
+__device__ inline void store_key(void *keys, int64_t n, int64_t i, uint64_t k)
+{
+    uint32_t *w = static_cast<uint32_t *>(keys);
+    w[i] = (uint32_t)(k >> 32);
+    w[n + i] = (uint32_t)k;
+}
+__device__ inline void store_key(void *keys, int64_t, int64_t i, uint32_t k)
+{
+    static_cast<uint32_t *>(keys)[i] = k;
+}
+
 template <typename S>
 struct KeyOf;
 template <>
@@ -141,8 +158,10 @@ void seg_table_release(SegTable &t);
 // Launchers (ns_sort.hip).  All asynchronous on `stream`.
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream);
 hipError_t launch_radius_keys(const float *r, uint32_t *keys, int64_t n, hipStream_t stream);
-// Sorts the key_bits-wide (32 or 64) keys in keys[0], stably.  On return
-// perm_out[k] = input position of rank k (perm_out aliases no vals buffer).
+// Sorts the key_bits-wide keys in keys[0], stably: 32-bit keys, or 64-bit
+// keys in two planes (store_key).  On return perm_out[k] = input position of
+// rank k (perm_out aliases no vals buffer).  The 64-bit sort uses both key
+// buffers, both payload buffers, the tile-count buffer and counters[48].
 struct Profiler;
 // One stable pass on the low byte of 32-bit keys in keys[0], ignoring any
 // segments (reduce-then-scan only): the order of a small integer label.
@@ -179,6 +198,7 @@ enum KernelId : int {
     K_INT_LATE,
     K_MERGE,
     K_PLACE,
+    K_SORT_FIXUP,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -93,7 +93,7 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
     static_cast<S *>(a.r1)[i] = rs;
     static_cast<S *>(a.v1)[i] = (S)v;
     if (a.keys)
-        static_cast<typename KeyOf<S>::type *>(a.keys)[i] = radius_key(rs);
+        store_key(a.keys, a.n, i, radius_key(rs));
     if (a.lvl)
         a.lvl[i] = (uint8_t)rec;
     if (hit)
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
         static_cast<S *>(a.l1)[i] = static_cast<const S *>(a.l0)[src];
         a.id1[i] = a.id0[src];
         if (a.keys)
-            static_cast<typename KeyOf<S>::type *>(a.keys)[i] = ~(typename KeyOf<S>::type)0;
+            store_key(a.keys, a.n, i, ~(typename KeyOf<S>::type)0);
     } else if (valid) {
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);

@@ -178,7 +178,8 @@ hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int
 hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *def_src,
                        const uint32_t *count, hipStream_t stream);
 // Radius keys of the deferred ranks of the last step (replacing the sentinel).
-hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, hipStream_t stream);
+hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
+                            hipStream_t stream);
 template <typename S>
 hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
                                S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);

@@ -493,12 +493,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_flag(OverlapBufs o)
 
 template <typename S>
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_fix_keys(OverlapBufs o, const S *__restrict__ r,
-                                                                   typename KeyOf<S>::type *__restrict__ keys)
+                                                                   void *__restrict__ keys, int64_t n)
 {
     const uint32_t j = threadIdx.x;
     if (j < o.ctl[1]) {
         const uint32_t t = o.def_pos[j];
-        keys[t] = radius_key(r[t]);
+        store_key(keys, n, t, radius_key(r[t]));
     }
 }
 
@@ -567,14 +567,15 @@ hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int
     return hipGetLastError();
 }
 
-hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, hipStream_t stream)
+hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
+                            hipStream_t stream)
 {
     if (f32)
         hipLaunchKernelGGL(k_overlap_fix_keys<float>, dim3(1), dim3(kPlanThreads), 0, stream, o,
-                           static_cast<const float *>(r), static_cast<uint32_t *>(keys));
+                           static_cast<const float *>(r), keys, n);
     else
         hipLaunchKernelGGL(k_overlap_fix_keys<double>, dim3(1), dim3(kPlanThreads), 0, stream, o,
-                           static_cast<const double *>(r), static_cast<uint64_t *>(keys));
+                           static_cast<const double *>(r), keys, n);
     return hipGetLastError();
 }
 

@@ -104,11 +104,11 @@ __device__ inline uint32_t digit_exclusive_scan(uint32_t x, uint32_t *s_wave)
 }
 
 template <typename S>
-__global__ void k_radius_keys(const S *__restrict__ r, typename KeyOf<S>::type *__restrict__ keys, int64_t n)
+__global__ void k_radius_keys(const S *__restrict__ r, void *__restrict__ keys, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
-        keys[i] = radius_key(r[i]);
+        store_key(keys, n, i, radius_key(r[i]));
 }
 
 // Id tiebreak: the ids as 32-bit keys, then the radius keys gathered through
@@ -141,12 +141,14 @@ __global__ void k_fill_u64(uint64_t *__restrict__ p, int64_t n, uint64_t v)
 }
 
 __global__ void k_sort_zero(uint32_t *__restrict__ hist, int nhist, uint32_t *__restrict__ counters,
-                            int ncounters)
+                            int ncounters, uint32_t *__restrict__ zero_word)
 {
     for (int i = threadIdx.x; i < nhist; i += blockDim.x)
         hist[i] = 0;
     for (int i = threadIdx.x; i < ncounters; i += blockDim.x)
         counters[i] = 0;
+    if (zero_word && threadIdx.x == 0)
+        *zero_word = 0;
 }
 
 // All digit histograms in one read of the keys.  Each thread takes
@@ -224,11 +226,14 @@ __global__ __launch_bounds__(kHistThreads) void k_sort_histogram(const KeyT *__r
 template <typename KeyT, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_rts_upsweep(const KeyT *__restrict__ keys, int64_t n, int shift,
                                                          uint32_t *__restrict__ counts, int64_t tiles,
-                                                         const int64_t *__restrict__ tile_start)
+                                                         const int64_t *__restrict__ tile_start,
+                                                         uint32_t *__restrict__ zero_word)
 {
     constexpr int WAVES = THREADS / 64;
     __shared__ uint32_t s_cnt[WAVES][kRadix];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (zero_word && blockIdx.x == 0 && tid == 0)
+        *zero_word = 0;  // a counter of a later kernel of the same sort
     for (int i = tid; i < WAVES * kRadix; i += THREADS)
         (&s_cnt[0][0])[i] = 0;
     __syncthreads();
@@ -581,7 +586,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_scatter(
 
 template <typename KeyT, int THREADS, int ITEMS, bool FIRST, bool LAST>
 hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, hipStream_t stream,
-                       Profiler *prof)
+                       Profiler *prof, uint32_t *zero_word)
 {
     constexpr int TILE = THREADS * ITEMS;
     const int64_t tiles = (n + TILE - 1) / TILE;
@@ -616,7 +621,7 @@ hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int 
     if (prof)
         prof->begin(K_SORT_UPSWEEP);
     hipLaunchKernelGGL((k_rts_upsweep<KeyT, THREADS, ITEMS>), dim3((unsigned)ntiles), dim3(THREADS), 0, stream,
-                       kin, n, shift, counts, ntiles, sv.tile_start);
+                       kin, n, shift, counts, ntiles, sv.tile_start, zero_word);
     if (prof)
         prof->end(K_SORT_UPSWEEP);
     if ((e = hipGetLastError()) != hipSuccess)
@@ -641,42 +646,48 @@ hipError_t launch_pass(SortWork &w, int src, uint32_t *vals_out, int64_t n, int 
     return hipGetLastError();
 }
 
+// have_vals: the first pass reads its payload from vals[src] instead of
+// generating the input positions; keep_keys: the last pass writes its keys
+// too (the sorted keys stay in keys[src ^ 1]); zero_word: cleared by the
+// first pass for a later kernel of the same sort.
+struct PassFlags {
+    bool have_vals;
+    bool keep_keys;
+    uint32_t *zero_word;
+};
+
 template <typename KeyT, int THREADS, int ITEMS>
 hipError_t launch_pass_cfg(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream, Profiler *prof, bool have_vals)
+                           hipStream_t stream, Profiler *prof, PassFlags f)
 {
-    // have_vals: the first pass reads its payload from vals[src] instead of
-    // generating the input positions
-    if (have_vals && pass == 0) {
-        if (passes == 1)
-            return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream, prof);
-        return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream, prof);
-    }
-    if (passes == 1)
-        return launch_pass<KeyT, THREADS, ITEMS, true, true>(w, src, vals_out, n, pass, stream, prof);
-    if (pass == 0)
-        return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream, prof);
-    if (pass == passes - 1)
-        return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream, prof);
-    return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream, prof);
+    const bool first = pass == 0 && !f.have_vals;
+    const bool last = pass == passes - 1 && !f.keep_keys;
+    uint32_t *zw = pass == 0 ? f.zero_word : nullptr;
+    if (first && last)
+        return launch_pass<KeyT, THREADS, ITEMS, true, true>(w, src, vals_out, n, pass, stream, prof, zw);
+    if (first)
+        return launch_pass<KeyT, THREADS, ITEMS, true, false>(w, src, vals_out, n, pass, stream, prof, zw);
+    if (last)
+        return launch_pass<KeyT, THREADS, ITEMS, false, true>(w, src, vals_out, n, pass, stream, prof, zw);
+    return launch_pass<KeyT, THREADS, ITEMS, false, false>(w, src, vals_out, n, pass, stream, prof, zw);
 }
 
 template <typename KeyT>
 hipError_t launch_pass_any(SortWork &w, int src, uint32_t *vals_out, int64_t n, int pass, int passes,
-                           hipStream_t stream, Profiler *prof, bool have_vals)
+                           hipStream_t stream, Profiler *prof, PassFlags f)
 {
     const SortConfig c = w.config;
     if (c.threads == 512 && c.items == 16)
-        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+        return launch_pass_cfg<KeyT, 512, 16>(w, src, vals_out, n, pass, passes, stream, prof, f);
     if (c.threads == 256 && c.items == 24)
-        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+        return launch_pass_cfg<KeyT, 256, 24>(w, src, vals_out, n, pass, passes, stream, prof, f);
     if (c.threads == 512 && c.items == 12)
-        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+        return launch_pass_cfg<KeyT, 512, 12>(w, src, vals_out, n, pass, passes, stream, prof, f);
     if (c.threads == 512 && c.items == 8)
-        return launch_pass_cfg<KeyT, 512, 8>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+        return launch_pass_cfg<KeyT, 512, 8>(w, src, vals_out, n, pass, passes, stream, prof, f);
     if (c.threads == 1024 && c.items == 6)
-        return launch_pass_cfg<KeyT, 1024, 6>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
-    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof, have_vals);
+        return launch_pass_cfg<KeyT, 1024, 6>(w, src, vals_out, n, pass, passes, stream, prof, f);
+    return launch_pass_cfg<KeyT, 256, 16>(w, src, vals_out, n, pass, passes, stream, prof, f);
 }
 
 // (Re)builds the device tile table of the segments for tile size `tile`:
@@ -722,8 +733,10 @@ hipError_t seg_table_build(SegTable &t, int tile, int64_t n)
 
 template <typename KeyT>
 hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof,
-                      int kPasses = (int)sizeof(KeyT), bool use_segments = true, bool have_vals = false)
+                      int kPasses = (int)sizeof(KeyT), bool use_segments = true, bool have_vals = false,
+                      bool keep_keys = false, uint32_t *zero_word = nullptr)
 {
+    const PassFlags flags{have_vals, keep_keys, zero_word};
     hipError_t e;
     // the segment table is set aside for an unsegmented sort of this call
     std::vector<int64_t> parked;
@@ -760,7 +773,7 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
         if (prof)
             prof->begin(K_SORT_ZERO);
         hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, stream, w.hist, kPasses * kRadix, w.counters,
-                           kPasses);
+                           kPasses, zero_word);
         if (prof)
             prof->end(K_SORT_ZERO);
         if ((e = hipGetLastError()) != hipSuccess)
@@ -783,11 +796,328 @@ hipError_t sort_typed(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t st
     int src = 0;
     for (int p = 0; p < kPasses; p++) {
         uint32_t *vout = (p == kPasses - 1) ? perm_out : w.vals[src ^ 1];
-        if ((e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream, prof, have_vals)) != hipSuccess)
+        if ((e = launch_pass_any<KeyT>(w, src, vout, n, p, kPasses, stream, prof, flags)) != hipSuccess)
             return e;
         src ^= 1;
     }
     return hipSuccess;
+}
+
+// ------------------------------------------------- 64-bit keys in two planes
+//
+// A 64-bit radius key is stored as a high and a low 32-bit plane (store_key).
+// The stable sort on the whole key is the stable sort on the high plane (four
+// 8-bit passes) followed by a stable sort on the low plane within each run of
+// equal high halves: runs are contiguous after the first sort and keep their
+// input order, so ordering a run by (low half, position) orders it by (key,
+// input position).  Between steps the radii of N = 1.2e7 particles share
+// their high half (sign, exponent and 20 mantissa bits) in runs of about
+// three, so the second stage is one light kernel instead of four more
+// passes.  Runs that a block cannot hold (longer than its look-ahead, or any
+// length on adversarial input) go to a kernel that sorts each on one
+// workgroup.
+
+constexpr int kLongThreads = 1024;
+constexpr int kLongBlocks = 64;
+constexpr int kLongRunCounter = 48;  // SortWork::counters word: runs left to k_split_long_runs
+
+// First index s of seg_start[0..nseg) with seg_start[s] >= x (nseg if none).
+__device__ inline int first_start_at_or_after(const uint32_t *seg_start, int nseg, int64_t x)
+{
+    int lo = 0, hi = nseg;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)seg_start[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Orders the runs of equal high halves that start in a wave's tile.  hs: the
+// high plane in high-half order; perm_hi: the input position of each (the
+// first sort's permutation); lo: the low plane by input position; out: the
+// final permutation.  A run is also cut where a segment starts.
+//
+// Each wave works alone (no workgroup barrier): it owns kFxTile positions and
+// reads one row of 64 more ahead, so that a run starting in its tile and
+// crossing the tile's end is still its own.  Run starts are one 64-bit ballot
+// per row; from them every lane finds the start and the end of its run with
+// bit scans and two uniform tables of the last and next start of each row.
+// Low halves are gathered only for positions that share their run; a run in
+// low-half order already (runs of one, exact ties, the key sentinel of
+// deferred particles) is copied, the others rank each element against their
+// run in LDS.  A run that reaches past the read-ahead row is recorded (its
+// first position) for k_split_long_runs.
+constexpr int kFxRows = 8;
+constexpr int kFxTile = 64 * kFxRows;   // positions whose runs one wave orders
+constexpr int kFxAll = kFxRows + 1;     // rows read, the last one ahead of the tile
+constexpr int kFxWaves = 4;             // independent waves per workgroup
+constexpr int kFxNone = 1 << 30;        // "no run start" in the uniform tables
+
+__global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *__restrict__ hs,
+                                                                 const uint32_t *__restrict__ lo,
+                                                                 const uint32_t *__restrict__ perm_hi,
+                                                                 uint32_t *__restrict__ out, int64_t n,
+                                                                 const uint32_t *__restrict__ seg_start, int nseg,
+                                                                 uint32_t *__restrict__ long_runs,
+                                                                 uint32_t *__restrict__ long_count)
+{
+    __shared__ uint32_t s_l[kFxWaves][kFxAll * 64];
+    __shared__ uint32_t s_uns[kFxWaves][kFxAll * 16];       // bytes by run start: the run has a descent
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t w0 = ((int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * kFxWaves + wv) * kFxTile;
+    if (w0 >= n)
+        return;  // the whole wave
+    uint8_t *uns = reinterpret_cast<uint8_t *>(s_uns[wv]);
+    uint32_t h[kFxAll], p[kFxAll];
+#pragma unroll
+    for (int r = 0; r < kFxAll; r++) {
+        const int64_t j = w0 + r * 64 + lane;
+        h[r] = j < n ? hs[j] : 0u;
+        p[r] = j < n ? perm_hi[j] : 0u;
+    }
+    for (int k = lane; k < kFxAll * 16; k += 64)
+        s_uns[wv][k] = 0;
+    const int64_t after = w0 + kFxAll * 64;  // the position past the rows
+    const uint32_t h_before = w0 > 0 ? hs[w0 - 1] : 0u;
+    const uint32_t h_after = after < n ? hs[after] : 0u;
+    // run starts: bit `lane` of row r; positions past n count as starts
+    uint64_t M[kFxAll + 1];
+    uint32_t carry = h_before;
+#pragma unroll
+    for (int r = 0; r < kFxAll; r++) {
+        const int64_t j = w0 + r * 64 + lane;
+        uint32_t prev = __shfl_up(h[r], 1, 64);
+        if (lane == 0)
+            prev = carry;
+        carry = __shfl(h[r], 63, 64);
+        M[r] = __ballot(j >= n || j == 0 || h[r] != prev);
+    }
+    M[kFxAll] = (after >= n || h_after != carry) ? 1ull : 0ull;
+    if (seg_start) {  // segment starts inside the rows (and right after them)
+        for (int sg = first_start_at_or_after(seg_start, nseg, w0); sg < nseg && (int64_t)seg_start[sg] <= after;
+             sg++) {
+            const int64_t q = (int64_t)seg_start[sg] - w0;
+#pragma unroll
+            for (int r = 0; r <= kFxAll; r++)
+                if ((q >> 6) == r)
+                    M[r] |= 1ull << (q & 63);
+        }
+    }
+    // uniform tables: the last start at or before the end of row r, the first
+    // start at or after the beginning of row r (window positions)
+    int lastS[kFxAll], nextS[kFxAll + 1];
+    int run_last = -1;
+#pragma unroll
+    for (int r = 0; r < kFxAll; r++) {
+        if (M[r])
+            run_last = r * 64 + 63 - __clzll((long long)M[r]);
+        lastS[r] = run_last;
+    }
+    nextS[kFxAll] = (M[kFxAll] & 1) ? kFxAll * 64 : kFxNone;
+#pragma unroll
+    for (int r = kFxAll - 1; r >= 0; r--)
+        nextS[r] = M[r] ? r * 64 + __ffsll((long long)M[r]) - 1 : nextS[r + 1];
+    const uint64_t le = (2ull << lane) - 1;  // bits 0..lane
+    // low halves (shared runs only), run bounds, descents
+    int ra[kFxAll], re[kFxAll];
+    uint32_t lprev_row = 0;
+#pragma unroll
+    for (int r = 0; r < kFxAll; r++) {
+        const int64_t j = w0 + r * 64 + lane;
+        const bool hd = (M[r] >> lane) & 1;
+        const bool next_hd = lane < 63 ? ((M[r] >> (lane + 1)) & 1) : (M[r + 1] & 1);
+        const uint32_t l = (j < n && !(hd && next_hd)) ? lo[p[r]] : 0u;
+        s_l[wv][r * 64 + lane] = l;
+        uint32_t lprev = __shfl_up(l, 1, 64);
+        if (lane == 0)
+            lprev = lprev_row;
+        lprev_row = __shfl(l, 63, 64);
+        const uint64_t mle = M[r] & le, mgt = M[r] & ~le;
+        const int a = mle ? r * 64 + 63 - __clzll((long long)mle) : (r > 0 ? lastS[r - 1] : -1);
+        const int e = mgt ? r * 64 + __ffsll((long long)mgt) - 1 : nextS[r + 1];
+        ra[r] = a;
+        re[r] = e;
+        if (j < n && !hd && l < lprev && a >= 0)
+            uns[a] = 1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < kFxAll; r++) {
+        const int jl = r * 64 + lane;
+        const int a = ra[r], e = re[r];
+        if (w0 + jl >= n || a < 0 || a >= kFxTile)
+            continue;  // past the end, or a run of an earlier or the next wave
+        if (e == kFxNone) {  // reaches past the rows read
+            if (jl == a)
+                long_runs[atomicAdd(long_count, 1u)] = (uint32_t)(w0 + a);
+            continue;
+        }
+        int rank = jl - a;
+        if (uns[a]) {  // rank by (low half, position) within the run
+            const uint32_t lk = s_l[wv][jl];
+            rank = 0;
+            for (int q = a; q < e; q++) {
+                const uint32_t lq = s_l[wv][q];
+                rank += (lq < lk || (lq == lk && q < jl)) ? 1 : 0;
+            }
+        }
+        out[w0 + a + rank] = p[r];
+    }
+}
+
+// The runs k_split_fixup left: one workgroup per run (a stride of runs per
+// workgroup).  The run's end is found by a scan; a run already in low-half
+// order (exact ties, the usual case) keeps its order; otherwise it is sorted
+// stably by its low halves, four 8-bit passes between the run's own slices
+// of two scratch pairs, (l0, out) and (l1, p1), ending in out.
+__global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
+    const uint32_t *__restrict__ hs, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm_hi,
+    uint32_t *__restrict__ out, int64_t n, const uint32_t *__restrict__ seg_start, int nseg,
+    const uint32_t *__restrict__ long_runs, const uint32_t *__restrict__ long_count, uint32_t *__restrict__ l0,
+    uint32_t *__restrict__ l1, uint32_t *__restrict__ p1)
+{
+    constexpr int WAVES = kLongThreads / 64;
+    __shared__ uint32_t s_wcnt[WAVES][kRadix];
+    __shared__ uint32_t s_off[kRadix], s_tot[kRadix];
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_stop;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nruns = *long_count;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t ri = blockIdx.x; ri < nruns; ri += gridDim.x) {
+        const int64_t a = long_runs[ri];
+        int64_t limit = n;
+        if (seg_start) {
+            const int sg = first_start_at_or_after(seg_start, nseg, a + 1);
+            if (sg < nseg)
+                limit = seg_start[sg];
+        }
+        const uint32_t h = hs[a];
+        int64_t e = limit;
+        for (int64_t base = a + 1; base < limit; base += kLongThreads) {
+            if (tid == 0)
+                s_stop = kLongThreads;
+            __syncthreads();
+            const int64_t k = base + tid;
+            if (k < limit && hs[k] != h)
+                atomicMin(&s_stop, (uint32_t)tid);
+            __syncthreads();
+            const uint32_t st = s_stop;
+            __syncthreads();
+            if (st < (uint32_t)kLongThreads) {
+                e = base + st;
+                break;
+            }
+        }
+        const int64_t m = e - a;
+        bool unsorted = false;
+        for (int64_t k = tid; k < m; k += kLongThreads) {
+            const uint32_t p = perm_hi[a + k];
+            l0[a + k] = lo[p];
+            out[a + k] = p;
+        }
+        __syncthreads();
+        for (int64_t k = tid + 1; k < m; k += kLongThreads)
+            unsorted |= l0[a + k] < l0[a + k - 1];
+        if (!__syncthreads_or(unsorted))
+            continue;  // in order already: out holds it
+        for (int pass = 0; pass < 4; pass++) {
+            const int shift = 8 * pass;
+            const uint32_t *sl = (pass & 1) ? l1 : l0;
+            const uint32_t *sp = (pass & 1) ? p1 : out;
+            uint32_t *dl = (pass & 1) ? l0 : l1;
+            uint32_t *dp = (pass & 1) ? out : p1;
+            for (int i = tid; i < WAVES * kRadix; i += kLongThreads)
+                (&s_wcnt[0][0])[i] = 0;
+            if (tid < kRadix)
+                s_tot[tid] = 0;
+            __syncthreads();
+            for (int64_t k = tid; k < m; k += kLongThreads)
+                atomicAdd(&s_tot[(sl[a + k] >> shift) & (kRadix - 1)], 1u);
+            __syncthreads();
+            const uint32_t ex = digit_exclusive_scan(tid < kRadix ? s_tot[tid] : 0u, s_wave);
+            if (tid < kRadix)
+                s_off[tid] = ex;
+            __syncthreads();
+            // chunks of kLongThreads in input order: stable ranks from wave
+            // ballots and per-wave digit counts
+            for (int64_t base = 0; base < m; base += kLongThreads) {
+                const int64_t k = base + tid;
+                const bool valid = k < m;
+                const uint32_t lv = valid ? sl[a + k] : 0u;
+                const uint32_t pv = valid ? sp[a + k] : 0u;
+                const uint32_t d = (lv >> shift) & (kRadix - 1);
+                const uint64_t peers = match_digit(d, __ballot(valid));
+                const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
+                if (valid && lane == leader)
+                    s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+                __syncthreads();
+                if (tid < kRadix) {
+                    uint32_t run = 0;
+                    for (int w = 0; w < WAVES; w++) {
+                        const uint32_t c = s_wcnt[w][tid];
+                        s_wcnt[w][tid] = run;
+                        run += c;
+                    }
+                    s_tot[tid] = run;
+                }
+                __syncthreads();
+                if (valid) {
+                    const uint32_t pos = s_off[d] + s_wcnt[wave][d] + (uint32_t)__popcll(peers & lt);
+                    dl[a + pos] = lv;
+                    dp[a + pos] = pv;
+                }
+                __syncthreads();
+                if (tid < kRadix) {
+                    s_off[tid] += s_tot[tid];
+                    for (int w = 0; w < WAVES; w++)
+                        s_wcnt[w][tid] = 0;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// The stable sort of the two-plane 64-bit keys in keys[0] (see above).
+hipError_t sort_split64(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
+{
+    uint32_t *count = w.counters + kLongRunCounter;
+    uint32_t *perm_hi = w.vals[0];
+    // four passes on the high plane; the last writes its keys, so the sorted
+    // high halves end in keys[0] (passes alternate keys[0] -> keys[1] -> ...),
+    // and the low plane at keys[0] + n is never written
+    hipError_t e = sort_typed<uint32_t>(w, n, perm_hi, stream, prof, 4, true, false, true, count);
+    if (e != hipSuccess)
+        return e;
+    const uint32_t *hs = reinterpret_cast<const uint32_t *>(w.keys[0]);
+    const uint32_t *lo = hs + n;
+    const uint32_t *seg_start = w.seg.sizes.empty() ? nullptr : w.seg.seg_start;
+    const int nseg = (int)w.seg.sizes.size();
+    // the tile counts are spent: their buffer lists the long runs (at most
+    // one per wave tile of kFxTile positions).  Onesweep's lookback words share
+    // that buffer, so the next onesweep sort must clear it first (epoch 0).
+    uint32_t *long_runs = reinterpret_cast<uint32_t *>(w.lookback);
+    w.epoch = 0;
+    if (prof)
+        prof->begin(K_SORT_FIXUP);
+    const int64_t waves = (n + kFxTile - 1) / kFxTile;
+    hipLaunchKernelGGL(k_split_fixup, dim3((unsigned)((waves + kFxWaves - 1) / kFxWaves)), dim3(64 * kFxWaves), 0,
+                       stream, hs, lo, perm_hi, perm_out, n, seg_start, nseg, long_runs, count);
+    if ((e = hipGetLastError()) == hipSuccess) {
+        uint32_t *l0 = reinterpret_cast<uint32_t *>(w.keys[1]);
+        hipLaunchKernelGGL(k_split_long_runs, dim3(kLongBlocks), dim3(kLongThreads), 0, stream, hs, lo, perm_hi,
+                           perm_out, n, seg_start, nseg, long_runs, count, l0, l0 + n, w.vals[1]);
+        e = hipGetLastError();
+    }
+    if (prof)
+        prof->end(K_SORT_FIXUP);
+    return e;
 }
 
 } // namespace
@@ -836,7 +1166,7 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
         return hipSuccess;
     if (key_bits == 32)
         return sort_typed<uint32_t>(w, n, perm_out, stream, prof);
-    return sort_typed<uint64_t>(w, n, perm_out, stream, prof);
+    return sort_split64(w, n, perm_out, stream, prof);
 }
 
 hipError_t radix_sort_perm_ids(SortWork &w, int64_t n, const void *r, bool f32, const uint32_t *id,

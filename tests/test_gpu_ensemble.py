@@ -45,14 +45,20 @@ def ensemble(ns, sizes, seed, lowl=0.02):
 
 
 @pytest.mark.parametrize("threads,items", TILES)
-@pytest.mark.parametrize("precision", [0, 1])
-def test_segmented_sort_matches_oracle_per_halo(ns, threads, items, precision):
+@pytest.mark.parametrize("precision,narrow", [(0, False), (1, False), (0, True)])
+def test_segmented_sort_matches_oracle_per_halo(ns, threads, items, precision, narrow):
+    """narrow: every radius shares the high 32 bits of its float64 key, so each
+    halo is one run of equal high halves that the sort must cut at the halo
+    boundaries."""
     from nsphere_amd import _lib
     sizes = [1, 63, 4096, 4097, 6144, 10_000, 33_333, 2, 70_001]
     n = sum(sizes)
-    rng = np.random.default_rng(threads + items + precision)
+    rng = np.random.default_rng(threads + items + precision + 10 * narrow)
     r = np.concatenate([np.sort(rng.uniform(0, 10, s)) + rng.normal(0, 1e-2, s) for s in sizes])
     r[rng.integers(0, n, n // 40)] = 2.5  # ties across and within halos
+    if narrow:
+        r = 1.0 + rng.integers(0, 1 << 32, n) * 2.0 ** -52
+        r[rng.integers(0, n, n // 40)] = 1.0 + 2.0 ** -30
     if precision:
         r = r.astype(np.float32).astype(np.float64)
     with ns.GpuContext(n) as ctx:

@@ -107,3 +107,43 @@ def test_sort_repeated_steps_stable_on_ties(ns):
     r = np.sort(rng.choice(np.arange(50, dtype=np.float64), 20000))
     got = gpu_sort_perm(ns, r)
     np.testing.assert_array_equal(got, np.arange(r.size, dtype=np.uint32))
+
+
+def grouped_radii(rng, lengths, shuffle=True):
+    """Groups of radii sharing the high 32 bits of their key (1 + g 2^-20 plus
+    less than 2^-20), each of the given length, with exact ties inside."""
+    parts = []
+    for g, m in enumerate(lengths):
+        low = rng.integers(0, 1 << 32, m)
+        low[rng.random(m) < 0.1] = low[0]            # exact ties
+        parts.append(1.0 + g * 2.0 ** -20 + low * 2.0 ** -52)
+    r = np.concatenate(parts)
+    if shuffle:
+        r = r[rng.permutation(r.size)]
+    return r
+
+
+@pytest.mark.parametrize("case", ["short_runs", "window_edges", "one_long_run", "mixed"])
+def test_sort_runs_of_equal_high_halves(ns, case):
+    """The float64 sort orders the high 32 bits of the keys by radix passes and
+    then each run of equal high halves by the low halves: runs inside one
+    block's window, runs that cross tiles, and runs longer than any window
+    (sorted on one workgroup) must all give the oracle's permutation."""
+    rng = np.random.default_rng(["short_runs", "window_edges", "one_long_run", "mixed"].index(case) + 100)
+    if case == "short_runs":
+        lengths = rng.integers(1, 9, 60_000)
+    elif case == "window_edges":
+        lengths = [2047, 2048, 2049, 2559, 2560, 2561, 2562, 1, 4095, 4096, 4097, 5000, 3]
+    elif case == "one_long_run":
+        lengths = [300_000]
+    else:
+        lengths = np.concatenate([rng.integers(1, 40, 3000), [70_000], rng.integers(100, 3000, 40)])
+    r = grouped_radii(rng, lengths)
+    np.testing.assert_array_equal(gpu_sort_perm(ns, r), oracle.sort_perm(r))
+
+
+def test_sort_runs_in_input_order_keep_it(ns):
+    """A long run already in low-half order (sorted input) stays as it is."""
+    rng = np.random.default_rng(7)
+    r = np.sort(grouped_radii(rng, [200_000, 5, 100_000]))
+    np.testing.assert_array_equal(gpu_sort_perm(ns, r), np.arange(r.size, dtype=np.uint32))
