@@ -88,7 +88,7 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 # per launch): the integrate kernels and the sort's scatter, which the
 # rooflines use.  The other families are timed over a short run after it.
 SORT_KERNELS = ["sort_upsweep", "sort_scan", "sort_scatter", "sort_fixup"]
-ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_deep", "integrate_late", *SORT_KERNELS]
+ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_lc", "integrate_deep", "integrate_late", *SORT_KERNELS]
 AFTER_STEPS = 20
 # One wave alone on an idle MI355X running the kernel's fast Levi-Civita
 # micro-step (reciprocal, one Newton step, force, velocity and time sums: nine

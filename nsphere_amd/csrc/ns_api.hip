@@ -59,7 +59,7 @@ struct ns_gpu_ctx {
     double *seg_mass = nullptr;
     unsigned long long *work = nullptr;  // force-evaluation counters (profiling)
     int64_t work_stride = 0;
-    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_count = nullptr;
+    uint32_t *q_lf = nullptr, *q_lc = nullptr, *q_lcw = nullptr, *q_count = nullptr;
     // NS_OPT_OVERLAP: merge buffers, the second stream and its fork / join
     // events (allocated on first use)
     int overlap = 1;
@@ -113,7 +113,7 @@ const char *kernel_name(int id)
                                          "integrate_deep", "mass_scan", "mlp_binning",
                                          "snapshot_records", "potential", "density",
                                          "rank_files", "integrate_late", "overlap_plan",
-                                         "overlap_place", "sort_fixup"};
+                                         "overlap_place", "sort_fixup", "integrate_lc"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -261,7 +261,7 @@ void free_all(ns_gpu_ctx *c)
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
-                    c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_count,
+                    c->energy_out, c->budget_hits, c->work, c->q_lf, c->q_lc, c->q_lcw, c->q_count,
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
                     c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.slow_rec, c->ov.sel, c->ov.def_pos, c->ov.def_src, c->ov.late_h,
@@ -377,6 +377,7 @@ ns::StepArgs make_args(ns_gpu_ctx *c, const ns_step_params *p)
     a.budget_hits = c->budget_hits;
     a.q_lf = c->q_lf;
     a.q_lc = c->q_lc;
+    a.q_lcw = c->q_lcw;
     a.q_count = c->q_count;
     a.cap_N = c->cap_N;  // leapfrog levels lane-per-particle (default 0..3)
     a.deep_blocks = c->deep_blocks;
@@ -719,6 +720,7 @@ int ns_gpu_ctx_create(int64_t npts, int device, ns_gpu_ctx **out)
     if (e == hipSuccess) e = dalloc(&c->work, ns::kWorkKernels * c->work_stride + ns::kChainWords);
     if (e == hipSuccess) e = dalloc(&c->q_lf, n);
     if (e == hipSuccess) e = dalloc(&c->q_lc, n);
+    if (e == hipSuccess) e = dalloc(&c->q_lcw, n);
     if (e == hipSuccess) e = dalloc(&c->q_count, ns::kQueueCounters);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(c->sw.lookback, 0, (size_t)tiles * ns::kRadix * sizeof(uint64_t));

@@ -199,6 +199,7 @@ enum KernelId : int {
     K_MERGE,
     K_PLACE,
     K_SORT_FIXUP,
+    K_INT_LC,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -16,9 +16,9 @@
 // FORM_BINNED(_FAST): a bulk kernel runs levels 0..3 lane-per-particle and
 //   queues the rest; a deep kernel then runs each queued leapfrog particle on
 //   a whole wave with one adaptive level per lane (the levels of one particle
-//   are independent passes from the same start) and the Levi-Civita particles
-//   lane-per-particle up to level 3 per tau step; a last kernel runs the
-//   Levi-Civita particles that need more on a whole wave, one level per lane.
+//   are independent passes from the same start); the Levi-Civita particles
+//   run lane-per-particle while their tau steps stay at the lowest levels,
+//   and on a whole wave, one level per lane, when one needs more.
 //   Results are identical to the plain form (strict arithmetic) because every
 //   level is the same computation.
 #include <hip/hip_runtime.h>
@@ -199,20 +199,71 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
     add_work(a, 0, spent);
 }
 
-// One wave per queued particle: every queued Levi-Civita particle (its tau
-// steps' levels one per lane), then every leapfrog particle past cap_N (its
-// levels one per lane).  Waves take particles by ticket, the Levi-Civita
-// queue first: it holds the particles with the longest chains, which then
-// start at once instead of after a lane-per-particle attempt.
+// The Levi-Civita particles the bulk kernel queued, one lane each: the
+// reference's sequential level search for every tau step while its levels
+// stay below kLcLaneCapN, which holds nearly every one of them (a tau step
+// of 1/(20 N_taumin) of the orbit's time scale converges at level 0); a
+// particle with a deeper tau step is queued for a whole wave, which starts
+// its step again from the beginning -- the same computation, so the same
+// result.  Wave w takes the chunks w, w + waves, ... of 64 queue entries:
+// the chunks cost about the same, and a wave with no chunk leaves at once
+// without touching a shared counter (thousands of waves contending for one
+// ticket cost more than the work).
+constexpr int64_t kLcLaneCapN = 2;
+static_assert(kLcLaneCapN == 2, "adaptive_lc_lane_fast runs levels 0 and 1, i.e. cap_N = 2");
+
+template <bool GATHER, bool FAST, typename S>
+__global__ __launch_bounds__(kThreads) void k_integrate_lc_lanes(StepArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_lc = a.q_count[1];
+    const uint32_t waves = gridDim.x * (kThreads / 64);
+    const uint32_t w = (blockIdx.x * kThreads + threadIdx.x) >> 6;
+    unsigned long long spent = 0;
+    for (uint32_t base = w * 64u; base < n_lc; base += waves * 64u) {
+        const uint32_t q = base + (uint32_t)lane;
+        bool escalate = false;
+        uint32_t i = 0;
+        if (q < n_lc) {
+            i = a.q_lc[q];
+            double r, v, ell;
+            load_particle<GATHER, S>(a, i, r, v, ell);
+            const RankMass rm = rank_mass(a, i);
+            const Phys ph = make_phys(a, i, rm, ell);
+            Budget bud{a.budget, false};
+            double r_new, v_new;
+            const int rec = (FAST && a.max_subdiv >= 2 * kLcLaneCapN)
+                                ? adaptive_lc_lane_fast(ph, a, r, v, r_new, v_new, bud)
+                                : adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud, kLcLaneCapN);
+            if (rec == kEscalate)
+                escalate = true;
+            else
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit, i, rm.k.i);
+            spent += (unsigned long long)(a.budget - bud.left);
+        }
+        wave_push(a.q_lcw, a.q_count + 3, escalate, i);
+    }
+    add_work(a, 1, spent);
+}
+
+// One wave per queued particle: every Levi-Civita particle the lane kernel
+// passed on (its tau steps' levels one per lane), then every leapfrog
+// particle past cap_N (its levels one per lane), the Levi-Civita queue
+// first: it holds the particles with the longest chains, which then start at
+// once.  Wave w takes entry w; entries past the grid go by ticket to the
+// waves that finish first, so waves without an entry never touch the
+// ticket.
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t n_lc = a.q_count[1], n_lf = a.q_count[0];
+    const uint32_t n_lc = a.q_count[3], n_lf = a.q_count[0];
+    const uint32_t waves = gridDim.x * (kThreads / 64);
     unsigned long long spent = 0;
-    for (uint32_t q = wave_ticket(a.q_count + 2); q < n_lc + n_lf; q = wave_ticket(a.q_count + 2)) {
+    for (uint32_t q = (blockIdx.x * kThreads + threadIdx.x) >> 6; q < n_lc + n_lf;
+         q = waves + wave_ticket(a.q_count + 2)) {
         const bool lc = q < n_lc;  // wave-uniform
-        const int64_t i = lc ? a.q_lc[q] : a.q_lf[q - n_lc];
+        const int64_t i = lc ? a.q_lcw[q] : a.q_lf[q - n_lc];
         double r, v, ell;
         load_particle<GATHER, S>(a, i, r, v, ell);
         const RankMass rm = rank_mass(a, i);
@@ -427,8 +478,13 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler
     end(K_INT_BULK);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    begin(K_INT_DEEP);
     const int deep_blocks = a.deep_blocks > 0 && a.deep_blocks <= kMaxDeepBlocks ? a.deep_blocks : 1024;
+    begin(K_INT_LC);
+    hipLaunchKernelGGL((k_integrate_lc_lanes<GATHER, FAST, S>), dim3(deep_blocks), dim3(kThreads), 0, stream, a);
+    end(K_INT_LC);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    begin(K_INT_DEEP);
     hipLaunchKernelGGL((k_integrate_deep<GATHER, FAST, S>), dim3(deep_blocks), dim3(kThreads), 0, stream, a);
     end(K_INT_DEEP);
     return hipGetLastError();

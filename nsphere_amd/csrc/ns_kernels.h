@@ -39,9 +39,11 @@ struct StepArgs {
     // ranks in stable order of predicted level
     const uint32_t *order;
     uint32_t *q_lf;              // leapfrog particles past cap_N (wave per particle)
-    uint32_t *q_lc;              // Levi-Civita particles (wave per particle)
-    uint32_t *q_count;           // lengths of q_lf, q_lc; the deep kernel's wave
-                                 // ticket (kQueueCounters)
+    uint32_t *q_lc;              // Levi-Civita particles (lane per particle first)
+    uint32_t *q_lcw;             // Levi-Civita particles whose tau steps need more
+                                 // than the lane levels (wave per particle)
+    uint32_t *q_count;           // [0] length of q_lf, [1] of q_lc, [2] the deep
+                                 // kernel's ticket, [3] length of q_lcw
     int64_t cap_N;               // deepest leapfrog N tried lane-per-particle
     int32_t deep_blocks;         // grid of the wave-per-particle kernel (256 threads each)
     // profiling only (else null): force evaluations spent, one slot per
@@ -72,7 +74,7 @@ constexpr int kWorkKernels = 3;
 constexpr int kChainMax = 0;
 constexpr int kChainSum = 1;
 constexpr int kChainWords = 2;
-constexpr int kQueueCounters = 3;
+constexpr int kQueueCounters = 4;
 constexpr uint32_t kDeferBit = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 constexpr int kSlowCap = 1024;   // particles deferred per step

@@ -433,6 +433,20 @@ struct LcFast {
     double rho, u, u_prev, T;
     double g, inv2;     // force terms at the current rho
 
+    // init with the dtau-dependent constants (LcPassConst) and 1/rho_in^2
+    // given: the same operations as init
+    __device__ inline void init_from(double hd_, double lh_, double gh_, double rho_in, double inv2_in, double v_in)
+    {
+        hd = hd_;
+        lh = lh_;
+        gh = gh_;
+        rho = rho_in;
+        T = 0.0;
+        inv2 = inv2_in;
+        g = fma(lh, inv2, gh);
+        u = fma(0.5 * g, inv2, hd * v_in);
+        u_prev = u;
+    }
     __device__ inline void init(const Phys &ph, double dtau, double rho_in, double v_in)
     {
         hd = 0.5 * dtau;
@@ -740,6 +754,149 @@ __device__ int adaptive_lc(const Phys &ph, const StepArgs &a, double r_in, doubl
         else
             lv = single_tau_lc<FAST>(ph, a, rho_current, v_current, t_cur, delta_tau, rho_next, v_next, t_next,
                                      bud, cap_N);
+        if (lv == kEscalate)
+            return kEscalate;
+        if ((lv & NS_LVL_MASK) > (rec & NS_LVL_MASK))
+            rec = (rec & ~NS_LVL_MASK) | (lv & NS_LVL_MASK);
+        rec |= (lv & NS_LVL_NOCONV);
+        if (t_next > dt) {
+            double alpha = 0.0;
+            if (fabs(t_next - t_cur) > 1.0e-30)
+                alpha = (dt - t_cur) / (t_next - t_cur);
+            const double rho_final = rho_current + alpha * (rho_next - rho_current);
+            const double v_final = v_current + alpha * (v_next - v_current);
+            r_out = rho_final * rho_final;
+            v_out = v_final;
+            return rec;
+        }
+        rho_current = rho_next;
+        v_current = v_next;
+        t_cur = t_next;
+        step_count++;
+        if (step_count > step_max || bud.hit) {
+            r_out = rho_current * rho_current;
+            v_out = v_current;
+            return rec | NS_LVL_NOCONV;
+        }
+    }
+    r_out = rho_current * rho_current;
+    v_out = v_current;
+    return rec;
+}
+
+// The constants LcFast::init derives from dtau = h / sub for a pass of sub
+// micro-steps over h (the same operations, done once per particle-step
+// instead of once per tau step).
+struct LcPassConst {
+    double hd, lh, gh;
+    __device__ inline void set(const Phys &ph, double h, int64_t sub)
+    {
+        const double dtau = h / (double)sub;
+        hd = 0.5 * dtau;
+        const double sc = hd * dtau;
+        lh = ph.l2 * sc;
+        gh = (ph.idg ? 0.0 : ph.gm) * sc;
+    }
+};
+
+__device__ inline bool lc_converged(const StepArgs &a, double rhoC, double vC, double rhoF, double vF)
+{
+    const double rF = rhoF * rhoF;
+    const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
+    const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
+    return (rhodif < a.radius_tol) && (vdif < a.velocity_tol);
+}
+
+// single_tau_lc<true> with cap_N = 2 (levels 0 and 1), for max_subdiv >= 4,
+// written out: the coarse and fine passes of level 0 (3 and 5 micro-steps)
+// start from the same state, so they run interleaved, and the level-1 pass
+// (9 micro-steps) only when level 0 fails.  The same passes, budget charges,
+// tests and outputs as the general form, hence the same results.
+__device__ inline int lc_tau_lane_fast(const StepArgs &a, const LcPassConst K[3], double rho_in, double v_in,
+                                       double t_in, double &rho_out, double &v_out, double &t_out, Budget &bud)
+{
+    if (!bud.take(4)) {
+        rho_out = rho_in;
+        v_out = v_in;
+        t_out = t_in;
+        return NS_LVL_NOCONV | NS_LVL_MASK;
+    }
+    const double inv = fast_rcp(rho_in);
+    const double inv2 = inv * inv;
+    LcFast c, f;
+    c.init_from(K[0].hd, K[0].lh, K[0].gh, rho_in, inv2, v_in);
+    f.init_from(K[1].hd, K[1].lh, K[1].gh, rho_in, inv2, v_in);
+    c.step();
+    f.step();
+    c.step();
+    f.step();
+    c.step();
+    f.step();
+    f.step();
+    f.step();
+    const double rhoC = c.rho, vC = c.v_end(), tC = c.t_end(t_in);
+    if (!bud.take(6)) {
+        rho_out = rhoC;
+        v_out = vC;
+        t_out = tC;
+        return NS_LVL_NOCONV;
+    }
+    const double rhoF = f.rho, vF = f.v_end(), tF = f.t_end(t_in);
+    if (lc_converged(a, rhoC, vC, rhoF, vF)) {
+        lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
+        return 0;
+    }
+    if (!bud.take(10)) {  // the coarse pass of level 1 is the fine pass of level 0
+        rho_out = rhoF;
+        v_out = vF;
+        t_out = tF;
+        return NS_LVL_NOCONV;
+    }
+    LcFast d;
+    d.init_from(K[2].hd, K[2].lh, K[2].gh, rho_in, inv2, v_in);
+#pragma unroll
+    for (int s = 0; s < 9; s++)
+        d.step();
+    const double rhoD = d.rho, vD = d.v_end(), tD = d.t_end(t_in);
+    if (lc_converged(a, rhoF, vF, rhoD, vD)) {
+        lc_output(a.out_type, rhoF, vF, tF, rhoD, vD, tD, rho_out, v_out, t_out);
+        return 1;
+    }
+    return kEscalate;  // level 2 is past the lane levels
+}
+
+// adaptive_lc<true> (lane form) with cap_N = 2 and max_subdiv >= 4: the
+// tau loop of doAdaptiveFullLeviCivita (nsphere.c:2231-2334) around
+// lc_tau_lane_fast, the pass constants computed once for the particle-step.
+__device__ inline int adaptive_lc_lane_fast(const Phys &ph, const StepArgs &a, double r_in, double v_in,
+                                            double &r_out, double &v_out, Budget &bud)
+{
+    int rec = NS_LVL_LC;
+    if (r_in < 1.0e-30) {
+        r_out = r_in;
+        v_out = v_in;
+        return rec;
+    }
+    const double dt = a.dt;
+    double rho_current = sqrt(r_in);
+    double v_current = v_in;
+    double t_cur = 0.0;
+    double delta_tau;
+    if (r_in > 1.0e-30 && a.N_taumin > 0)
+        delta_tau = dt / (2.0 * r_in * (double)a.N_taumin);
+    else
+        delta_tau = dt / 100.0;
+    LcPassConst K[3];
+    K[0].set(ph, delta_tau, 3);
+    K[1].set(ph, delta_tau, 5);
+    K[2].set(ph, delta_tau, 9);
+    int64_t step_count = 0;
+    const int64_t step_max = 100000000;
+    while (true) {
+        if (t_cur >= dt)
+            break;
+        double rho_next, v_next, t_next;
+        const int lv = lc_tau_lane_fast(a, K, rho_current, v_current, t_cur, rho_next, v_next, t_next, bud);
         if (lv == kEscalate)
             return kEscalate;
         if ((lv & NS_LVL_MASK) > (rec & NS_LVL_MASK))
