@@ -71,16 +71,16 @@ def bytes_per_particle(state_bytes: int) -> dict:
     return out
 
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
-# (N = 1.2e7, float32 state), scaled by the counters' reading for kernels of known traffic
-# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md); sort_scatter is the mean over its 4 passes
-# (8 with float64 keys), integrate_late from the overlapped run
-# (profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md).
-MEASURED_TRAFFIC = {("f32", 12_000_000): {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6,
-                                          "integrate_bulk": 499.3e6, "integrate_deep": 14.0e6},
-                    ("f64", 12_000_000): {"sort_scatter": 275.9e6, "sort_upsweep": 98.1e6,
-                                          "integrate_bulk": 835.9e6, "integrate_deep": 16.1e6,
-                                          "integrate_late": 0.23e6}}
+# HBM bytes per launch of the benchmark workload (N = 1.2e7), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of the same kernels, scaled by the counters' reading for kernels of known traffic;
+# the profile each figure comes from is named beside it.  These are not measured in the bench run
+# itself (a counter run is a separate rocprofv3 invocation), and a kernel whose code changed since
+# its profile has no entry.
+TRAFFIC_PROFILES = {
+    ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md",
+                          {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6,
+                           "integrate_bulk": 499.3e6, "integrate_deep": 14.0e6}),
+}
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
             "integrate_late": "late"}
@@ -90,12 +90,10 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 SORT_KERNELS = ["sort_upsweep", "sort_scan", "sort_scatter", "sort_fixup"]
 ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_lc", "integrate_deep", "integrate_late", *SORT_KERNELS]
 AFTER_STEPS = 20
-# One wave alone on an idle MI355X running the kernel's fast Levi-Civita
-# micro-step (reciprocal, one Newton step, force, velocity and time sums: nine
-# dependent FP64 operations) in a loop unrolled by four on a 32-bit scalar trip
-# counter: 23.4 ns per micro-step, the issue time of its instructions.  The
-# late kernel's chains run at that rate at best.
-CHAIN_FLOOR_NS = 23.4
+# The late kernel's chains run at best at the rate of one wave alone on an idle
+# device running its fast Levi-Civita micro-step (ns_gpu_chain_floor, measured
+# at the start of each run).
+CHAIN_FLOOR_STEPS = 1 << 22
 
 
 
@@ -175,6 +173,33 @@ def cpu_baseline(state, params, budget_s: float, work_cap: int, halos=None) -> d
                   f"({el:.1f} s); C restatement with the reference's -Ofast flags, chunked sort "
                   f"(--sort 1), work cap {work_cap}; {threads} OpenMP threads on {cpu_info()}",
     }
+
+
+def cpu_p1(ns, seed: int, n: int = 100_000, steps: int = 1000, wall_limit_s: float = 120.0) -> dict:
+    """The reference's CPU case run in full: Plummer, N = 1e5 equal masses
+    (no low-ell tail), 1000 steps at dt = 1e-3, on the oracle with the
+    reference's -Ofast flags, its default chunked sort and OpenMP on the host
+    cores, with no work cap, as the reference runs it.  The initial exact sort
+    (nsphere.c:5339-5342) is outside the timing; a wall limit stops the run
+    early and says so."""
+    import oracle
+
+    r, v, ell, ids = ns.generate_ic(0, n, seed, 100.0, 0.0, 1e-3)
+    q = oracle.sort_perm(r)
+    r, v, ell, ids = (a[q].copy() for a in (r, v, ell, ids))
+    po = oracle.params(dt=1e-3, budget=0)
+    threads = oracle.lib(True).orc_num_threads()
+    done = 0
+    t0 = time.perf_counter()
+    while done < steps and time.perf_counter() - t0 < wall_limit_s:
+        oracle.step(r, v, ell, ids, po, chunked=True, fast=True)
+        done += 1
+    el = time.perf_counter() - t0
+    return {"value": n * done / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "steps": done, "seconds": round(el, 3),
+            "sample": f"N={n} Plummer (no low-ell tail), {done} of {steps} steps at dt=1e-3 run in full "
+                      f"({el:.1f} s); C restatement with the reference's -Ofast flags, chunked sort "
+                      f"(--sort 1), no work cap; {threads} OpenMP threads on {cpu_info()}"}
 
 
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
@@ -423,7 +448,10 @@ def measure_rank_files(ns, precision: str, state, path: str = "/dev/null") -> di
 
 
 def traffic_for(run: dict, n: int, kernel: str):
-    return MEASURED_TRAFFIC.get((run["precision"], n), {}).get(kernel)
+    """(bytes per launch, profile) for the kernel, or (None, None)."""
+    src, table = TRAFFIC_PROFILES.get((run["precision"], n), (None, {}))
+    b = table.get(kernel)
+    return (b, src) if b is not None else (None, None)
 
 
 def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
@@ -438,16 +466,18 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
     if kernel in run["bpp"]:
         bpp = run["bpp"][kernel]
         achieved = bpp * n / avg_s / 1e9
+        traffic, src = traffic_for(run, n, kernel)
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(run, n, kernel),
-                "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "traffic_source": src, "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
     if kernel in WORK_ROW:
         ev = run["evals"][WORK_ROW[kernel]] / launches
         flops = FLOPS_PER_FORCE_EVAL * ev
         achieved = flops / avg_s / 1e12
+        traffic, src = traffic_for(run, n, kernel)
         out = {"kernel": kernel, "bound": "fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
-               "traffic": traffic_for(run, n, kernel),
+               "traffic": traffic, "traffic_source": src,
                "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
                "avg_us": round(avg_s * 1e6, 2)}
         if kernel == "integrate_late" and run.get("chain"):
@@ -455,11 +485,27 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
             # per micro-step of that chain against one wave running the same
             # micro-step alone on an idle GPU
             ns_step = ms * 1e6 / run["chain"]
+            floor = run.get("chain_floor_ns")
             out["latency"] = {"chain_micro_steps_per_launch": round(run["chain"] / launches, 1),
-                              "ns_per_micro_step": round(ns_step, 2), "floor_ns": CHAIN_FLOOR_NS,
-                              "frac": round(CHAIN_FLOOR_NS / ns_step, 3)}
+                              "ns_per_micro_step": round(ns_step, 2), "floor_ns": floor,
+                              "floor_source": "ns_gpu_chain_floor, one wave alone, measured in this run",
+                              "frac": round(floor / ns_step, 3) if floor else None}
         return out
     return None
+
+
+SIDE_STREAM = {"integrate_late", "overlap_plan"}
+
+
+def dominant_kernel(run: dict) -> str:
+    """The kernel family with the most time on the stream that bounds the
+    step: in an overlapped run the late kernel's stream when its kernels add
+    up to more than the context stream's, else the context stream."""
+    stats = run["stats"]
+    side = {k: v[1] for k, v in stats.items() if k in SIDE_STREAM}
+    main = {k: v[1] for k, v in stats.items() if k not in SIDE_STREAM}
+    pool = side if side and sum(side.values()) > sum(main.values()) else main
+    return max(pool.items(), key=lambda kv: kv[1])[0]
 
 
 def sort_roofline(run: dict, n: int) -> dict | None:
@@ -547,6 +593,7 @@ def main() -> int:
 
     import nsphere_amd as ns
 
+    chain_floor = ns.chain_floor(0, CHAIN_FLOOR_STEPS)  # on the idle device, before any other work
     kind, n_default, rmax, lowl = WORKLOADS[args.workload]
     t_ic = time.perf_counter()
     p = ns.unit_params(dt=args.dt)
@@ -592,7 +639,8 @@ def main() -> int:
         plain = measure(ns, "plain", "f64", state, p, args.plain_steps or args.steps, args.warmup, args.budget,
                         mass, halos)
 
-    dom = max(chosen["stats"].items(), key=lambda kv: kv[1][1])[0]
+    dom = dominant_kernel(chosen)
+    chosen["chain_floor_ns"] = chain_floor
     out = {
         "metric": "particle-steps/s at N=1.2e7, chosen form vs plain FP64 GPU at |dE/E|<=1e-6",
         "value": chosen["value"],
@@ -613,6 +661,7 @@ def main() -> int:
         "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": sort_roofline(chosen, n),
+        "roofline_late": roofline_for(chosen, n, "integrate_late"),
         "roofline_scatter": roofline_for(chosen, n, "sort_scatter"),
         "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
                           "canonical_bytes_per_particle_step": CANONICAL_STEP_BYTES[args.precision],
@@ -654,6 +703,7 @@ def main() -> int:
     if not args.no_cpu:
         print("[bench] timing CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(state, p, args.cpu_seconds, args.budget, halos)
+        out["cpu_baseline"]["p1_full_run"] = cpu_p1(ns, args.seed)
         if mass is not None:
             out["cpu_baseline"]["sample"] += "; equal masses from the rank (the oracle step has no mass scan)"
     else:

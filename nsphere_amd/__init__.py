@@ -75,6 +75,14 @@ def generate_ic(kind: int, n: int, seed: int, rmax: float = 100.0, lowl_frac: fl
     return r, v, ell, ids
 
 
+def chain_floor(device: int = 0, micro_steps: int = 1 << 22) -> float:
+    """ns per micro-step of one wave running the late kernel's fast
+    Levi-Civita micro-step alone on the device (the floor of its chains)."""
+    out = ctypes.c_double()
+    check(lib.ns_gpu_chain_floor(int(device), int(micro_steps), ctypes.byref(out)))
+    return out.value
+
+
 def philox4x32_10(ctr, key) -> tuple:
     c = (ctypes.c_uint32 * 4)(*ctr)
     k = (ctypes.c_uint32 * 2)(*key)

@@ -123,6 +123,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_kernel_stats_reset": (i32, [vp]),
         "ns_gpu_force_evals": (i32, [vp, c.POINTER(c.c_uint64)]),
         "ns_gpu_chain_steps": (i32, [vp, c.POINTER(c.c_uint64)]),
+        "ns_gpu_chain_floor": (i32, [i32, i64, c.POINTER(c.c_double)]),
         "ns_gpu_set_masses": (i32, [vp, vp]),
         "ns_gpu_set_halos": (i32, [vp, i32, vp, vp]),
         "ns_gpu_autotune": (i32, [vp, c.POINTER(StepParams), i32, c.POINTER(TuneResult)]),

@@ -1894,6 +1894,39 @@ int ns_gpu_chain_steps(ns_gpu_ctx *c, uint64_t *out)
     return NS_OK;
 }
 
+int ns_gpu_chain_floor(int device, int64_t micro_steps, double *ns_per_step)
+{
+    ns::clear_error();
+    if (!ns_per_step || micro_steps < 1000 || micro_steps > ((int64_t)1 << 34))
+        return set_error(NS_ERR_ARG, "ns_gpu_chain_floor: null output or micro_steps outside [1e3, 2^34]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return set_error(NS_ERR_NODEV, "ns_gpu_chain_floor: no device %d", device);
+    DeviceGuard g(device);
+    hipStream_t s = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    double *d = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipMalloc((void **)&d, sizeof(double));
+    float ms = 0.f;
+    if (e == hipSuccess) e = ns::launch_chain_floor(1000, d, s);  // warm the code object
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    if (e == hipSuccess) e = ns::launch_chain_floor(micro_steps, d, s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (d) (void)hipFree(d);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess)
+        return set_error(NS_ERR_HIP, "ns_gpu_chain_floor: %s", hipGetErrorString(e));
+    *ns_per_step = (double)ms * 1.0e6 / (double)micro_steps;
+    return NS_OK;
+}
+
 int ns_gpu_force_evals(ns_gpu_ctx *c, uint64_t out[3])
 {
     if (int rc = check_ctx(c, "ns_gpu_force_evals"))

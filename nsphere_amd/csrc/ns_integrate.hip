@@ -158,44 +158,76 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
     add_work(a, 0, spent);
 }
 
+// Particles per thread of the bulk kernel: a thread issues the permutation
+// loads of all of them, then all their state gathers, before it integrates
+// any, so that a wave has that many dependent gathers in flight at once.
+constexpr int kBulkPPT = 2;
+
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 {
-    const int64_t t = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool valid = t < a.n;
-    // learned binning: slot t holds the rank order[t], ranks in stable order
-    // of predicted level, so that a wave's lanes share their expected work
-    const int64_t i = (a.order && valid) ? (int64_t)a.order[t] : t;
-    bool to_lc = false, to_lf = false;
-    unsigned long long spent = 0;
-    if (valid && GATHER && a.slow_list && (a.perm[i] & kDeferBit)) {
+    const int64_t base =
+        (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * (kThreads * kBulkPPT) + threadIdx.x;
+    int64_t idx[kBulkPPT], src[kBulkPPT];
+    bool valid[kBulkPPT], defer[kBulkPPT];
+#pragma unroll
+    for (int k = 0; k < kBulkPPT; k++) {
+        const int64_t t = base + (int64_t)k * kThreads;
+        valid[k] = t < a.n;
+        // learned binning: slot t holds the rank order[t], ranks in stable
+        // order of predicted level, so that a wave's lanes share their
+        // expected work
+        idx[k] = (a.order && valid[k]) ? (int64_t)a.order[t] : t;
+    }
+    uint32_t pe[kBulkPPT];
+#pragma unroll
+    for (int k = 0; k < kBulkPPT; k++)
+        pe[k] = (GATHER && valid[k]) ? a.perm[idx[k]] : 0u;
+    double r[kBulkPPT], v[kBulkPPT], ell[kBulkPPT];
+    uint32_t id[kBulkPPT];
+#pragma unroll
+    for (int k = 0; k < kBulkPPT; k++) {
+        src[k] = GATHER ? (int64_t)(pe[k] & kIndexMask) : idx[k];
         // integrated by the late kernel on the side stream
-        const int64_t src = source_of<GATHER>(a, i);
-        static_cast<S *>(a.l1)[i] = static_cast<const S *>(a.l0)[src];
-        a.id1[i] = a.id0[src];
-        if (a.keys)
-            store_key(a.keys, a.n, i, ~(typename KeyOf<S>::type)0);
-    } else if (valid) {
-        double r, v, ell;
-        load_particle<GATHER, S>(a, i, r, v, ell);
-        carry_ell_id<GATHER, S>(a, i, ell);
-        const RankMass rm = rank_mass(a, i);
-        if (takes_levi_civita(a, rm, r, ell)) {
-            to_lc = true;
-        } else {
-            const Phys ph = make_phys(a, i, rm, ell);
-            Budget bud{a.budget, false};
-            double r_new, v_new;
-            const int rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, a.cap_N);
-            if (rec == kEscalate)
-                to_lf = true;
-            else
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
-            spent = (unsigned long long)(a.budget - bud.left);
+        defer[k] = GATHER && a.slow_list && (pe[k] & kDeferBit);
+        if (valid[k]) {
+            r[k] = (double)static_cast<const S *>(a.r0)[src[k]];
+            v[k] = (double)static_cast<const S *>(a.v0)[src[k]];
+            ell[k] = (double)static_cast<const S *>(a.l0)[src[k]];
+            id[k] = GATHER ? a.id0[src[k]] : 0u;
         }
     }
-    wave_push(a.q_lc, a.q_count + 1, to_lc, (uint32_t)i);
-    wave_push(a.q_lf, a.q_count + 0, to_lf, (uint32_t)i);
+    unsigned long long spent = 0;
+#pragma unroll
+    for (int k = 0; k < kBulkPPT; k++) {
+        const int64_t i = idx[k];
+        bool to_lc = false, to_lf = false;
+        if (valid[k] && GATHER) {
+            static_cast<S *>(a.l1)[i] = (S)ell[k];
+            a.id1[i] = id[k];
+        }
+        if (valid[k] && defer[k]) {
+            if (a.keys)
+                store_key(a.keys, a.n, i, ~(typename KeyOf<S>::type)0);
+        } else if (valid[k]) {
+            const RankMass rm = rank_mass(a, i);
+            if (takes_levi_civita(a, rm, r[k], ell[k])) {
+                to_lc = true;
+            } else {
+                const Phys ph = make_phys(a, i, rm, ell[k]);
+                Budget bud{a.budget, false};
+                double r_new, v_new;
+                const int rec = adaptive_leap<FAST>(ph, a, r[k], v[k], r_new, v_new, bud, a.cap_N);
+                if (rec == kEscalate)
+                    to_lf = true;
+                else
+                    write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
+                spent += (unsigned long long)(a.budget - bud.left);
+            }
+        }
+        wave_push(a.q_lc, a.q_count + 1, to_lc, (uint32_t)i);
+        wave_push(a.q_lf, a.q_count + 0, to_lf, (uint32_t)i);
+    }
     add_work(a, 0, spent);
 }
 
@@ -322,6 +354,22 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
     if (a.work && (threadIdx.x & 63) == 0 && bud.chain > 0)
         atomicMax(a.work + kWorkKernels * a.work_stride + kChainMax, (unsigned long long)bud.chain);
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);
+}
+
+// One wave running the fast Levi-Civita micro-step (LcFast::step, the loop
+// body of the late kernel's chains) `steps` times on a scalar trip counter,
+// as the wave form does; out keeps the result live.
+__global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__restrict__ out)
+{
+    Phys ph;
+    ph.gm = -1.0e-3;
+    ph.l2 = 1.0e-9;
+    ph.idg = false;
+    LcFast lf;
+    lf.init(ph, 1.0e-3 / (double)(steps > 0 ? steps : 1), 0.1, -0.2);
+    repeat_uniform(steps, [&]() { lf.step(); });
+    if ((threadIdx.x & 63) == 0)
+        out[0] = lf.rho + lf.v_end();
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -474,7 +522,8 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler
     if (e != hipSuccess)
         return e;
     begin(K_INT_BULK);
-    hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST, S>), grid, dim3(kThreads), 0, stream, a);
+    const dim3 bulk_grid((unsigned)((a.n + kThreads * kBulkPPT - 1) / (kThreads * kBulkPPT)));
+    hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST, S>), bulk_grid, dim3(kThreads), 0, stream, a);
     end(K_INT_BULK);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
@@ -516,6 +565,12 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
         return hipSuccess;
     return f32 ? launch_typed<float>(a, gather, form, stream, prof)
                : launch_typed<double>(a, gather, form, stream, prof);
+}
+
+hipError_t launch_chain_floor(int64_t steps, double *out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_chain_floor, dim3(1), dim3(64), 0, stream, steps, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *def_src,

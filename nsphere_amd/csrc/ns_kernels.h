@@ -179,6 +179,8 @@ hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int
 // the side stream.
 hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *def_pos, const uint32_t *def_src,
                        const uint32_t *count, hipStream_t stream);
+// One wave running the late kernel's fast Levi-Civita micro-step `steps` times.
+hipError_t launch_chain_floor(int64_t steps, double *out, hipStream_t stream);
 // Radius keys of the deferred ranks of the last step (replacing the sentinel).
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
                             hipStream_t stream);

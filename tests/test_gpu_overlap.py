@@ -259,3 +259,12 @@ def test_overlap_ensemble_is_bit_identical(ns, budget, ties, form, precision):
             outs.append(dict(state=ctx.download(), levels=ctx.levels(), hits=ctx.cap_hits()))
     assert outs[0]["hits"] > 0
     assert_same(outs[0], outs[1])
+
+
+def test_chain_floor_measures_one_wave(ns):
+    """ns_gpu_chain_floor: one wave alone running the late kernel's micro-step;
+    a few tens of ns per micro-step on MI355X, and stable between calls."""
+    a = ns.chain_floor(0, 1 << 20)
+    b = ns.chain_floor(0, 1 << 21)
+    assert 5.0 < a < 200.0 and 5.0 < b < 200.0
+    assert abs(a - b) < 0.2 * max(a, b)

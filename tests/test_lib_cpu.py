@@ -147,3 +147,13 @@ def test_mixed_ic_alternates_recipes():
     he = ns.generate_ic(1, 10_000, 3, 100.0, 0.0, 1e-3)
     np.testing.assert_array_equal(mix[0][0::2], pl[0][0::2])
     np.testing.assert_array_equal(mix[0][1::2], he[0][1::2])
+
+
+def test_chain_floor_rejects_bad_arguments():
+    """Refused on the host before any device call."""
+    import ctypes
+    from nsphere_amd._lib import NS_ERR_ARG, lib
+    out = ctypes.c_double()
+    assert lib.ns_gpu_chain_floor(0, 10, ctypes.byref(out)) == NS_ERR_ARG
+    assert lib.ns_gpu_chain_floor(0, 1 << 40, ctypes.byref(out)) == NS_ERR_ARG
+    assert lib.ns_gpu_chain_floor(0, 1 << 20, None) == NS_ERR_ARG
