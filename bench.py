@@ -78,8 +78,12 @@ def bytes_per_particle(state_bytes: int) -> dict:
 # its profile has no entry.
 TRAFFIC_PROFILES = {
     ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md",
-                          {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6,
-                           "integrate_bulk": 499.3e6, "integrate_deep": 14.0e6}),
+                          {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6, "integrate_bulk": 499.3e6}),
+    # sort_scatter: mean of the 4 passes' read + write
+    ("f64", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md",
+                          {"sort_scatter": 183.2e6, "sort_upsweep": 50.05e6, "sort_fixup": 239.5e6,
+                           "integrate_bulk": 833.9e6, "integrate_lc": 9.78e6, "integrate_deep": 0.41e6,
+                           "integrate_late": 0.25e6}),
 }
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
