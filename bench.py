@@ -31,35 +31,38 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 (vector and matrix) peak
 FLOPS_PER_FORCE_EVAL = 16     # FP64 flops per force evaluation + kick/drift, a divide counted as one
-# HBM bytes per particle-step of the whole step.  Canonical (the survey's
-# model: 8-bit digits over keys as wide as the state, so 8 passes of 64-bit
-# keys for float64): float64 304, float32 124.  This build's algorithm: the
-# float64 sort runs 4 passes over the 32-bit high halves of the keys and one
-# fix-up pass over runs of equal high halves (92 B instead of 200 B), so a
-# float64 step moves 196 B.
+# HBM bytes per particle-step of the whole step.  Canonical (8-bit digits
+# over keys as wide as the state, so 8 passes of 64-bit keys for float64):
+# float64 304, float32 124.  This build's algorithm: between steps the
+# float64 sort runs 3 passes over 24-bit quantile keys and one fix-up pass
+# over runs of equal keys (72 B instead of 200 B), and the integrate writes a
+# 4-byte key instead of 8, so a float64 step moves 172 B.
 CANONICAL_STEP_BYTES = {"f64": 304, "f32": 124}
-STEP_BYTES = {"f64": 196, "f32": 124}
+STEP_BYTES = {"f64": 172, "f32": 124}
 
 
 def bytes_per_particle(state_bytes: int) -> dict:
     """Algorithmic HBM bytes per particle of each kernel family of one step,
     for float64 (8) or float32 (4) state, uint32 permutation and ids, 8-bit
-    digits.  Both sorts pass 4 times over 32-bit keys: the float32 radius
-    keys, or the high halves of the float64 keys, whose last pass also writes
-    its keys for the fix-up that orders runs of equal high halves by the low
-    halves (read high half 4 and permutation 4, gather the low half 4, write
-    the permutation 4)."""
+    digits.  The float32 sort passes 4 times over its 32-bit radius keys.
+    The float64 sort between steps passes 3 times over the 24-bit quantile
+    keys the integrate wrote (4 bytes each); its last pass also writes its
+    keys for the fix-up that orders runs of equal keys by radius (read key 4
+    and permutation 4, the 8-byte radius of the half of the positions that
+    share their key, write the permutation 4: 16)."""
     s = float(state_bytes)
     wide = state_bytes == 8
+    passes = 3 if wide else 4
+    key = 4.0                        # the sort key and the key the integrate writes
     first = 4 + (4 + 4)              # read key; write key + payload
     middle = 2 * (4 + 4)             # read and write key + payload
     last = (4 + 4) + (8 if wide else 4)  # read key + payload; write (key +) permutation
-    # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key s, level 1
-    integrate = (4 + 3 * s + 4) + (3 * s + 4 + s + 1)
+    # read perm 4, r v ell 3s, id 4; write r v ell 3s, id 4, key, level 1
+    integrate = (4 + 3 * s + 4) + (3 * s + 4 + key + 1)
     out = {
         "sort_histogram": 4,                                     # read the key (onesweep)
         "sort_upsweep": 4,                                       # read the key (reduce-then-scan)
-        "sort_scatter": (first + 2 * middle + last) / 4,
+        "sort_scatter": (first + (passes - 2) * middle + last) / passes,
         "integrate": integrate,                                  # plain form
         "integrate_bulk": integrate,                             # binned forms: every particle's gather + write
         "radius_keys": s + s,                                    # read r; write the key
@@ -79,11 +82,8 @@ def bytes_per_particle(state_bytes: int) -> dict:
 TRAFFIC_PROFILES = {
     ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md",
                           {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6, "integrate_bulk": 499.3e6}),
-    # sort_scatter: mean of the 4 passes' read + write
     ("f64", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md",
-                          {"sort_scatter": 183.2e6, "sort_upsweep": 50.05e6, "sort_fixup": 239.5e6,
-                           "integrate_bulk": 833.9e6, "integrate_lc": 9.78e6, "integrate_deep": 0.41e6,
-                           "integrate_late": 0.25e6}),
+                          {"sort_upsweep": 50.05e6, "integrate_deep": 0.41e6}),
 }
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",

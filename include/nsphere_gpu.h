@@ -103,9 +103,12 @@ enum {
     NS_OPT_DEEP_BLOCKS = 10, /* binned forms: workgroups (4 waves each) of the
                                 wave-per-particle kernel (1..4096, default
                                 1024).  Same result.                          */
-    NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures two steps into a HIP graph
+    NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures steps into a HIP graph
                               once (per parameter set and option state) and
-                              replays it, one launch per two steps; needs
+                              replays it: two steps per launch, or, when the
+                              run overlaps (NS_OPT_OVERLAP), one step eagerly
+                              and then 16 overlapped steps per launch (both
+                              streams' work, ending joined); needs
                               NS_OPT_SORT_ALG 0 and profiling off (else the
                               steps are launched one by one).  Same result. */
     NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
@@ -121,7 +124,7 @@ enum {
                               (ns_gpu_cap_hits).  Not in the reference, whose
                               Levi-Civita tau steps converge only to first
                               order and can run 2^28 subdivisions each       */
-    NS_OPT_OVERLAP = 11    /* binned forms, ns_gpu_run of two or more steps:
+    NS_OPT_OVERLAP = 11,   /* binned forms, ns_gpu_run of two or more steps:
                               1 (default): particles whose previous step was
                               slow (flagged NS_LVL_NOCONV, at level 6 or
                               deeper, or among the 64 innermost ranks of
@@ -131,8 +134,18 @@ enum {
                               step's radii; a merge then puts them at their
                               sorted ranks.  0: every kernel on the context's
                               stream.  Same result; with halos, per halo.
-                              Not taken with graphs, tiebreak by id,
-                              per-particle masses or learned binning.        */
+                              Not taken with tiebreak by id, per-particle
+                              masses or learned binning.                     */
+    NS_OPT_SORT_KEYS = 13  /* float64 state, one halo, npts >= 65536, the sort
+                              of a step after an integrate: 1 (default) sorts
+                              24-bit quantile keys that the integrate wrote
+                              (the radius's bucket in 4096 quantiles of the
+                              step's start radii, then 12 bits of position in
+                              the bucket): three 8-bit passes, then each run
+                              of equal keys ordered by radius; 0: the 64-bit
+                              radius keys (four passes on their high halves,
+                              then each run ordered by the low halves).  Same
+                              permutation.                                   */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;

@@ -65,9 +65,14 @@ struct ns_gpu_ctx {
     int overlap = 1;
     ns::OverlapBufs ov = {};
     hipStream_t side = nullptr;
-    hipEvent_t ev_sorted = nullptr, ev_merged = nullptr, ev_late = nullptr;
+    // fork / join events of eager steps, and a second set that only graph
+    // captures record (a captured record leaves its event unusable eagerly)
+    hipEvent_t ev[3] = {}, ev_cap[3] = {};
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
+    bool keys_q24 = false;     // ... quantile keys on qtab (else two-plane radius keys)
+    int sort_keys = 1;         // NS_OPT_SORT_KEYS
+    ns::QEntry *qtab = nullptr;
     bool levels_valid = false;
     int tiebreak = 0;
     int integrator = 0;
@@ -75,13 +80,14 @@ struct ns_gpu_ctx {
     int64_t cap_N = 8;       // NS_OPT_LANE_LEVELS: levels 0..log2(cap_N) lane-per-particle
     int deep_blocks = 1024;  // NS_OPT_DEEP_BLOCKS
     ns::Profiler prof;
-    // NS_OPT_GRAPH: ns_gpu_run replays two captured steps
+    // NS_OPT_GRAPH: ns_gpu_run replays captured steps (two per launch, or
+    // kGraphOverlapSteps overlapped ones)
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     struct GraphKey {
         ns_step_params p;
-        int cur, integrator, f32, nseg, has_mass;
+        int cur, integrator, f32, nseg, has_mass, overlapped, sort_keys;
         int64_t budget, cap_N;
         int deep_blocks;
         ns::SortConfig cfg;
@@ -265,13 +271,14 @@ void free_all(ns_gpu_ctx *c)
                     c->rf[0], c->rf[1], c->vf[0], c->vf[1], c->lf[0], c->lf[1], c->md[0], c->md[1],
                     c->mf[0], c->mf[1], c->menc, c->ms.words, c->ms.ctl, c->seg_off, c->seg_mass,
                     c->order, c->id_order, c->ov.perm, c->ov.slow, c->ov.slow_rec, c->ov.sel, c->ov.def_pos, c->ov.def_src, c->ov.late_h,
-                    c->ov.late_t, c->ov.late_slot, c->ov.ins, c->ov.found, c->ov.ctl};
+                    c->ov.late_t, c->ov.late_slot, c->ov.ins, c->ov.found, c->ov.ctl, c->qtab};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
-    for (hipEvent_t ev : {c->ev_sorted, c->ev_merged, c->ev_late})
-        if (ev)
-            (void)hipEventDestroy(ev);
+    for (int k = 0; k < 3; k++)
+        for (hipEvent_t e : {c->ev[k], c->ev_cap[k]})
+            if (e)
+                (void)hipEventDestroy(e);
     if (c->side)
         (void)hipStreamDestroy(c->side);
 }
@@ -299,9 +306,10 @@ int ensure_overlap(ns_gpu_ctx *c)
         e = dalloc(&c->ov.ctl, 8);
     if (e == hipSuccess && !c->side)
         e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    for (hipEvent_t *ev : {&c->ev_sorted, &c->ev_merged, &c->ev_late})
-        if (e == hipSuccess && !*ev)
-            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    for (int k = 0; k < 3; k++)
+        for (hipEvent_t *ev : {&c->ev[k], &c->ev_cap[k]})
+            if (e == hipSuccess && !*ev)
+                e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess)
         return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "overlap buffers (npts=%lld): %s",
                          (long long)c->n, hipGetErrorString(e));
@@ -447,11 +455,13 @@ void release_graph(ns_gpu_ctx *c)
     c->graph = nullptr;
 }
 
-ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p)
+ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p, bool overlapped)
 {
     ns_gpu_ctx::GraphKey k = {};
     k.p = *p;
     k.cur = c->cur;
+    k.overlapped = overlapped;
+    k.sort_keys = c->sort_keys;
     k.integrator = c->integrator;
     k.f32 = c->f32;
     k.nseg = c->nseg;
@@ -466,7 +476,9 @@ ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p)
 bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
 {
     return std::memcmp(&a.p, &b.p, sizeof a.p) == 0 && a.cur == b.cur && a.integrator == b.integrator &&
-           a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.budget == b.budget &&
+           a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.overlapped == b.overlapped &&
+           a.sort_keys == b.sort_keys &&
+           a.budget == b.budget &&
            a.cap_N == b.cap_N && a.deep_blocks == b.deep_blocks &&
            a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
 }
@@ -487,6 +499,11 @@ int sort_perm(ns_gpu_ctx *c, uint32_t *out = nullptr)
         c->keys_valid = false;
         return NS_OK;
     }
+    if (c->keys_valid && c->keys_q24) {
+        HIP_TRY(ns::radix_sort_perm_q24(c->sw, c->n, c->r[c->cur], out, c->stream, &c->prof));
+        c->keys_valid = false;  // the passes overwrite keys[0]
+        return NS_OK;
+    }
     if (!c->keys_valid) {
         c->prof.begin(ns::K_KEYS);
         if (c->f32)
@@ -498,6 +515,32 @@ int sort_perm(ns_gpu_ctx *c, uint32_t *out = nullptr)
     }
     HIP_TRY(ns::radix_sort_perm(c->sw, c->n, c->f32 ? 32 : 64, out, c->stream, &c->prof));
     c->keys_valid = false;  // the passes overwrite keys[0]
+    return NS_OK;
+}
+
+// Whether the integrate of a gathered step writes quantile keys.
+bool quantile_applies(const ns_gpu_ctx *c)
+{
+    return c->sort_keys && !c->f32 && c->nseg == 0 && c->n >= ns::kQMinN;
+}
+
+// The quantile table of the step's start radii (buffer cur, in the order of
+// perm) for the keys the integrate writes, when they are quantile keys.
+int quantile_table(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm)
+{
+    a.qtab = nullptr;
+    a.q_stride = 0;
+    if (!quantile_applies(c) || !a.keys)
+        return NS_OK;
+    if (!c->qtab) {
+        const hipError_t e = hipMalloc(&c->qtab, (size_t)ns::kQBuckets * sizeof(ns::QEntry));
+        if (e != hipSuccess)
+            return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "quantile table: %s",
+                             hipGetErrorString(e));
+    }
+    HIP_TRY(ns::launch_quantile_table(c->r[c->cur], perm, c->n, c->qtab, c->stream));
+    a.qtab = c->qtab;
+    a.q_stride = (uint32_t)(c->n / ns::kQBuckets);
     return NS_OK;
 }
 
@@ -549,11 +592,14 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
         c->prof.end(ns::K_MLP);
         a.order = c->order;
     }
+    if (int rc = quantile_table(c, a, gather ? c->perm : nullptr))
+        return rc;
     HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
     if (gather)
         c->last_perm = c->perm;
     c->cur = d;
     c->keys_valid = true;
+    c->keys_q24 = a.qtab != nullptr;
     c->levels_valid = true;
     return NS_OK;
 }
@@ -570,58 +616,134 @@ bool overlap_applies(const ns_gpu_ctx *c, int64_t nsteps)
 // step's late kernel: the plan, then this step's late kernel -- one chain
 // hands over to the next without a round trip through the context's stream.
 // Context's stream, after the plan: the placement of the early elements and
-// the bulk and deep kernels.
-int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p)
+// the bulk and deep kernels.  ev: the fork / join events (sorted, merged,
+// late), c->ev eagerly or c->ev_cap under a capture.
+int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev)
 {
     if (int rc = sort_perm(c))
         return rc;
-    HIP_TRY(hipEventRecord(c->ev_sorted, c->stream));
-    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_sorted, 0));
+    HIP_TRY(hipEventRecord(ev[0], c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, ev[0], 0));
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
     c->prof.begin(ns::K_MERGE, c->side);
     HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->seg_off, c->nseg, c->n,
                              c->prof.on ? c->work + ns::kWorkKernels * c->work_stride : nullptr, c->side));
     c->prof.end(ns::K_MERGE, c->side);
-    HIP_TRY(hipEventRecord(c->ev_merged, c->side));
+    HIP_TRY(hipEventRecord(ev[1], c->side));
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
     a.slow_list = c->ov.slow;
     a.slow_rec = c->ov.slow_rec;
     a.slow_count = c->ov.ctl;  // ctl[0] and ctl[3]
     ns::StepArgs late = a;
     late.keys = nullptr;  // the sentinel stays until the next merge
+    late.qtab = nullptr;
     c->prof.begin(ns::K_INT_LATE, c->side);
     HIP_TRY(ns::launch_late(late, c->integrator, c->f32, c->ov.def_pos, c->ov.def_src, c->ov.ctl + 1, c->side));
     c->prof.end(ns::K_INT_LATE, c->side);
-    HIP_TRY(hipEventRecord(c->ev_late, c->side));
-    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_merged, 0));
+    HIP_TRY(hipEventRecord(ev[2], c->side));
+    HIP_TRY(hipStreamWaitEvent(c->stream, ev[1], 0));
     c->prof.begin(ns::K_PLACE);
     HIP_TRY(ns::overlap_place(c->ov, c->perm, c->seg_off, c->nseg, c->n, c->stream));
     c->prof.end(ns::K_PLACE);
+    if (int rc = quantile_table(c, a, c->ov.perm))
+        return rc;
     HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
     c->last_perm = c->ov.perm;
     c->cur ^= 1;
     c->keys_valid = true;
+    c->keys_q24 = a.qtab != nullptr;
     c->levels_valid = true;
     return NS_OK;
 }
 
+// Overlapped steps per graph launch: even, so the double buffers return to
+// the captured parity.  Each launch ends by joining the second stream, so
+// the next launch's first sort cannot overlap the late kernel before it; 16
+// steps keep that to one step in 16.
+constexpr int kGraphOverlapSteps = 16;
+
+// Capture kGraphOverlapSteps overlapped steps and the join of the second
+// stream into c->graph_exec (the fork / join events become the graph's
+// edges between its two branches).  Capture records without running, so the
+// host-side state the steps advance is put back after recording; on success
+// the recorded steps leave it where it started (even parity flips, keys and
+// levels valid).
+int capture_overlapped(ns_gpu_ctx *c, const ns_step_params *p, const ns_gpu_ctx::GraphKey &key)
+{
+    release_graph(c);
+    const int cur = c->cur;
+    const bool keys_valid = c->keys_valid, keys_q24 = c->keys_q24, levels_valid = c->levels_valid;
+    const uint32_t *last_perm = c->last_perm;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = NS_OK;
+    for (int k = 0; k < kGraphOverlapSteps && rc == NS_OK; k++)
+        rc = overlapped_step(c, p, c->ev_cap);
+    const hipError_t ej = hipStreamWaitEvent(c->stream, c->ev_cap[2], 0);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    c->cur = cur;
+    c->keys_valid = keys_valid;
+    c->keys_q24 = keys_q24;
+    c->levels_valid = levels_valid;
+    c->last_perm = last_perm;
+    if (rc || ej != hipSuccess || e != hipSuccess) {
+        if (gr)
+            (void)hipGraphDestroy(gr);
+        if (rc)
+            return rc;
+        return set_error(NS_ERR_HIP, "ns_gpu_run: capture of overlapped steps failed: %s",
+                         hipGetErrorString(ej != hipSuccess ? ej : e));
+    }
+    c->graph = gr;
+    const hipError_t ei = hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        release_graph(c);
+        return set_error(NS_ERR_HIP, "ns_gpu_run: graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    c->gkey = key;
+    return NS_OK;
+}
+
 // nsteps >= 2 overlapped steps; the context's stream then waits for the
-// second stream and the last step's late ranks get their true keys.
-int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
+// second stream and the last step's late ranks get their true keys.  With
+// graphs: the first step eagerly, a join, then launches of
+// kGraphOverlapSteps captured steps each (which end joined), then the rest
+// eagerly.
+int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool graph)
 {
     if (int rc = ensure_overlap(c))
         return rc;
     // the slow list of the last step, from its level records (whatever form
     // ran it): the first step defers the chains that step found slow
     HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->seg_off, c->nseg, c->n, c->stream));
-    int rc = NS_OK;
-    int64_t done = 0;
-    for (; done < nsteps && rc == NS_OK; done++)
-        rc = overlapped_step(c, p);
-    if (done > 0) {
+    const int64_t launches = graph ? (nsteps - 1) / kGraphOverlapSteps : 0;
+    int rc = overlapped_step(c, p, c->ev);
+    int64_t done = 1;
+    bool joined = false;
+    if (rc == NS_OK && launches > 0) {
+        // the graph's first plan must follow this step's late kernel, an
+        // event from outside the graph: join before the first launch
+        hipError_t e = hipStreamWaitEvent(c->stream, c->ev[2], 0);
+        joined = true;
+        if (e != hipSuccess)
+            rc = set_error(NS_ERR_HIP, "ns_gpu_run: join of the second stream: %s", hipGetErrorString(e));
+        const ns_gpu_ctx::GraphKey key = graph_key(c, p, true);
+        if (rc == NS_OK && (!c->graph_exec || !same_key(key, c->gkey)))
+            rc = capture_overlapped(c, p, key);
+        for (int64_t k = 0; k < launches && rc == NS_OK; k++) {
+            if ((e = hipGraphLaunch(c->graph_exec, c->stream)) != hipSuccess)
+                rc = set_error(NS_ERR_HIP, "ns_gpu_run: graph launch: %s", hipGetErrorString(e));
+            done += kGraphOverlapSteps;
+        }
+    }
+    for (; done < nsteps && rc == NS_OK; done++) {
+        rc = overlapped_step(c, p, c->ev);
+        joined = false;
+    }
+    if (!joined) {
         // join even after a failed step, so the second stream is never left
         // running past the call
-        const hipError_t e = hipStreamWaitEvent(c->stream, c->ev_late, 0);
+        const hipError_t e = hipStreamWaitEvent(c->stream, c->ev[2], 0);
         if (rc == NS_OK && e != hipSuccess)
             rc = set_error(NS_ERR_HIP, "ns_gpu_run: join of the second stream: %s", hipGetErrorString(e));
     }
@@ -630,7 +752,8 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
         return rc;
     }
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
-    HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->stream));
+    HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->keys_q24 ? c->qtab : nullptr,
+                                 (uint32_t)(c->n / ns::kQBuckets), c->stream));
     return NS_OK;
 }
 
@@ -807,6 +930,11 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_OVERLAP takes 0 or 1");
         c->overlap = (int)value;
+        return NS_OK;
+    case NS_OPT_SORT_KEYS:
+        if (value != 0 && value != 1)
+            return set_error(NS_ERR_ARG, "NS_OPT_SORT_KEYS takes 0 or 1");
+        c->sort_keys = (int)value;
         return NS_OK;
     case NS_OPT_INTEGRATOR:
         if (value < ns::FORM_PLAIN || value > ns::FORM_BINNED_FAST)
@@ -1360,11 +1488,11 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
     if (nsteps < 0)
         return set_error(NS_ERR_ARG, "ns_gpu_run: nsteps < 0");
     const bool graph = c && c->use_graph && !c->prof.on && c->sw.config.alg == ns::kSortReduceScan && nsteps >= 3;
-    if (!graph && c && c->uploaded && overlap_applies(c, nsteps)) {
+    if (c && c->uploaded && overlap_applies(c, nsteps)) {
         if (int rc = check_params(p, "ns_gpu_run"))
             return rc;
         DeviceGuard g(c->device);
-        return run_overlapped(c, p, nsteps);
+        return run_overlapped(c, p, nsteps, graph);
     }
     if (!graph) {
         for (int64_t s = 0; s < nsteps; s++)
@@ -1380,7 +1508,7 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
         return rc;
     nsteps--;
     DeviceGuard g(c->device);
-    const ns_gpu_ctx::GraphKey key = graph_key(c, p);
+    const ns_gpu_ctx::GraphKey key = graph_key(c, p, false);
     if (!c->graph_exec || !same_key(key, c->gkey)) {
         release_graph(c);
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

@@ -85,6 +85,158 @@ __device__ inline void store_key(void *keys, int64_t, int64_t i, uint32_t k)
     static_cast<uint32_t *>(keys)[i] = k;
 }
 
+// ------------------------------------------------------------ quantile keys
+//
+// Between steps the float64 radii are sorted on a 24-bit key instead of
+// their 64 bits: the bucket of the radius in a table of kQBuckets quantiles
+// of the step's start radii (the radius keys of ranks 0, s, 2s, ... for the
+// stride s = n / kQBuckets), then kQSubBits of linear position inside the
+// bucket.  The key is a non-decreasing function of the radius key, so a
+// stable sort on it followed by a stable sort of each run of equal keys on
+// the full radius key is the stable sort on the radius.  Quantile buckets
+// hold about s particles each and the sub-bins under one particle each on
+// average, so three 8-bit passes sort it and about half of the positions
+// share their key with another.
+constexpr int kQBuckets = 4096;
+constexpr int kQSubBits = 12;
+constexpr uint32_t kQKeyMax = 0xFFFFFEu;  // keys stay below the deferral sentinel's low 24 bits
+constexpr int64_t kQMinN = 65536;         // smaller sorts keep the 64-bit keys
+constexpr int kQDBits = 20;               // the sub-bin offset is scaled to under 2^20 first
+
+// One bucket: k = radius key of its first quantile; ms = (shift << 56) | mul,
+// sub-bin = (((key - k) >> shift) * mul) >> 32 (integers only, so the map is
+// exactly monotone).
+struct QEntry {
+    uint64_t k;
+    uint64_t ms;
+};
+
+// Bucket parameters from the radius keys of its quantile (k0) and of the next
+// one (k1; for the last bucket the largest start radius, inclusive).
+__host__ __device__ inline uint64_t quantile_ms(uint64_t k0, uint64_t k1, bool last)
+{
+    const uint64_t span = k1 - k0 + (last ? 1u : 0u);  // keys k0 .. k0 + span - 1
+    if (span == 0)
+        return 0;
+    int t = 0;
+    while (((span - 1) >> t) >= ((uint64_t)1 << kQDBits))
+        t++;
+    const uint64_t d = ((span - 1) >> t) + 1;  // offsets after the shift: 0 .. d - 1
+    const uint64_t mul = ((uint64_t)1 << (32 + kQSubBits)) / d;
+    return ((uint64_t)t << 56) | mul;
+}
+
+// Largest j in [0, n) with j == 0 or key_at(j) <= k, for non-decreasing
+// key_at: a galloping search from the guess b (then bisection), so that a
+// key near its guess costs two or three reads.
+template <class KeyAt>
+__device__ inline int gallop_bucket(KeyAt key_at, int n, int b, uint64_t k)
+{
+    b = b < 0 ? 0 : (b > n - 1 ? n - 1 : b);
+    int lo, hi;
+    if (b > 0 && key_at(b) > k) {
+        hi = b;
+        lo = b - 1;
+        int step = 1;
+        while (lo > 0 && key_at(lo) > k) {
+            hi = lo;
+            step <<= 1;
+            lo = b - step > 0 ? b - step : 0;
+        }
+    } else {
+        lo = b;
+        hi = b + 1;
+        int step = 1;
+        while (hi < n && key_at(hi) <= k) {
+            lo = hi;
+            step <<= 1;
+            hi = b + step < n ? b + step : n;
+        }
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (key_at(mid) <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Buckets staged in LDS by a kernel: k[j] = key of bucket w0 + j for j < wn,
+// k[wn] = key of bucket w0 + wn (all ones past the last bucket); ms[j] the
+// bucket's sub-bin parameters (null: read from the table).
+struct QWin {
+    const uint64_t *k = nullptr;
+    const uint64_t *ms = nullptr;
+    int w0 = 0, wn = 0;
+};
+
+// Stages buckets [w0, w0 + wn) (and the key after them) of q into k (wn + 1
+// words) and ms (wn words, or null), with all threads of the block; the
+// caller synchronises the block before the first search.
+__device__ inline QWin qwin_stage(const QEntry *__restrict__ q, int w0, int wn, uint64_t *k, uint64_t *ms)
+{
+    // rounds of kPer independent loads per thread, then their stores, so a
+    // thread waits for one round trip per round (the whole table: two)
+    constexpr int kPer = 9;
+    const int nt = (int)blockDim.x;
+    for (int base = 0; base <= wn; base += kPer * nt) {
+        uint64_t kk[kPer], mm[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int j = base + u * nt + (int)threadIdx.x;
+            kk[u] = (j <= wn && w0 + j < kQBuckets) ? q[w0 + j].k : ~0ull;
+            mm[u] = (ms && j < wn) ? q[w0 + j].ms : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int j = base + u * nt + (int)threadIdx.x;
+            if (j <= wn)
+                k[j] = kk[u];
+            if (ms && j < wn)
+                ms[j] = mm[u];
+        }
+    }
+    QWin w;
+    w.k = k;
+    w.ms = ms;
+    w.w0 = w0;
+    w.wn = wn;
+    return w;
+}
+
+// 24-bit quantile key of radius key k for the particle of start rank `rank`
+// (the search starts at the bucket of its start radius): in the staged
+// buckets when they hold k's bucket, else in the table.
+__device__ inline uint32_t quantile_key(const QEntry *__restrict__ q, uint32_t stride, int64_t rank, uint64_t k,
+                                        const QWin &w = QWin())
+{
+    const int guess = (int)(rank / (int64_t)stride);
+    int b;
+    uint64_t base, ms;
+    if (w.k && (w.w0 == 0 || w.k[0] <= k) && k < w.k[w.wn]) {
+        const int j = gallop_bucket([&](int x) { return w.k[x]; }, w.wn, guess - w.w0, k);
+        b = w.w0 + j;
+        base = w.k[j];
+        ms = w.ms ? w.ms[j] : q[b].ms;
+    } else {
+        b = gallop_bucket([&](int x) { return q[x].k; }, kQBuckets, guess, k);
+        base = q[b].k;
+        ms = q[b].ms;
+    }
+    uint32_t sub = 0;
+    if (k > base) {
+        uint64_t d = (k - base) >> (ms >> 56);
+        const uint64_t dmax = ((uint64_t)1 << kQDBits) - 1;
+        d = d < dmax ? d : dmax;  // only the last bucket reaches past its span
+        const uint64_t s = (d * (ms & (((uint64_t)1 << 56) - 1))) >> 32;
+        sub = s < ((1u << kQSubBits) - 1) ? (uint32_t)s : ((1u << kQSubBits) - 1);
+    }
+    const uint32_t key = ((uint32_t)b << kQSubBits) | sub;
+    return key < kQKeyMax ? key : kQKeyMax;
+}
+
 template <typename S>
 struct KeyOf;
 template <>
@@ -173,6 +325,12 @@ hipError_t radix_sort_perm_ids(SortWork &w, int64_t n, const void *r, bool f32, 
                                uint32_t *order, uint32_t *perm_out, hipStream_t stream, Profiler *prof);
 hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_out, hipStream_t stream,
                            Profiler *prof);
+// Sorts the 24-bit quantile keys in keys[0] (quantile_key of the radii r,
+// n >= kQMinN) stably, then orders each run of equal keys by the radius key
+// of r and input position: the stable sort on r.  Uses both key and payload
+// buffers, the tile-count buffer and counters[48].
+hipError_t radix_sort_perm_q24(SortWork &w, int64_t n, const double *r, uint32_t *perm_out, hipStream_t stream,
+                               Profiler *prof);
 
 // Kernel-family ids for profiling.
 enum KernelId : int {

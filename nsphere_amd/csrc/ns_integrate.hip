@@ -83,17 +83,47 @@ __device__ inline uint32_t wave_ticket(uint32_t *ticket)
     return __shfl(q, 0, 64);
 }
 
+// The sort key of new radius rs at rank i: a quantile key when the step
+// has a table (searched in the kernel's staged buckets w first), else the
+// two-plane radius key.
+__device__ inline void write_key(const StepArgs &a, int64_t i, double rs, const QWin &w)
+{
+    if (a.qtab)
+        static_cast<uint32_t *>(a.keys)[i] = quantile_key(a.qtab, a.q_stride, i, radius_key(rs), w);
+    else
+        store_key(a.keys, a.n, i, radius_key(rs));
+}
+__device__ inline void write_key(const StepArgs &a, int64_t i, float rs, const QWin &)
+{
+    store_key(a.keys, a.n, i, radius_key(rs));
+}
+
+// Quantile buckets staged around the first rank of a block whose threads
+// take consecutive ranks (a block's particles move a few buckets a step):
+// kQWin buckets from kQWin / 2 - 1 below the first rank's bucket.  None
+// without a table or when threads take ranks in another order.  Every thread
+// of the block calls it; the block synchronises before the first search.
+constexpr int kQWin = 64;
+__device__ inline QWin stage_rank_window(const StepArgs &a, int64_t first, uint64_t *sk, uint64_t *sms)
+{
+    if (!a.qtab || a.order)
+        return QWin();
+    int w0 = (int)(first / (int64_t)a.q_stride) - kQWin / 2 + 1;
+    w0 = w0 < 0 ? 0 : (w0 > kQBuckets - kQWin ? kQBuckets - kQWin : w0);
+    return qwin_stage(a.qtab, w0, kQWin, sk, sms);
+}
+
 // slow_tag: what the slow list records for a slow result (the rank, or the
 // late kernel's slot | kDeferBit); local: the rank within its halo.
 template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit,
-                                    uint32_t slow_tag, int64_t local)
+                                    uint32_t slow_tag, int64_t local, const QWin &qw = QWin())
 {
     const S rs = (S)r;
     static_cast<S *>(a.r1)[i] = rs;
     static_cast<S *>(a.v1)[i] = (S)v;
     if (a.keys)
-        store_key(a.keys, a.n, i, radius_key(rs));
+        write_key(a, i, rs, qw);
     if (a.lvl)
         a.lvl[i] = (uint8_t)rec;
     if (hit)
@@ -137,7 +167,11 @@ __device__ inline void carry_ell_id(const StepArgs &a, int64_t i, double ell)
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
 {
-    const int64_t i = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t first = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x;
+    const int64_t i = first + threadIdx.x;
+    __shared__ uint64_t s_qk[kQWin + 1], s_qms[kQWin];
+    const QWin qw = stage_rank_window(a, first, s_qk, s_qms);
+    __syncthreads();
     unsigned long long spent = 0;
     if (i < a.n) {
         double r, v, ell;
@@ -152,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_plain(StepArgs a)
             rec = adaptive_lc<FAST>(ph, a, r, v, r_new, v_new, bud);
         else
             rec = adaptive_leap<FAST>(ph, a, r, v, r_new, v_new, bud, 0);
-        write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i, qw);
         spent = (unsigned long long)(a.budget - bud.left);
     }
     add_work(a, 0, spent);
@@ -166,8 +200,10 @@ constexpr int kBulkPPT = 2;
 template <bool GATHER, bool FAST, typename S>
 __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
 {
-    const int64_t base =
-        (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * (kThreads * kBulkPPT) + threadIdx.x;
+    const int64_t first = (int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * (kThreads * kBulkPPT);
+    const int64_t base = first + threadIdx.x;
+    __shared__ uint64_t s_qk[kQWin + 1], s_qms[kQWin];
+    const QWin qw = stage_rank_window(a, first, s_qk, s_qms);
     int64_t idx[kBulkPPT], src[kBulkPPT];
     bool valid[kBulkPPT], defer[kBulkPPT];
 #pragma unroll
@@ -197,6 +233,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
             id[k] = GATHER ? a.id0[src[k]] : 0u;
         }
     }
+    __syncthreads();  // the staged buckets (their loads overlap the gathers)
     unsigned long long spent = 0;
 #pragma unroll
     for (int k = 0; k < kBulkPPT; k++) {
@@ -207,8 +244,12 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
             a.id1[i] = id[k];
         }
         if (valid[k] && defer[k]) {
-            if (a.keys)
-                store_key(a.keys, a.n, i, ~(typename KeyOf<S>::type)0);
+            if (a.keys) {
+                if (a.qtab)
+                    static_cast<uint32_t *>(a.keys)[i] = ~0u;
+                else
+                    store_key(a.keys, a.n, i, ~(typename KeyOf<S>::type)0);
+            }
         } else if (valid[k]) {
             const RankMass rm = rank_mass(a, i);
             if (takes_levi_civita(a, rm, r[k], ell[k])) {
@@ -221,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
                 if (rec == kEscalate)
                     to_lf = true;
                 else
-                    write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
+                    write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i, qw);
                 spent += (unsigned long long)(a.budget - bud.left);
             }
         }
@@ -251,6 +292,14 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_lanes(StepArgs a)
     const uint32_t n_lc = a.q_count[1];
     const uint32_t waves = gridDim.x * (kThreads / 64);
     const uint32_t w = (blockIdx.x * kThreads + threadIdx.x) >> 6;
+    // the queued ranks are scattered: the keys of every quantile bucket, for
+    // blocks that take a particle
+    __shared__ uint64_t s_qk[kQBuckets + 1];
+    QWin qw;
+    if (a.qtab && blockIdx.x * (uint32_t)kThreads < n_lc) {
+        qw = qwin_stage(a.qtab, 0, kQBuckets, s_qk, nullptr);
+        __syncthreads();
+    }
     unsigned long long spent = 0;
     for (uint32_t base = w * 64u; base < n_lc; base += waves * 64u) {
         const uint32_t q = base + (uint32_t)lane;
@@ -270,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_lanes(StepArgs a)
             if (rec == kEscalate)
                 escalate = true;
             else
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit, i, rm.k.i);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit, i, rm.k.i, qw);
             spent += (unsigned long long)(a.budget - bud.left);
         }
         wave_push(a.q_lcw, a.q_count + 3, escalate, i);
@@ -327,6 +376,10 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
     const uint32_t j = blockIdx.x;
     if (j >= *count)
         return;
+    // the late chains bound an overlapped step when they run long, and their
+    // waves share SIMDs with the next step's sort and bulk kernels: their
+    // instructions issue first
+    __builtin_amdgcn_s_setprio(3);
     const int64_t i = def_pos[j], src = def_src[j];
     const double r = (double)static_cast<const S *>(a.r0)[src];
     const double v = (double)static_cast<const S *>(a.v0)[src];
@@ -358,7 +411,8 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
 
 // One wave running the fast Levi-Civita micro-step (LcFast::step, the loop
 // body of the late kernel's chains) `steps` times on a scalar trip counter,
-// as the wave form does; out keeps the result live.
+// as the wave form does; out keeps the result live, the elapsed-time sum
+// included (the chains need it, so it is part of their floor).
 __global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__restrict__ out)
 {
     Phys ph;
@@ -369,7 +423,22 @@ __global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__res
     lf.init(ph, 1.0e-3 / (double)(steps > 0 ? steps : 1), 0.1, -0.2);
     repeat_uniform(steps, [&]() { lf.step(); });
     if ((threadIdx.x & 63) == 0)
-        out[0] = lf.rho + lf.v_end();
+        out[0] = lf.rho + lf.v_end() + lf.t_end(0.0);
+}
+
+__global__ void k_quantile_table(const double *__restrict__ r0, const uint32_t *__restrict__ perm, int64_t n,
+                                 uint32_t stride, QEntry *__restrict__ q)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= kQBuckets)
+        return;
+    const bool last = b == kQBuckets - 1;
+    const int64_t i0 = (int64_t)b * stride, i1 = last ? n - 1 : i0 + stride;
+    const int64_t p0 = perm ? (int64_t)(perm[i0] & kIndexMask) : i0;
+    const int64_t p1 = perm ? (int64_t)(perm[i1] & kIndexMask) : i1;
+    const uint64_t k0 = radius_key(r0[p0]), k1 = radius_key(r0[p1]);
+    q[b].k = k0;
+    q[b].ms = quantile_ms(k0, k1, last);
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -592,6 +661,15 @@ hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *de
             hipLaunchKernelGGL((k_integrate_late<false, double>), grid, block, 0, stream, a, def_pos, def_src,
                                count);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_quantile_table(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, hipStream_t stream)
+{
+    if (n < kQMinN)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_quantile_table, dim3(kQBuckets / 256), dim3(256), 0, stream, r0, perm, n,
+                       (uint32_t)(n / kQBuckets), q);
     return hipGetLastError();
 }
 

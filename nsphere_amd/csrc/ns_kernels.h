@@ -19,6 +19,11 @@ struct StepArgs {
     void *r1, *v1, *l1;          // state in the new rank order
     uint32_t *id1;
     void *keys;                  // radius keys of the new r (may be null)
+    // float64 state (else null): the keys are 24-bit quantile keys on this
+    // table of the start radii (quantile_key; the key sentinel is all ones)
+    // instead of two-plane radius keys
+    const QEntry *qtab;
+    uint32_t q_stride;
     uint8_t *lvl;                // level records (may be null)
     unsigned long long *budget_hits;
     int64_t n;
@@ -181,9 +186,12 @@ hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *de
                        const uint32_t *count, hipStream_t stream);
 // One wave running the late kernel's fast Levi-Civita micro-step `steps` times.
 hipError_t launch_chain_floor(int64_t steps, double *out, hipStream_t stream);
+// The quantile table of the start radii r0 in the order of perm (rank ->
+// position, kDeferBit ignored; null: r0 is in rank order), n >= kQMinN.
+hipError_t launch_quantile_table(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, hipStream_t stream);
 // Radius keys of the deferred ranks of the last step (replacing the sentinel).
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
-                            hipStream_t stream);
+                            const QEntry *qtab, uint32_t q_stride, hipStream_t stream);
 template <typename S>
 hipError_t launch_gather_state(const S *r0, const S *v0, const S *l0, const uint32_t *id0, const uint32_t *perm,
                                S *r1, S *v1, S *l1, uint32_t *id1, int64_t n, hipStream_t stream);

@@ -493,11 +493,18 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_flag(OverlapBufs o)
 
 template <typename S>
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_fix_keys(OverlapBufs o, const S *__restrict__ r,
-                                                                   void *__restrict__ keys, int64_t n)
+                                                                   void *__restrict__ keys, int64_t n,
+                                                                   const QEntry *__restrict__ qtab, uint32_t stride)
 {
     const uint32_t j = threadIdx.x;
     if (j < o.ctl[1]) {
         const uint32_t t = o.def_pos[j];
+        if constexpr (sizeof(S) == 8) {
+            if (qtab) {
+                static_cast<uint32_t *>(keys)[t] = quantile_key(qtab, stride, t, radius_key(r[t]));
+                return;
+            }
+        }
         store_key(keys, n, t, radius_key(r[t]));
     }
 }
@@ -568,14 +575,14 @@ hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int
 }
 
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
-                            hipStream_t stream)
+                            const QEntry *qtab, uint32_t q_stride, hipStream_t stream)
 {
     if (f32)
         hipLaunchKernelGGL(k_overlap_fix_keys<float>, dim3(1), dim3(kPlanThreads), 0, stream, o,
-                           static_cast<const float *>(r), keys, n);
+                           static_cast<const float *>(r), keys, n, nullptr, 0u);
     else
         hipLaunchKernelGGL(k_overlap_fix_keys<double>, dim3(1), dim3(kPlanThreads), 0, stream, o,
-                           static_cast<const double *>(r), keys, n);
+                           static_cast<const double *>(r), keys, n, qtab, q_stride);
     return hipGetLastError();
 }
 

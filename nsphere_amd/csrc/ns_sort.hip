@@ -856,15 +856,35 @@ constexpr int kFxAll = kFxRows + 1;     // rows read, the last one ahead of the 
 constexpr int kFxWaves = 4;             // independent waves per workgroup
 constexpr int kFxNone = 1 << 30;        // "no run start" in the uniform tables
 
-__global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *__restrict__ hs,
-                                                                 const uint32_t *__restrict__ lo,
+// Sources of the key that orders a run of equal sort keys, by input
+// position: the low plane of two-plane 64-bit keys, or the radius key of the
+// float64 radius itself (after a sort on quantile keys).
+struct TieLowPlane {
+    using T = uint32_t;
+    const uint32_t *lo;
+    __device__ T operator()(uint32_t p) const { return lo[p]; }
+};
+struct TieRadius {
+    using T = uint64_t;
+    const double *r;
+    __device__ T operator()(uint32_t p) const { return radius_key(r[p]); }
+};
+
+// The deferral sentinel (all ones, on no radius) marks a run of exact ties:
+// its tie keys are never read (the late kernel may still be writing those
+// particles' radii).
+constexpr uint32_t kSentinelWord = 0xFFFFFFFFu;
+
+template <class Lo>
+__global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *__restrict__ hs, Lo lo,
                                                                  const uint32_t *__restrict__ perm_hi,
                                                                  uint32_t *__restrict__ out, int64_t n,
                                                                  const uint32_t *__restrict__ seg_start, int nseg,
                                                                  uint32_t *__restrict__ long_runs,
                                                                  uint32_t *__restrict__ long_count)
 {
-    __shared__ uint32_t s_l[kFxWaves][kFxAll * 64];
+    using T = typename Lo::T;
+    __shared__ T s_l[kFxWaves][kFxAll * 64];
     __shared__ uint32_t s_uns[kFxWaves][kFxAll * 16];       // bytes by run start: the run has a descent
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t w0 = ((int64_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * kFxWaves + wv) * kFxTile;
@@ -923,15 +943,15 @@ __global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *_
     const uint64_t le = (2ull << lane) - 1;  // bits 0..lane
     // low halves (shared runs only), run bounds, descents
     int ra[kFxAll], re[kFxAll];
-    uint32_t lprev_row = 0;
+    T lprev_row = 0;
 #pragma unroll
     for (int r = 0; r < kFxAll; r++) {
         const int64_t j = w0 + r * 64 + lane;
         const bool hd = (M[r] >> lane) & 1;
         const bool next_hd = lane < 63 ? ((M[r] >> (lane + 1)) & 1) : (M[r + 1] & 1);
-        const uint32_t l = (j < n && !(hd && next_hd)) ? lo[p[r]] : 0u;
+        const T l = (j < n && !(hd && next_hd) && h[r] != kSentinelWord) ? lo(p[r]) : (T)0;
         s_l[wv][r * 64 + lane] = l;
-        uint32_t lprev = __shfl_up(l, 1, 64);
+        T lprev = __shfl_up(l, 1, 64);
         if (lane == 0)
             lprev = lprev_row;
         lprev_row = __shfl(l, 63, 64);
@@ -958,11 +978,11 @@ __global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *_
             continue;
         }
         int rank = jl - a;
-        if (uns[a]) {  // rank by (low half, position) within the run
-            const uint32_t lk = s_l[wv][jl];
+        if (uns[a]) {  // rank by (tie key, position) within the run
+            const T lk = s_l[wv][jl];
             rank = 0;
             for (int q = a; q < e; q++) {
-                const uint32_t lq = s_l[wv][q];
+                const T lq = s_l[wv][q];
                 rank += (lq < lk || (lq == lk && q < jl)) ? 1 : 0;
             }
         }
@@ -971,17 +991,22 @@ __global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *_
 }
 
 // The runs k_split_fixup left: one workgroup per run (a stride of runs per
-// workgroup).  The run's end is found by a scan; a run already in low-half
-// order (exact ties, the usual case) keeps its order; otherwise it is sorted
-// stably by its low halves, four 8-bit passes between the run's own slices
-// of two scratch pairs, (l0, out) and (l1, p1), ending in out.
+// workgroup).  The run's end is found by a scan; a run already in tie-key
+// order (exact ties, the deferral sentinel, the usual cases) keeps its order;
+// otherwise it is sorted stably by its tie keys, 8-bit passes (four for the
+// low plane, eight for a radius key) between the run's own slices of out and
+// p1, ending in out.  Each pass gathers the tie keys through the permutation
+// it reads, so the run needs no key staging.
+template <class Lo>
 __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
-    const uint32_t *__restrict__ hs, const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm_hi,
-    uint32_t *__restrict__ out, int64_t n, const uint32_t *__restrict__ seg_start, int nseg,
-    const uint32_t *__restrict__ long_runs, const uint32_t *__restrict__ long_count, uint32_t *__restrict__ l0,
-    uint32_t *__restrict__ l1, uint32_t *__restrict__ p1)
+    const uint32_t *__restrict__ hs, Lo lo, const uint32_t *__restrict__ perm_hi, uint32_t *__restrict__ out,
+    int64_t n, const uint32_t *__restrict__ seg_start, int nseg, const uint32_t *__restrict__ long_runs,
+    const uint32_t *__restrict__ long_count, uint32_t *__restrict__ p1)
 {
+    using T = typename Lo::T;
     constexpr int WAVES = kLongThreads / 64;
+    constexpr int kPasses = (int)sizeof(T);
+    static_assert(kPasses % 2 == 0, "an even pass count ends in out");
     __shared__ uint32_t s_wcnt[WAVES][kRadix];
     __shared__ uint32_t s_off[kRadix], s_tot[kRadix];
     __shared__ uint32_t s_wave[4];
@@ -1015,22 +1040,18 @@ __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
             }
         }
         const int64_t m = e - a;
+        for (int64_t k = tid; k < m; k += kLongThreads)
+            out[a + k] = perm_hi[a + k];
+        if (h == kSentinelWord)
+            continue;  // exact ties: input order
         bool unsorted = false;
-        for (int64_t k = tid; k < m; k += kLongThreads) {
-            const uint32_t p = perm_hi[a + k];
-            l0[a + k] = lo[p];
-            out[a + k] = p;
-        }
-        __syncthreads();
         for (int64_t k = tid + 1; k < m; k += kLongThreads)
-            unsorted |= l0[a + k] < l0[a + k - 1];
+            unsorted |= lo(perm_hi[a + k]) < lo(perm_hi[a + k - 1]);
         if (!__syncthreads_or(unsorted))
             continue;  // in order already: out holds it
-        for (int pass = 0; pass < 4; pass++) {
+        for (int pass = 0; pass < kPasses; pass++) {
             const int shift = 8 * pass;
-            const uint32_t *sl = (pass & 1) ? l1 : l0;
             const uint32_t *sp = (pass & 1) ? p1 : out;
-            uint32_t *dl = (pass & 1) ? l0 : l1;
             uint32_t *dp = (pass & 1) ? out : p1;
             for (int i = tid; i < WAVES * kRadix; i += kLongThreads)
                 (&s_wcnt[0][0])[i] = 0;
@@ -1038,7 +1059,7 @@ __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
                 s_tot[tid] = 0;
             __syncthreads();
             for (int64_t k = tid; k < m; k += kLongThreads)
-                atomicAdd(&s_tot[(sl[a + k] >> shift) & (kRadix - 1)], 1u);
+                atomicAdd(&s_tot[(uint32_t)(lo(sp[a + k]) >> shift) & (kRadix - 1)], 1u);
             __syncthreads();
             const uint32_t ex = digit_exclusive_scan(tid < kRadix ? s_tot[tid] : 0u, s_wave);
             if (tid < kRadix)
@@ -1049,9 +1070,8 @@ __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
             for (int64_t base = 0; base < m; base += kLongThreads) {
                 const int64_t k = base + tid;
                 const bool valid = k < m;
-                const uint32_t lv = valid ? sl[a + k] : 0u;
                 const uint32_t pv = valid ? sp[a + k] : 0u;
-                const uint32_t d = (lv >> shift) & (kRadix - 1);
+                const uint32_t d = valid ? (uint32_t)(lo(pv) >> shift) & (kRadix - 1) : 0u;
                 const uint64_t peers = match_digit(d, __ballot(valid));
                 const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : lane;
                 if (valid && lane == leader)
@@ -1069,7 +1089,6 @@ __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
                 __syncthreads();
                 if (valid) {
                     const uint32_t pos = s_off[d] + s_wcnt[wave][d] + (uint32_t)__popcll(peers & lt);
-                    dl[a + pos] = lv;
                     dp[a + pos] = pv;
                 }
                 __syncthreads();
@@ -1084,6 +1103,35 @@ __global__ __launch_bounds__(kLongThreads) void k_split_long_runs(
     }
 }
 
+// The run stage after the passes: hs = the sorted sort keys, perm_hi = the
+// passes' permutation (no alias of out or p1).
+template <class Lo>
+hipError_t order_runs(SortWork &w, int64_t n, const uint32_t *hs, Lo lo, const uint32_t *perm_hi,
+                      uint32_t *perm_out, uint32_t *p1, uint32_t *count, hipStream_t stream, Profiler *prof)
+{
+    const uint32_t *seg_start = w.seg.sizes.empty() ? nullptr : w.seg.seg_start;
+    const int nseg = (int)w.seg.sizes.size();
+    // the tile counts are spent: their buffer lists the long runs (at most
+    // one per wave tile of kFxTile positions).  Onesweep's lookback words share
+    // that buffer, so the next onesweep sort must clear it first (epoch 0).
+    uint32_t *long_runs = reinterpret_cast<uint32_t *>(w.lookback);
+    w.epoch = 0;
+    if (prof)
+        prof->begin(K_SORT_FIXUP);
+    const int64_t waves = (n + kFxTile - 1) / kFxTile;
+    hipLaunchKernelGGL(k_split_fixup<Lo>, dim3((unsigned)((waves + kFxWaves - 1) / kFxWaves)), dim3(64 * kFxWaves),
+                       0, stream, hs, lo, perm_hi, perm_out, n, seg_start, nseg, long_runs, count);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_split_long_runs<Lo>, dim3(kLongBlocks), dim3(kLongThreads), 0, stream, hs, lo, perm_hi,
+                           perm_out, n, seg_start, nseg, long_runs, count, p1);
+        e = hipGetLastError();
+    }
+    if (prof)
+        prof->end(K_SORT_FIXUP);
+    return e;
+}
+
 // The stable sort of the two-plane 64-bit keys in keys[0] (see above).
 hipError_t sort_split64(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t stream, Profiler *prof)
 {
@@ -1096,28 +1144,7 @@ hipError_t sort_split64(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t 
     if (e != hipSuccess)
         return e;
     const uint32_t *hs = reinterpret_cast<const uint32_t *>(w.keys[0]);
-    const uint32_t *lo = hs + n;
-    const uint32_t *seg_start = w.seg.sizes.empty() ? nullptr : w.seg.seg_start;
-    const int nseg = (int)w.seg.sizes.size();
-    // the tile counts are spent: their buffer lists the long runs (at most
-    // one per wave tile of kFxTile positions).  Onesweep's lookback words share
-    // that buffer, so the next onesweep sort must clear it first (epoch 0).
-    uint32_t *long_runs = reinterpret_cast<uint32_t *>(w.lookback);
-    w.epoch = 0;
-    if (prof)
-        prof->begin(K_SORT_FIXUP);
-    const int64_t waves = (n + kFxTile - 1) / kFxTile;
-    hipLaunchKernelGGL(k_split_fixup, dim3((unsigned)((waves + kFxWaves - 1) / kFxWaves)), dim3(64 * kFxWaves), 0,
-                       stream, hs, lo, perm_hi, perm_out, n, seg_start, nseg, long_runs, count);
-    if ((e = hipGetLastError()) == hipSuccess) {
-        uint32_t *l0 = reinterpret_cast<uint32_t *>(w.keys[1]);
-        hipLaunchKernelGGL(k_split_long_runs, dim3(kLongBlocks), dim3(kLongThreads), 0, stream, hs, lo, perm_hi,
-                           perm_out, n, seg_start, nseg, long_runs, count, l0, l0 + n, w.vals[1]);
-        e = hipGetLastError();
-    }
-    if (prof)
-        prof->end(K_SORT_FIXUP);
-    return e;
+    return order_runs(w, n, hs, TieLowPlane{hs + n}, perm_hi, perm_out, w.vals[1], count, stream, prof);
 }
 
 } // namespace
@@ -1167,6 +1194,23 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
     if (key_bits == 32)
         return sort_typed<uint32_t>(w, n, perm_out, stream, prof);
     return sort_split64(w, n, perm_out, stream, prof);
+}
+
+hipError_t radix_sort_perm_q24(SortWork &w, int64_t n, const double *r, uint32_t *perm_out, hipStream_t stream,
+                               Profiler *prof)
+{
+    if (n <= 0)
+        return hipSuccess;
+    uint32_t *count = w.counters + kLongRunCounter;
+    // three passes on the low 24 bits; the last keeps its keys, which end in
+    // keys[1] (keys[0] -> keys[1] -> keys[0] -> keys[1]), and its payload
+    // lands in vals[1] (vals[1] -> vals[0] -> vals[1])
+    uint32_t *perm_hi = w.vals[1];
+    hipError_t e = sort_typed<uint32_t>(w, n, perm_hi, stream, prof, 3, true, false, true, count);
+    if (e != hipSuccess)
+        return e;
+    const uint32_t *hs = reinterpret_cast<const uint32_t *>(w.keys[1]);
+    return order_runs(w, n, hs, TieRadius{r}, perm_hi, perm_out, w.vals[0], count, stream, prof);
 }
 
 hipError_t radix_sort_perm_ids(SortWork &w, int64_t n, const void *r, bool f32, const uint32_t *id,

@@ -1,10 +1,14 @@
-"""NS_OPT_GRAPH: ns_gpu_run replays two captured steps per graph launch.
+"""NS_OPT_GRAPH: ns_gpu_run replays captured steps.
 
-Every replay must equal the eagerly launched steps bit for bit: several runs
-in a row with downloads, energies and option changes between them (each
-forces a new capture or a replay of the existing one), odd and even step
-counts, both precisions, an ensemble, and per-particle masses (whose scan
-keeps its lookback epoch in device memory so that replays stay valid).
+Without the second stream each graph launch replays two steps; an overlapped
+run (binned forms, NS_OPT_OVERLAP) takes one step eagerly and then replays 16
+overlapped steps per launch, the second stream's plan and late kernels as a
+branch of the graph.  Every replay must equal the eagerly launched steps bit
+for bit: several runs in a row with downloads, energies and option changes
+between them (each forces a new capture or a replay of the existing one), odd
+and even step counts, both precisions, an ensemble, and per-particle masses
+(whose scan keeps its lookback epoch in device memory so that replays stay
+valid).
 """
 import numpy as np
 import pytest
@@ -66,8 +70,42 @@ def test_graph_replays_equal_eager(ns, precision, form):
         ctx.set_option(ns.NS_OPT_PRECISION, precision)
         ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
         ctx.set_option(ns.NS_OPT_BUDGET, 1 << 16)
+        ctx.set_option(ns.NS_OPT_OVERLAP, 0)
 
     run_both(ns, state, setup, [(p, 7), (p, 4), (p2, 5), (p, 3), (p, 1), (p, 6)])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("form", [1, 2])
+def test_graph_overlapped_equal_eager(ns, precision, form):
+    """Runs of 17 steps or more replay captured overlapped steps: 33 = 1 + 2 x 16
+    (two launches, none left over), 20 and 40 (launches then eager steps), 17
+    with another dt (a new capture), 16 and 2 (eager only, after replays)."""
+    state = sorted_ic(ns, 0, 200_000, 33)
+    p = ns.unit_params()
+    p2 = p.replace(dt=5e-4)
+
+    def setup(ctx):
+        ctx.set_option(ns.NS_OPT_PRECISION, precision)
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, form)
+        ctx.set_option(ns.NS_OPT_OVERLAP, 1)
+
+    run_both(ns, state, setup, [(p, 33), (p, 20), (p2, 17), (p, 40), (p, 16), (p, 2)])
+
+
+def test_graph_overlapped_ensemble(ns):
+    sizes = [30_000, 5_000, 60_000]
+    parts = [sorted_ic(ns, h % 2, s, 50 + h) for h, s in enumerate(sizes)]
+    base = np.cumsum([0] + sizes[:-1])
+    state = [np.concatenate([pt[k] + (b if k == 3 else 0) for pt, b in zip(parts, base)]) for k in range(4)]
+    state[3] = state[3].astype(np.uint32)
+    p = ns.unit_params()
+
+    def ens(ctx):
+        ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+        ctx.set_halos(sizes, [1.0, 2.0, 0.5])
+
+    run_both(ns, state, ens, [(p, 18), (p, 35)])
 
 
 def test_graph_ensemble_and_masses(ns):
@@ -80,6 +118,7 @@ def test_graph_ensemble_and_masses(ns):
 
     def ens(ctx):
         ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+        ctx.set_option(ns.NS_OPT_OVERLAP, 0)
         ctx.set_halos(sizes, [1.0, 2.0, 0.5, 1.0])
 
     run_both(ns, state, ens, [(p, 5), (p, 4)])
