@@ -34,7 +34,7 @@ namespace {
 constexpr int kPlanThreads = 1024;  // == kSlowCap: one thread per entry
 constexpr int kSearchThreads = 256;
 constexpr int kShiftThreads = 256;
-constexpr int kShiftItems = 4;
+constexpr int kShiftItems = 8;
 constexpr int kShiftTile = kShiftThreads * kShiftItems;
 constexpr int kShiftHalos = 64;  // halo bounds of a shift tile held in LDS
 static_assert(kPlanThreads == kSlowCap, "one plan thread per slow-list entry");
@@ -406,19 +406,27 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Ha
 __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, Halos g,
                                                                 const uint32_t *__restrict__ sorted)
 {
-    __shared__ uint64_t s_ins[kSlowCap];
+    __shared__ uint64_t s_all[kSlowCap];
     __shared__ uint32_t s_start[kShiftHalos + 1];
     __shared__ uint32_t s_before[kShiftHalos], s_kh[kShiftHalos];  // each staged halo's run of pairs
+    __shared__ uint32_t s_wlo[kShiftHalos], s_whi[kShiftHalos];    // ... and its pairs inside the tile
     __shared__ uint32_t s_rng[4];  // first halo, halo count, first insertion pair, pair count
-    const uint32_t K = o.ctl[2];
+    uint32_t K = o.ctl[2];
+    K = K < (uint32_t)kSlowCap ? K : (uint32_t)kSlowCap;
     const uint32_t b0 = blockIdx.x * (uint32_t)kShiftTile;
     if (b0 >= g.n)
         return;
     const uint32_t b1 = b0 + (uint32_t)kShiftTile < g.n ? b0 + (uint32_t)kShiftTile : g.n;
+    // every insertion pair into LDS (coalesced, one round trip), so that the
+    // tile's range of pairs is found by searches in LDS rather than chains
+    // of dependent global loads
+    for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads)
+        s_all[q] = o.ins[q];
+    __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t h0 = g.of(b0), h1 = g.of(b1 - 1);
-        const uint32_t i0 = count_lt(o.ins, K, hl_pack(h0, 0));
-        const uint32_t i1 = count_le(o.ins, K, hl_pack(h1, ~0u));
+        const uint32_t i0 = count_lt(s_all, K, hl_pack(h0, 0));
+        const uint32_t i1 = count_le(s_all, K, hl_pack(h1, ~0u));
         s_rng[0] = h0;
         s_rng[1] = h1 - h0 + 1;
         s_rng[2] = i0;
@@ -427,8 +435,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
     __syncthreads();
     const uint32_t h0 = s_rng[0], nh = s_rng[1], i0 = s_rng[2], ni = s_rng[3];
     const bool staged = nh <= (uint32_t)kShiftHalos;
-    for (uint32_t q = threadIdx.x; q < ni; q += kShiftThreads)
-        s_ins[q] = o.ins[i0 + q];
+    const uint64_t *s_ins = s_all + i0;  // the tile's pairs
     if (staged)
         for (uint32_t q = threadIdx.x; q <= nh; q += kShiftThreads)
             s_start[q] = g.start(h0 + q);
@@ -436,8 +443,22 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
     if (staged) {
         for (uint32_t q = threadIdx.x; q < nh; q += kShiftThreads) {
             const uint32_t before = count_lt(s_ins, ni, hl_pack(h0 + q, 0));
+            const uint32_t kh = count_le(s_ins, ni, hl_pack(h0 + q, ~0u)) - before;
             s_before[q] = before;
-            s_kh[q] = count_le(s_ins, ni, hl_pack(h0 + q, ~0u)) - before;
+            s_kh[q] = kh;
+            // the halo's pairs before the tile's first local index ef number
+            // wlo; [wlo, whi) are those up to its last one el: an element
+            // searches only those (usually none)
+            const uint32_t hs = s_start[q], he = s_start[q + 1];
+            uint32_t wlo = 0, whi = 0;
+            if (he > hs) {
+                const uint32_t ef = b0 > hs ? b0 - hs : 0;
+                const uint32_t el = (b1 < he ? b1 : he) - 1 - hs;
+                wlo = count_lt(s_ins + before, kh, hl_pack(h0 + q, ef));
+                whi = count_le(s_ins + before, kh, hl_pack(h0 + q, el));
+            }
+            s_wlo[q] = wlo;
+            s_whi[q] = whi;
         }
         __syncthreads();
     }
@@ -446,7 +467,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
         const uint32_t p = b0 + (uint32_t)it * kShiftThreads + threadIdx.x;
         if (p >= b1)
             break;
-        uint32_t hs, he, before, kh;
+        uint32_t hs, he, before, kh, wlo, whi;
         if (staged) {
             uint32_t lo = 0, hi = nh;  // s_start[lo] <= p < s_start[hi]
             while (hi - lo > 1) {
@@ -460,17 +481,23 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
             he = s_start[lo + 1];
             before = s_before[lo];
             kh = s_kh[lo];
+            wlo = s_wlo[lo];
+            whi = s_whi[lo];
         } else {
             const uint32_t h = g.of(p);
             hs = g.start(h);
             he = g.start(h + 1);
             before = count_lt(s_ins, ni, hl_pack(h, 0));
             kh = count_le(s_ins, ni, hl_pack(h, ~0u)) - before;
+            wlo = 0;
+            whi = kh;
         }
         const uint32_t e = p - hs;
         if (e < he - hs - kh) {
-            // insertion points of this halo at or before e: its run is s_ins[before, before + kh)
-            uint32_t lo = 0, hi = kh;
+            // insertion points of this halo at or before e: its run is
+            // s_ins[before, before + kh), of which those before wlo lie
+            // below e and those from whi on above it
+            uint32_t lo = wlo, hi = whi;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if ((uint32_t)s_ins[before + mid] <= e)
