@@ -206,13 +206,21 @@ def cpu_p1(ns, seed: int, n: int = 100_000, steps: int = 1000, wall_limit_s: flo
                       f"(--sort 1), no work cap; {threads} OpenMP threads on {cpu_info()}"}
 
 
+def graph_steps_for(warmup: int) -> int:
+    """Overlapped steps per graph launch: the most (up to 16, even) that the
+    warmup run replays once after its first eager step, so that the capture
+    happens there and not in the timed region."""
+    return max(2, min(16, (warmup - 1) // 2 * 2))
+
+
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
             mass=None, halos=None, graph=False, autotune=0, mlp=None, overlap=True) -> dict:
     """Time `steps` steps after `warmup`.  HIP-event timing of the roofline
     kernel families (ROOFLINE_KERNELS) runs over the timed region, unless
-    graph: then the timed region replays captured steps (no per-kernel
-    events).  Every other family is timed over AFTER_STEPS profiled steps
-    after the timed region."""
+    graph: then the warmup captures the steps, the timed region replays them
+    (graph nodes carry no per-kernel events) and the roofline families are
+    timed over profiled eager steps right after it.  Every other family is
+    timed over AFTER_STEPS profiled steps after the timed region."""
     n = state[0].size
     ctx = ns.GpuContext(n)
     # one host sync (and progress line) per chunk: each ns_gpu_run call ends
@@ -224,6 +232,13 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
         ctx.set_option(ns.NS_OPT_OVERLAP, int(overlap))
+        if graph:
+            # every option before the warmup (setting one drops the captured
+            # graph): the warmup run captures the graph, the timed region
+            # only replays it
+            ctx.set_option(ns.NS_OPT_GRAPH, 1)
+            ctx.set_option(ns.NS_OPT_GRAPH_STEPS, graph_steps_for(warmup))
+            ctx.set_option(ns.NS_OPT_PROFILE, 0)
         if halos is not None:
             ctx.set_halos(halos)
         ctx.upload(*state)
@@ -235,10 +250,10 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.run(p, warmup)
         ctx.sync()
         tune = ctx.autotune(p, autotune) if autotune else None
-        ctx.set_option(ns.NS_OPT_GRAPH, int(graph))
         ctx.cap_hits(reset=True)
-        ctx.set_option(ns.NS_OPT_PROFILE, 0 if graph else 1)
-        ctx.profile_kernels(ROOFLINE_KERNELS)
+        if not graph:
+            ctx.set_option(ns.NS_OPT_PROFILE, 1)
+            ctx.profile_kernels(ROOFLINE_KERNELS)
         ctx.reset_kernel_stats()
         ctx.sync()
         t0 = time.perf_counter()
@@ -255,10 +270,14 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         hits = ctx.cap_hits()
         K1, W1 = ctx.energy(p)
         lv = ctx.levels()
-        if graph:  # kernel table from a short profiled run after the timed region
+        if graph:
+            # graph nodes carry no per-kernel events: the roofline kernels'
+            # durations come from as many profiled eager steps (up to 20)
+            # right after the timed region
             ctx.set_option(ns.NS_OPT_PROFILE, 1)
+            ctx.profile_kernels(ROOFLINE_KERNELS)
             ctx.reset_kernel_stats()
-            ctx.run(p, 10)
+            ctx.run(p, min(steps, 20))
             ctx.sync()
         stats = ctx.kernel_stats()
         evals = ctx.force_evals()
@@ -307,6 +326,9 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         "E1": K1 + W1,
         "cap_hits": hits,
         "graph": graph,
+        "timing": (f"HIP events over {min(steps, 20)} profiled eager steps right after the graph-replayed "
+                   "timed region (graph nodes carry no per-kernel events)") if graph else
+                  "HIP events over the timed region",
         "autotune": tune,
         "levels": {"hist": np.bincount(lev, minlength=8)[:24].tolist(),
                    "levi_civita": int(((lv & ns.NS_LVL_LC) != 0).sum())},
@@ -473,7 +495,8 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
         traffic, src = traffic_for(run, n, kernel)
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "traffic_source": src, "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2)}
+                "traffic_source": src, "bytes_per_particle": bpp, "avg_us": round(avg_s * 1e6, 2),
+                "timing": run["timing"]}
     if kernel in WORK_ROW:
         ev = run["evals"][WORK_ROW[kernel]] / launches
         flops = FLOPS_PER_FORCE_EVAL * ev
@@ -483,7 +506,7 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
                "traffic": traffic, "traffic_source": src,
                "force_evals_per_launch": ev, "flops_per_force_eval": FLOPS_PER_FORCE_EVAL,
-               "avg_us": round(avg_s * 1e6, 2)}
+               "avg_us": round(avg_s * 1e6, 2), "timing": run["timing"]}
         if kernel == "integrate_late" and run.get("chain"):
             # the launch lasts as long as its longest Levi-Civita chain: time
             # per micro-step of that chain against one wave running the same
@@ -564,8 +587,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="every kernel on one stream (NS_OPT_OVERLAP=0): the same results, for comparison")
-    ap.add_argument("--graph", action="store_true",
-                    help="time the chosen form with captured HIP graphs (NS_OPT_GRAPH)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step's kernels eagerly (by default the timed region replays HIP "
+                         "graphs of captured steps, NS_OPT_GRAPH, captured during the warmup)")
     ap.add_argument("--mlp", type=int, default=0, metavar="N_LOG",
                     help="learned binning: log levels of the plain adaptive integrator on a separate "
                          "N_LOG-particle run of the same recipe (10 steps), train the level predictor on "
@@ -630,10 +654,10 @@ def main() -> int:
     if args.mlp:
         mlp, learned = train_level_predictor(ns, kind, args.mlp, rmax, lowl, args.seed, p, args.budget)
     chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass, halos,
-                     args.graph, args.autotune, mlp, overlap=not args.no_overlap)
+                     not args.no_graph, args.autotune, mlp, overlap=not args.no_overlap)
     if mlp is not None:  # the same form without the network
         without = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass,
-                          halos, args.graph, args.autotune, overlap=not args.no_overlap)
+                          halos, not args.no_graph, args.autotune, overlap=not args.no_overlap)
         learned.update({"value_without": without["value"], "ms_per_step_without": without["ms_per_step"],
                         "gain": chosen["value"] / without["value"],
                         "kernels_without": without["kernels"],
@@ -661,7 +685,7 @@ def main() -> int:
         "config": {"workload": args.workload, "n": n, "dt": p.dt, "form": args.form,
                    "state": args.precision, "lowl_frac": lowl, "lowl_scale": 1e-3, "seed": args.seed,
                    "masses": args.masses, "halos": None if halos is None else int(len(halos)),
-                   "graph": args.graph, "overlap": not args.no_overlap and args.form != "plain"},
+                   "graph": not args.no_graph, "overlap": not args.no_overlap and args.form != "plain"},
         "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": sort_roofline(chosen, n),

@@ -106,11 +106,18 @@ enum {
     NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures steps into a HIP graph
                               once (per parameter set and option state) and
                               replays it: two steps per launch, or, when the
-                              run overlaps (NS_OPT_OVERLAP), one step eagerly
-                              and then 16 overlapped steps per launch (both
-                              streams' work, ending joined); needs
+                              run overlaps (NS_OPT_OVERLAP), one or two steps
+                              eagerly (two when that reaches the buffer
+                              parity of the graph already captured) and then
+                              NS_OPT_GRAPH_STEPS overlapped steps per launch
+                              (both streams' work, ending joined); needs
                               NS_OPT_SORT_ALG 0 and profiling off (else the
                               steps are launched one by one).  Same result. */
+    NS_OPT_GRAPH_STEPS = 14, /* overlapped steps per graph launch: even, 2..64
+                              (default 16).  Each launch ends by joining the
+                              second stream, so fewer steps per launch give
+                              up more overlap at launch boundaries; a run
+                              replays only when it has more steps than this. */
     NS_OPT_PRECISION = 5,  /* 0: float64 state (default); 1: mixed precision --
                               float32 r, v_r, ell in HBM and 32-bit radius
                               keys (4 sort passes), float64 arithmetic in the

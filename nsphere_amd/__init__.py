@@ -22,7 +22,7 @@ import numpy as np
 
 from ._lib import (FORM_BINNED, FORM_BINNED_FAST, FORM_PLAIN, NS_LVL_LC, NS_LVL_MASK, NS_LVL_NOCONV,
                    NS_OPT_BUDGET, NS_OPT_INTEGRATOR, NS_OPT_PRECISION, NS_OPT_PROFILE, NS_OPT_PROFILE_KERNELS,
-                   NS_OPT_SORT_KEYS,
+                   NS_OPT_SORT_KEYS, NS_OPT_GRAPH_STEPS,
                    NS_OPT_GRAPH, NS_OPT_OVERLAP, NS_OPT_SORT_ALG, NS_OPT_SORT_TILE, NS_OPT_TIEBREAK,
                    SORT_ONESWEEP, SORT_REDUCE_SCAN, NsError, StepParams, check, lib)
 
@@ -32,7 +32,7 @@ __all__ = [
     "NS_LVL_MASK", "NS_LVL_NOCONV", "NS_LVL_LC",
     "NS_OPT_TIEBREAK", "NS_OPT_PROFILE", "NS_OPT_INTEGRATOR", "NS_OPT_BUDGET", "NS_OPT_PRECISION",
     "NS_OPT_SORT_TILE", "NS_OPT_SORT_ALG", "NS_OPT_GRAPH", "NS_OPT_OVERLAP", "NS_OPT_PROFILE_KERNELS",
-    "NS_OPT_SORT_KEYS",
+    "NS_OPT_SORT_KEYS", "NS_OPT_GRAPH_STEPS",
     "Mlp", "level_samples", "ApdWriter", "APD_RECORD",
     "read_all_particle_data", "write_ic_file", "read_ic_file", "SORT_REDUCE_SCAN", "SORT_ONESWEEP",
 ]

@@ -19,6 +19,7 @@ NS_OPT_LANE_LEVELS, NS_OPT_DEEP_BLOCKS = 9, 10
 NS_OPT_OVERLAP = 11
 NS_OPT_PROFILE_KERNELS = 12
 NS_OPT_SORT_KEYS = 13
+NS_OPT_GRAPH_STEPS = 14
 SORT_REDUCE_SCAN, SORT_ONESWEEP = 0, 1
 FORM_PLAIN, FORM_BINNED, FORM_BINNED_FAST = 0, 1, 2
 NS_LVL_MASK, NS_LVL_NOCONV, NS_LVL_LC = 0x3F, 0x40, 0x80

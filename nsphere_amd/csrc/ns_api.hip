@@ -81,13 +81,14 @@ struct ns_gpu_ctx {
     int deep_blocks = 1024;  // NS_OPT_DEEP_BLOCKS
     ns::Profiler prof;
     // NS_OPT_GRAPH: ns_gpu_run replays captured steps (two per launch, or
-    // kGraphOverlapSteps overlapped ones)
+    // graph_steps overlapped ones, NS_OPT_GRAPH_STEPS)
     bool use_graph = false;
+    int graph_steps = 16;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     struct GraphKey {
         ns_step_params p;
-        int cur, integrator, f32, nseg, has_mass, overlapped, sort_keys;
+        int cur, integrator, f32, nseg, has_mass, overlapped, sort_keys, graph_steps;
         int64_t budget, cap_N;
         int deep_blocks;
         ns::SortConfig cfg;
@@ -462,6 +463,7 @@ ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p, boo
     k.cur = c->cur;
     k.overlapped = overlapped;
     k.sort_keys = c->sort_keys;
+    k.graph_steps = overlapped ? c->graph_steps : 0;
     k.integrator = c->integrator;
     k.f32 = c->f32;
     k.nseg = c->nseg;
@@ -477,7 +479,7 @@ bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
 {
     return std::memcmp(&a.p, &b.p, sizeof a.p) == 0 && a.cur == b.cur && a.integrator == b.integrator &&
            a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.overlapped == b.overlapped &&
-           a.sort_keys == b.sort_keys &&
+           a.sort_keys == b.sort_keys && a.graph_steps == b.graph_steps &&
            a.budget == b.budget &&
            a.cap_N == b.cap_N && a.deep_blocks == b.deep_blocks &&
            a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
@@ -656,14 +658,9 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev
     return NS_OK;
 }
 
-// Overlapped steps per graph launch: even, so the double buffers return to
-// the captured parity.  Each launch ends by joining the second stream, so
-// the next launch's first sort cannot overlap the late kernel before it; 16
-// steps keep that to one step in 16.
-constexpr int kGraphOverlapSteps = 16;
-
-// Capture kGraphOverlapSteps overlapped steps and the join of the second
-// stream into c->graph_exec (the fork / join events become the graph's
+// Capture c->graph_steps overlapped steps (an even count, so that the double
+// buffers return to the captured parity) and the join of the second stream
+// into c->graph_exec (the fork / join events become the graph's
 // edges between its two branches).  Capture records without running, so the
 // host-side state the steps advance is put back after recording; on success
 // the recorded steps leave it where it started (even parity flips, keys and
@@ -676,7 +673,7 @@ int capture_overlapped(ns_gpu_ctx *c, const ns_step_params *p, const ns_gpu_ctx:
     const uint32_t *last_perm = c->last_perm;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = NS_OK;
-    for (int k = 0; k < kGraphOverlapSteps && rc == NS_OK; k++)
+    for (int k = 0; k < c->graph_steps && rc == NS_OK; k++)
         rc = overlapped_step(c, p, c->ev_cap);
     const hipError_t ej = hipStreamWaitEvent(c->stream, c->ev_cap[2], 0);
     hipGraph_t gr = nullptr;
@@ -706,9 +703,10 @@ int capture_overlapped(ns_gpu_ctx *c, const ns_step_params *p, const ns_gpu_ctx:
 
 // nsteps >= 2 overlapped steps; the context's stream then waits for the
 // second stream and the last step's late ranks get their true keys.  With
-// graphs: the first step eagerly, a join, then launches of
-// kGraphOverlapSteps captured steps each (which end joined), then the rest
-// eagerly.
+// graphs: the first step eagerly (the first two, when the captured graph
+// expects the other buffer parity and the run is long enough), a join, then
+// launches of c->graph_steps captured steps each (which end joined), then
+// the rest eagerly.
 int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool graph)
 {
     if (int rc = ensure_overlap(c))
@@ -716,9 +714,18 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
     // the slow list of the last step, from its level records (whatever form
     // ran it): the first step defers the chains that step found slow
     HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->seg_off, c->nseg, c->n, c->stream));
-    const int64_t launches = graph ? (nsteps - 1) / kGraphOverlapSteps : 0;
+    const int64_t G = c->graph_steps;
     int rc = overlapped_step(c, p, c->ev);
     int64_t done = 1;
+    if (rc == NS_OK && graph && c->graph_exec && nsteps - 2 >= G) {
+        ns_gpu_ctx::GraphKey other = graph_key(c, p, true);
+        other.cur ^= 1;
+        if (!same_key(graph_key(c, p, true), c->gkey) && same_key(other, c->gkey)) {
+            rc = overlapped_step(c, p, c->ev);
+            done++;
+        }
+    }
+    const int64_t launches = graph ? (nsteps - done) / G : 0;
     bool joined = false;
     if (rc == NS_OK && launches > 0) {
         // the graph's first plan must follow this step's late kernel, an
@@ -733,7 +740,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
         for (int64_t k = 0; k < launches && rc == NS_OK; k++) {
             if ((e = hipGraphLaunch(c->graph_exec, c->stream)) != hipSuccess)
                 rc = set_error(NS_ERR_HIP, "ns_gpu_run: graph launch: %s", hipGetErrorString(e));
-            done += kGraphOverlapSteps;
+            done += G;
         }
     }
     for (; done < nsteps && rc == NS_OK; done++) {
@@ -930,6 +937,11 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         if (value != 0 && value != 1)
             return set_error(NS_ERR_ARG, "NS_OPT_OVERLAP takes 0 or 1");
         c->overlap = (int)value;
+        return NS_OK;
+    case NS_OPT_GRAPH_STEPS:
+        if (value < 2 || value > 64 || (value & 1))
+            return set_error(NS_ERR_ARG, "NS_OPT_GRAPH_STEPS takes an even count in 2..64");
+        c->graph_steps = (int)value;
         return NS_OK;
     case NS_OPT_SORT_KEYS:
         if (value != 0 && value != 1)
@@ -1501,13 +1513,23 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
         return NS_OK;
     }
     // One step eagerly (radius keys from the integrate epilogue, segment
-    // table built), then pairs of steps from a captured graph: two steps
+    // table built), or two when the graph already captured expects the other
+    // buffer parity, then pairs of steps from a captured graph: two steps
     // return the double buffers to the captured parity.  Capture records
     // the launches without running them.
     if (int rc = ns_gpu_step(c, p))
         return rc;
     nsteps--;
     DeviceGuard g(c->device);
+    if (c->graph_exec && nsteps >= 3) {
+        ns_gpu_ctx::GraphKey other = graph_key(c, p, false);
+        other.cur ^= 1;
+        if (!same_key(graph_key(c, p, false), c->gkey) && same_key(other, c->gkey)) {
+            if (int rc = ns_gpu_step(c, p))
+                return rc;
+            nsteps--;
+        }
+    }
     const ns_gpu_ctx::GraphKey key = graph_key(c, p, false);
     if (!c->graph_exec || !same_key(key, c->gkey)) {
         release_graph(c);
