@@ -80,10 +80,15 @@ def bytes_per_particle(state_bytes: int) -> dict:
 # itself (a counter run is a separate rocprofv3 invocation), and a kernel whose code changed since
 # its profile has no entry.
 TRAFFIC_PROFILES = {
-    ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast.md",
-                          {"sort_scatter": 171.2e6, "sort_upsweep": 50.05e6, "integrate_bulk": 499.3e6}),
+    # sort_scatter: mean of the 4 passes' read + write
+    ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f32_overlap.md",
+                          {"sort_scatter": 171.4e6, "sort_upsweep": 50.04e6, "integrate_bulk": 496.6e6,
+                           "integrate_lc": 7.51e6, "integrate_deep": 0.41e6, "integrate_late": 0.20e6}),
+    # sort_scatter: mean of the 3 passes' read + write
     ("f64", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md",
-                          {"sort_upsweep": 50.05e6, "integrate_deep": 0.41e6}),
+                          {"sort_scatter": 180.1e6, "sort_upsweep": 50.05e6, "sort_fixup": 243.5e6,
+                           "integrate_bulk": 785.2e6, "integrate_lc": 10.01e6, "integrate_deep": 0.45e6,
+                           "integrate_late": 0.25e6}),
 }
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
