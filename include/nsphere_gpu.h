@@ -133,9 +133,10 @@ enum {
                               order and can run 2^28 subdivisions each       */
     NS_OPT_OVERLAP = 11,   /* binned forms, ns_gpu_run of two or more steps:
                               1 (default): particles whose previous step was
-                              slow (flagged NS_LVL_NOCONV, at level 6 or
-                              deeper, or among the 64 innermost ranks of
-                              their halo; up to 1024 per step) are
+                              slow (flagged NS_LVL_NOCONV, at leapfrog level
+                              4 or Levi-Civita level 2 or deeper -- past the
+                              lane kernels -- or among the 64 innermost ranks
+                              of their halo; up to 1024 per step) are
                               integrated on a second stream while the main
                               stream integrates the rest and sorts the next
                               step's radii; a merge then puts them at their

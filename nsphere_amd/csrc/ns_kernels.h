@@ -85,19 +85,23 @@ constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 constexpr int kSlowCap = 1024;   // particles deferred per step
 constexpr int kSlowList = 4096;  // class-2 candidates kept per step
 constexpr int kCoreList = 1024;  // class-1 candidates kept per step
-// Which particles are likely to need a long chain in the next step.  Class 2:
-// the chain hit the cap or reached level 6 (2^7 or more sequential
-// micro-steps) -- a particle plunging towards the centre climbs through the
-// deep leapfrog levels before it turns into a capped Levi-Civita chain.
-// Class 1: otherwise, the particle sits among the innermost kCoreRanks ranks
-// of its halo, where the plunges end.  On the Plummer benchmark the two
-// catch every chain of level 12 or more, or capped, of 200 steps; the
+// Which particles are likely to need a wave of their own in the next step.
+// Class 2: the chain hit the cap, or it went past the lane kernels -- a
+// leapfrog level of kSlowLevel or more (past the bulk kernel's levels 0-3)
+// or a Levi-Civita tau step at level kSlowLevelLc or more (past the lane
+// kernel's levels 0-1) -- so that the previous step's wave particles leave
+// the context's stream for the late kernel; a particle plunging towards the
+// centre climbs through these levels before it turns into a capped
+// Levi-Civita chain.  Class 1: otherwise, the particle sits among the
+// innermost kCoreRanks ranks of its halo, where the plunges end.  The
 // deferred particles are taken class 2 first, costliest first.
-constexpr int kSlowLevel = 6;
+constexpr int kSlowLevel = 4;
+constexpr int kSlowLevelLc = 2;
 constexpr int64_t kCoreRanks = 64;
 __host__ __device__ inline int slow_class(int rec, int64_t local_rank)
 {
-    if ((rec & NS_LVL_NOCONV) || (rec & NS_LVL_MASK) >= kSlowLevel)
+    const int level = rec & NS_LVL_MASK;
+    if ((rec & NS_LVL_NOCONV) || level >= ((rec & NS_LVL_LC) ? kSlowLevelLc : kSlowLevel))
         return 2;
     return local_rank < kCoreRanks ? 1 : 0;
 }

@@ -1,6 +1,7 @@
 """Overlapped runs (NS_OPT_OVERLAP=1): the particles whose previous step was
-slow (not converged, or level 6 or deeper; else among a halo's 64 innermost
-ranks) are integrated on a second stream while the rest of the step runs and
+slow (not converged, a leapfrog level of 4 or deeper or a Levi-Civita tau
+step at level 2 or deeper; else among a halo's 64 innermost ranks) are
+integrated on a second stream while the rest of the step runs and
 the next step's radii are sorted, then merged back at their ranks.  The merge must reproduce the
 stable sort of the true keys, so every result is bit-identical to the run
 with every kernel on one stream -- including exact radius ties, more deferred
