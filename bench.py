@@ -212,9 +212,10 @@ def cpu_p1(ns, seed: int, n: int = 100_000, steps: int = 1000, wall_limit_s: flo
 
 
 def graph_steps_for(warmup: int) -> int:
-    """Overlapped steps per graph launch: the most (up to 16, even) that the
-    warmup run replays once after its first eager step, so that the capture
-    happens there and not in the timed region."""
+    """Overlapped steps per graph launch of a run that is not captured whole
+    (odd, or past 128 steps): the most (up to 16, even) that the warmup run
+    replays once after its first eager step, so that the capture happens
+    there and not in the timed region."""
     return max(2, min(16, (warmup - 1) // 2 * 2))
 
 
@@ -256,7 +257,9 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.sync()
         tune = ctx.autotune(p, autotune) if autotune else None
         ctx.cap_hits(reset=True)
-        if not graph:
+        if graph:  # an even run is one graph: capture it before the timed region
+            ctx.graph_prepare(p, min(chunk, steps))
+        else:
             ctx.set_option(ns.NS_OPT_PROFILE, 1)
             ctx.profile_kernels(ROOFLINE_KERNELS)
         ctx.reset_kernel_stats()

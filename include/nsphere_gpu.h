@@ -105,14 +105,18 @@ enum {
                                 1024).  Same result.                          */
     NS_OPT_GRAPH = 8,      /* 1: ns_gpu_run captures steps into a HIP graph
                               once (per parameter set and option state) and
-                              replays it: two steps per launch, or, when the
-                              run overlaps (NS_OPT_OVERLAP), one or two steps
-                              eagerly (two when that reaches the buffer
-                              parity of the graph already captured) and then
-                              NS_OPT_GRAPH_STEPS overlapped steps per launch
-                              (both streams' work, ending joined); needs
-                              NS_OPT_SORT_ALG 0 and profiling off (else the
-                              steps are launched one by one).  Same result. */
+                              replays it: two steps per launch; when the run
+                              overlaps (NS_OPT_OVERLAP), the whole run in one
+                              launch for an even step count up to 128
+                              (ns_gpu_graph_prepare captures it ahead), else
+                              one or two steps eagerly (two when that reaches
+                              the buffer parity of the graph already
+                              captured) and then NS_OPT_GRAPH_STEPS
+                              overlapped steps per launch; both streams'
+                              work is captured and each launch ends joined.
+                              Needs NS_OPT_SORT_ALG 0 and profiling off (else
+                              the steps are launched one by one).  Same
+                              result.                                        */
     NS_OPT_GRAPH_STEPS = 14, /* overlapped steps per graph launch: even, 2..64
                               (default 16).  Each launch ends by joining the
                               second stream, so fewer steps per launch give
@@ -226,6 +230,13 @@ int ns_gpu_step(ns_gpu_ctx *ctx, const ns_step_params *p);
  * NS_OPT_OVERLAP the context's stream waits for the second stream before the
  * call returns, so the context's stream orders all later work). */
 int ns_gpu_run(ns_gpu_ctx *ctx, const ns_step_params *p, int64_t nsteps);
+/* With NS_OPT_GRAPH, an overlapped ns_gpu_run of an even number of steps up
+ * to 128 replays one graph of the whole run (one join, at its end).  This
+ * captures that graph for a run of nsteps steps from the current state,
+ * without running anything, so that the run itself only replays it; for any
+ * other run it does nothing.  Not in the reference (a scheduling hint for
+ * callers that time their runs). */
+int ns_gpu_graph_prepare(ns_gpu_ctx *ctx, const ns_step_params *p, int64_t nsteps);
 int ns_gpu_sync(ns_gpu_ctx *ctx);
 /* Particle-steps stopped by the work cap (NS_OPT_BUDGET) since the context
  * was created or last reset; synchronises the context's stream. */

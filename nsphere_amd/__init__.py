@@ -189,6 +189,10 @@ class GpuContext:
     def run(self, params: StepParams, nsteps: int) -> None:
         check(lib.ns_gpu_run(self._h, ctypes.byref(params), int(nsteps)))
 
+    def graph_prepare(self, params: StepParams, nsteps: int) -> None:
+        """Capture ahead the graph that run(params, nsteps) replays (NS_OPT_GRAPH)."""
+        check(lib.ns_gpu_graph_prepare(self._h, ctypes.byref(params), int(nsteps)))
+
     def sync(self) -> None:
         check(lib.ns_gpu_sync(self._h))
 

@@ -116,6 +116,7 @@ def _load() -> ctypes.CDLL:
         "ns_gpu_integrate": (i32, [vp, c.POINTER(StepParams)]),
         "ns_gpu_step": (i32, [vp, c.POINTER(StepParams)]),
         "ns_gpu_run": (i32, [vp, c.POINTER(StepParams), i64]),
+        "ns_gpu_graph_prepare": (i32, [vp, c.POINTER(StepParams), i64]),
         "ns_gpu_sync": (i32, [vp]),
         "ns_gpu_cap_hits": (i32, [vp, c.POINTER(c.c_uint64), i32]),
         "ns_gpu_levels": (i32, [vp, vp]),

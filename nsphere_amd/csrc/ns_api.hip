@@ -88,11 +88,18 @@ struct ns_gpu_ctx {
     hipGraphExec_t graph_exec = nullptr;
     struct GraphKey {
         ns_step_params p;
-        int cur, integrator, f32, nseg, has_mass, overlapped, sort_keys, graph_steps;
+        int cur, integrator, f32, nseg, has_mass, overlapped, sort_keys, graph_steps, keys_state;
+        int64_t run_steps;  // a whole overlapped run in one graph (else 0)
         int64_t budget, cap_N;
         int deep_blocks;
         ns::SortConfig cfg;
     } gkey = {};
+    // the host-side state a replay of the whole-run graph leaves
+    struct GraphPost {
+        int cur;
+        bool keys_valid, keys_q24, levels_valid;
+        const uint32_t *last_perm;
+    } gpost = {};
 };
 
 namespace ns {
@@ -456,9 +463,11 @@ void release_graph(ns_gpu_ctx *c)
     c->graph = nullptr;
 }
 
-ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p, bool overlapped)
+ns_gpu_ctx::GraphKey graph_key(const ns_gpu_ctx *c, const ns_step_params *p, bool overlapped, int64_t run_steps = 0)
 {
     ns_gpu_ctx::GraphKey k = {};
+    k.run_steps = run_steps;
+    k.keys_state = (c->keys_valid ? 2 : 0) | (c->keys_q24 ? 1 : 0);
     k.p = *p;
     k.cur = c->cur;
     k.overlapped = overlapped;
@@ -479,7 +488,8 @@ bool same_key(const ns_gpu_ctx::GraphKey &a, const ns_gpu_ctx::GraphKey &b)
 {
     return std::memcmp(&a.p, &b.p, sizeof a.p) == 0 && a.cur == b.cur && a.integrator == b.integrator &&
            a.f32 == b.f32 && a.nseg == b.nseg && a.has_mass == b.has_mass && a.overlapped == b.overlapped &&
-           a.sort_keys == b.sort_keys && a.graph_steps == b.graph_steps &&
+           a.sort_keys == b.sort_keys && a.graph_steps == b.graph_steps && a.run_steps == b.run_steps &&
+           a.keys_state == b.keys_state &&
            a.budget == b.budget &&
            a.cap_N == b.cap_N && a.deep_blocks == b.deep_blocks &&
            a.cfg.alg == b.cfg.alg && a.cfg.threads == b.cfg.threads && a.cfg.items == b.cfg.items;
@@ -526,6 +536,17 @@ bool quantile_applies(const ns_gpu_ctx *c)
     return c->sort_keys && !c->f32 && c->nseg == 0 && c->n >= ns::kQMinN;
 }
 
+int ensure_qtab(ns_gpu_ctx *c)
+{
+    if (c->qtab)
+        return NS_OK;
+    const hipError_t e = hipMalloc(&c->qtab, (size_t)ns::kQBuckets * sizeof(ns::QEntry));
+    if (e != hipSuccess)
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "quantile table: %s",
+                         hipGetErrorString(e));
+    return NS_OK;
+}
+
 // The quantile table of the step's start radii (buffer cur, in the order of
 // perm) for the keys the integrate writes, when they are quantile keys.
 int quantile_table(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm)
@@ -534,12 +555,8 @@ int quantile_table(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm)
     a.q_stride = 0;
     if (!quantile_applies(c) || !a.keys)
         return NS_OK;
-    if (!c->qtab) {
-        const hipError_t e = hipMalloc(&c->qtab, (size_t)ns::kQBuckets * sizeof(ns::QEntry));
-        if (e != hipSuccess)
-            return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "quantile table: %s",
-                             hipGetErrorString(e));
-    }
+    if (int rc = ensure_qtab(c))
+        return rc;
     HIP_TRY(ns::launch_quantile_table(c->r[c->cur], perm, c->n, c->qtab, c->stream));
     a.qtab = c->qtab;
     a.q_stride = (uint32_t)(c->n / ns::kQBuckets);
@@ -658,26 +675,38 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev
     return NS_OK;
 }
 
-// Capture c->graph_steps overlapped steps (an even count, so that the double
+// Overlapped runs of an even number of steps up to this many are captured
+// whole: one launch, one join at the end, no step eagerly.
+constexpr int64_t kGraphRunMax = 128;
+
+bool whole_run_graph(int64_t nsteps) { return nsteps >= 2 && nsteps <= kGraphRunMax && (nsteps & 1) == 0; }
+
+// Capture nsteps overlapped steps (an even count, so that the double
 // buffers return to the captured parity) and the join of the second stream
 // into c->graph_exec (the fork / join events become the graph's
 // edges between its two branches).  Capture records without running, so the
 // host-side state the steps advance is put back after recording; on success
 // the recorded steps leave it where it started (even parity flips, keys and
 // levels valid).
-int capture_overlapped(ns_gpu_ctx *c, const ns_step_params *p, const ns_gpu_ctx::GraphKey &key)
+int capture_overlapped(ns_gpu_ctx *c, const ns_step_params *p, const ns_gpu_ctx::GraphKey &key, int64_t nsteps)
 {
     release_graph(c);
+    // what a step sets up on its first call, which a capture must not meet
+    HIP_TRY(ns::sort_prepare(c->sw, c->n));
+    if (quantile_applies(c))
+        if (int rc = ensure_qtab(c))
+            return rc;
     const int cur = c->cur;
     const bool keys_valid = c->keys_valid, keys_q24 = c->keys_q24, levels_valid = c->levels_valid;
     const uint32_t *last_perm = c->last_perm;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = NS_OK;
-    for (int k = 0; k < c->graph_steps && rc == NS_OK; k++)
+    for (int64_t k = 0; k < nsteps && rc == NS_OK; k++)
         rc = overlapped_step(c, p, c->ev_cap);
     const hipError_t ej = hipStreamWaitEvent(c->stream, c->ev_cap[2], 0);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    c->gpost = {c->cur, c->keys_valid, c->keys_q24, c->levels_valid, c->last_perm};
     c->cur = cur;
     c->keys_valid = keys_valid;
     c->keys_q24 = keys_q24;
@@ -714,6 +743,22 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
     // the slow list of the last step, from its level records (whatever form
     // ran it): the first step defers the chains that step found slow
     HIP_TRY(ns::overlap_reset(c->ov, c->levels_valid ? c->lvl : nullptr, c->seg_off, c->nseg, c->n, c->stream));
+    if (graph && whole_run_graph(nsteps)) {  // the whole run, one launch
+        const ns_gpu_ctx::GraphKey key = graph_key(c, p, true, nsteps);
+        if (!c->graph_exec || !same_key(key, c->gkey))
+            if (int rc = capture_overlapped(c, p, key, nsteps))
+                return rc;
+        HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+        c->cur = c->gpost.cur;
+        c->keys_valid = c->gpost.keys_valid;
+        c->keys_q24 = c->gpost.keys_q24;
+        c->levels_valid = c->gpost.levels_valid;
+        c->last_perm = c->gpost.last_perm;
+        const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
+        HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->keys_q24 ? c->qtab : nullptr,
+                                     (uint32_t)(c->n / ns::kQBuckets), c->stream));
+        return NS_OK;
+    }
     const int64_t G = c->graph_steps;
     int rc = overlapped_step(c, p, c->ev);
     int64_t done = 1;
@@ -736,7 +781,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
             rc = set_error(NS_ERR_HIP, "ns_gpu_run: join of the second stream: %s", hipGetErrorString(e));
         const ns_gpu_ctx::GraphKey key = graph_key(c, p, true);
         if (rc == NS_OK && (!c->graph_exec || !same_key(key, c->gkey)))
-            rc = capture_overlapped(c, p, key);
+            rc = capture_overlapped(c, p, key, G);
         for (int64_t k = 0; k < launches && rc == NS_OK; k++) {
             if ((e = hipGraphLaunch(c->graph_exec, c->stream)) != hipSuccess)
                 rc = set_error(NS_ERR_HIP, "ns_gpu_run: graph launch: %s", hipGetErrorString(e));
@@ -1563,6 +1608,26 @@ int ns_gpu_run(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
     if (nsteps & 1)
         return ns_gpu_step(c, p);
     return NS_OK;
+}
+
+int ns_gpu_graph_prepare(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps)
+{
+    if (int rc = check_ctx(c, "ns_gpu_graph_prepare"))
+        return rc;
+    if (int rc = check_params(p, "ns_gpu_graph_prepare"))
+        return rc;
+    if (!c->uploaded)
+        return set_error(NS_ERR_STATE, "ns_gpu_graph_prepare: no state uploaded");
+    const bool graph = c->use_graph && !c->prof.on && c->sw.config.alg == ns::kSortReduceScan;
+    if (!graph || !overlap_applies(c, nsteps) || !whole_run_graph(nsteps))
+        return NS_OK;  // that run captures nothing ahead
+    DeviceGuard g(c->device);
+    if (int rc = ensure_overlap(c))
+        return rc;
+    const ns_gpu_ctx::GraphKey key = graph_key(c, p, true, nsteps);
+    if (c->graph_exec && same_key(key, c->gkey))
+        return NS_OK;
+    return capture_overlapped(c, p, key, nsteps);
 }
 
 int ns_gpu_sync(ns_gpu_ctx *c)

@@ -306,6 +306,10 @@ struct SortWork {
 };
 // Frees the device part of a segment table.
 void seg_table_release(SegTable &t);
+// The host-side set-up that the first segmented sort of n keys would do (the
+// segment table for the current tile), done ahead: a graph capture must not
+// meet its synchronous copies.
+hipError_t sort_prepare(SortWork &w, int64_t n);
 
 // Launchers (ns_sort.hip).  All asynchronous on `stream`.
 hipError_t launch_radius_keys(const double *r, uint64_t *keys, int64_t n, hipStream_t stream);

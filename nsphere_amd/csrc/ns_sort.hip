@@ -1149,6 +1149,14 @@ hipError_t sort_split64(SortWork &w, int64_t n, uint32_t *perm_out, hipStream_t 
 
 } // namespace
 
+hipError_t sort_prepare(SortWork &w, int64_t n)
+{
+    if (w.seg.sizes.empty() || w.config.alg != kSortReduceScan)
+        return hipSuccess;
+    const int tile = w.config.threads * w.config.items;
+    return w.seg.tile == tile ? hipSuccess : seg_table_build(w.seg, tile, n);
+}
+
 void seg_table_release(SegTable &t)
 {
     void *bufs[] = {t.tile_start, t.tile_seg, t.seg_first, t.seg_start, t.base, t.counts};

@@ -93,6 +93,38 @@ def test_graph_overlapped_equal_eager(ns, precision, form):
     run_both(ns, state, setup, [(p, 33), (p, 20), (p2, 17), (p, 40), (p, 16), (p, 2)])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_whole_run_prepared(ns, precision):
+    """An even overlapped run is replayed as one graph, captured ahead by
+    ns_gpu_graph_prepare (which runs nothing) or by the run itself; the state
+    equals eager runs after every run, including runs whose keys were
+    invalidated by an energy evaluation in between (a separate capture)."""
+    state = sorted_ic(ns, 0, 200_000, 34)
+    p = ns.unit_params()
+    out = []
+    for graph in (0, 1):
+        with ns.GpuContext(state[0].size) as ctx:
+            ctx.set_option(ns.NS_OPT_PRECISION, precision)
+            ctx.set_option(ns.NS_OPT_INTEGRATOR, 2)
+            ctx.set_option(ns.NS_OPT_GRAPH, graph)
+            ctx.upload(*state)
+            ctx.run(p, 5)
+            ctx.graph_prepare(p, 20)
+            before = ctx.download()  # the prepare ran nothing
+            trace = [before]
+            for k in (20, 20, 6, 20):
+                ctx.run(p, k)
+                trace.append(ctx.download())
+            trace.append(ctx.energy(p))
+            ctx.run(p, 20)
+            trace.append(ctx.download())
+            out.append(trace)
+    for i, (a, b) in enumerate(zip(out[0][:-2], out[1][:-2])):
+        assert_state(a, b, f"after run {i}")
+    assert out[0][-2] == out[1][-2]
+    assert_state(out[0][-1], out[1][-1], "after the energy and another run")
+
+
 def test_graph_overlapped_ensemble(ns):
     sizes = [30_000, 5_000, 60_000]
     parts = [sorted_ic(ns, h % 2, s, 50 + h) for h, s in enumerate(sizes)]
