@@ -34,11 +34,12 @@ FLOPS_PER_FORCE_EVAL = 16     # FP64 flops per force evaluation + kick/drift, a 
 # HBM bytes per particle-step of the whole step.  Canonical (8-bit digits
 # over keys as wide as the state, so 8 passes of 64-bit keys for float64):
 # float64 304, float32 124.  This build's algorithm: between steps the
-# float64 sort runs 3 passes over 24-bit quantile keys and one fix-up pass
-# over runs of equal keys (72 B instead of 200 B), and the integrate writes a
-# 4-byte key instead of 8, so a float64 step moves 172 B.
+# float64 sort buckets 24-bit quantile keys (one scatter of key and position
+# into per-bucket regions, 12 B, then one pass ordering each bucket, 16 B:
+# 28 B instead of 200 B), and the integrate writes a 4-byte key instead of 8,
+# so a float64 step moves 128 B.
 CANONICAL_STEP_BYTES = {"f64": 304, "f32": 124}
-STEP_BYTES = {"f64": 172, "f32": 124}
+STEP_BYTES = {"f64": 128, "f32": 124}
 
 
 def bytes_per_particle(state_bytes: int) -> dict:
@@ -71,6 +72,11 @@ def bytes_per_particle(state_bytes: int) -> dict:
     }
     if wide:
         out["sort_fixup"] = 16
+        # bucket sort of the quantile keys: read the key, write (key, position);
+        # then per bucket read (key, position), the radius of the half of the
+        # positions that share their sub-bin, write the permutation
+        out["qsort_scatter"] = 4 + 8
+        out["qsort_bucket"] = 8 + 4 + 4
     return out
 
 
@@ -96,7 +102,7 @@ WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "in
 # Kernel families timed with HIP events inside the timed region (two events
 # per launch): the integrate kernels and the sort's scatter, which the
 # rooflines use.  The other families are timed over a short run after it.
-SORT_KERNELS = ["sort_upsweep", "sort_scan", "sort_scatter", "sort_fixup"]
+SORT_KERNELS = ["sort_upsweep", "sort_scan", "sort_scatter", "sort_fixup", "qsort_scatter", "qsort_bucket"]
 ROOFLINE_KERNELS = ["integrate", "integrate_bulk", "integrate_lc", "integrate_deep", "integrate_late", *SORT_KERNELS]
 AFTER_STEPS = 20
 # The late kernel's chains run at best at the rate of one wave alone on an idle
@@ -549,9 +555,9 @@ def sort_roofline(run: dict, n: int) -> dict | None:
     the algorithmic bytes of all its launches over their summed durations
     (the scan moves a few MB of tile counts; counted as zero)."""
     stats = run["stats"]
-    if "sort_scatter" not in stats:
+    if "sort_scatter" not in stats and "qsort_scatter" not in stats:
         return None
-    steps = run["stats"].get("sort_fixup", (None,))[0] or stats["sort_scatter"][0] / 4
+    steps = (stats.get("qsort_scatter", (0,))[0] + stats.get("sort_fixup", (0,))[0]) or stats["sort_scatter"][0] / 4
     total_ms, total_bytes, parts = 0.0, 0.0, {}
     for k in SORT_KERNELS:
         if k not in stats:
@@ -698,7 +704,8 @@ def main() -> int:
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": sort_roofline(chosen, n),
         "roofline_late": roofline_for(chosen, n, "integrate_late"),
-        "roofline_scatter": roofline_for(chosen, n, "sort_scatter"),
+        "roofline_scatter": roofline_for(chosen, n, "qsort_scatter") or roofline_for(chosen, n, "sort_scatter"),
+        "roofline_bucket": roofline_for(chosen, n, "qsort_bucket"),
         "roofline_step": {"bound": "hbm", "bytes_per_particle_step": STEP_BYTES[args.precision],
                           "canonical_bytes_per_particle_step": CANONICAL_STEP_BYTES[args.precision],
                           "achieved": round(chosen["value"] * STEP_BYTES[args.precision] / 1e9, 1),

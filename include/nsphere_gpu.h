@@ -153,11 +153,15 @@ enum {
                               24-bit quantile keys that the integrate wrote
                               (the radius's bucket in 4096 quantiles of the
                               step's start radii, then 12 bits of position in
-                              the bucket): three 8-bit passes, then each run
-                              of equal keys ordered by radius; 0: the 64-bit
-                              radius keys (four passes on their high halves,
-                              then each run ordered by the low halves).  Same
-                              permutation.                                   */
+                              the bucket) by bucket -- one scatter into
+                              per-bucket regions, then one workgroup per
+                              bucket orders it by (key, radius, position) --
+                              when a bucket fits a workgroup (npts up to about
+                              5e7), else as 2; 2: the same keys by three 8-bit
+                              passes, then each run of equal keys ordered by
+                              radius; 0: the 64-bit radius keys (four passes on
+                              their high halves, then each run ordered by the
+                              low halves).  Same permutation.                */
 };
 
 typedef struct ns_gpu_ctx ns_gpu_ctx;

@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "libnsphere_gpu.so")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["ns_api.hip", "ns_sort.hip", "ns_integrate.hip", "ns_scan.hip", "ns_mlp.hip", "ns_io.hip",
-               "ns_density.hip", "ns_rank.hip", "ns_overlap.hip", "ns_noalloc.hip"]
+               "ns_density.hip", "ns_rank.hip", "ns_overlap.hip", "ns_noalloc.hip", "ns_qsort.hip"]
 HOST_SOURCES = ["ns_ic.cpp", "ns_dropin.cpp"]
 DRIVER_SRC = os.path.join(ROOT, "tests", "c", "dropin_driver.c")
 DRIVER_BIN = os.path.join(ROOT, "tests", "c", "dropin_driver")

@@ -127,7 +127,8 @@ const char *kernel_name(int id)
                                          "integrate_deep", "mass_scan", "mlp_binning",
                                          "snapshot_records", "potential", "density",
                                          "rank_files", "integrate_late", "overlap_plan",
-                                         "overlap_place", "sort_fixup", "integrate_lc"};
+                                         "overlap_place", "sort_fixup", "integrate_lc",
+                                         "qsort_scatter", "qsort_bucket"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -272,6 +273,7 @@ hipError_t dalloc(T **p, int64_t count)
 void free_all(ns_gpu_ctx *c)
 {
     ns::seg_table_release(c->sw.seg);
+    ns::qsort_release(c->sw.qs);
     void *bufs[] = {c->r[0], c->r[1], c->v[0], c->v[1], c->l[0], c->l[1], c->id[0], c->id[1],
                     c->sw.keys[0], c->sw.keys[1], c->sw.vals[0], c->sw.vals[1], c->lvl, c->perm,
                     c->inv, c->sw.hist, c->sw.lookback, c->sw.counters, c->energy_partial,
@@ -512,7 +514,7 @@ int sort_perm(ns_gpu_ctx *c, uint32_t *out = nullptr)
         return NS_OK;
     }
     if (c->keys_valid && c->keys_q24) {
-        HIP_TRY(ns::radix_sort_perm_q24(c->sw, c->n, c->r[c->cur], out, c->stream, &c->prof));
+        HIP_TRY(ns::radix_sort_perm_q24(c->sw, c->n, c->r[c->cur], out, c->stream, &c->prof, c->sort_keys == 1));
         c->keys_valid = false;  // the passes overwrite keys[0]
         return NS_OK;
     }
@@ -536,13 +538,18 @@ bool quantile_applies(const ns_gpu_ctx *c)
     return c->sort_keys && !c->f32 && c->nseg == 0 && c->n >= ns::kQMinN;
 }
 
+// The quantile table, and the bucket sort's workspace for the sort that
+// follows (allocated here, outside any capture, since they are set up
+// synchronously).
 int ensure_qtab(ns_gpu_ctx *c)
 {
-    if (c->qtab)
-        return NS_OK;
-    const hipError_t e = hipMalloc(&c->qtab, (size_t)ns::kQBuckets * sizeof(ns::QEntry));
+    hipError_t e = hipSuccess;
+    if (!c->qtab)
+        e = hipMalloc(&c->qtab, (size_t)ns::kQBuckets * sizeof(ns::QEntry));
+    if (e == hipSuccess && c->sort_keys == 1)
+        e = ns::qsort_prepare(c->sw.qs, c->n);
     if (e != hipSuccess)
-        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "quantile table: %s",
+        return set_error(e == hipErrorOutOfMemory ? NS_ERR_NOMEM : NS_ERR_HIP, "quantile keys: %s",
                          hipGetErrorString(e));
     return NS_OK;
 }
@@ -559,7 +566,7 @@ int quantile_table(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm)
         return rc;
     HIP_TRY(ns::launch_quantile_table(c->r[c->cur], perm, c->n, c->qtab, c->stream));
     a.qtab = c->qtab;
-    a.q_stride = (uint32_t)(c->n / ns::kQBuckets);
+    a.q_stride = ns::quantile_stride(c->n);
     return NS_OK;
 }
 
@@ -756,7 +763,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
         c->last_perm = c->gpost.last_perm;
         const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
         HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->keys_q24 ? c->qtab : nullptr,
-                                     (uint32_t)(c->n / ns::kQBuckets), c->stream));
+                                     ns::quantile_stride(c->n), c->stream));
         return NS_OK;
     }
     const int64_t G = c->graph_steps;
@@ -805,7 +812,7 @@ int run_overlapped(ns_gpu_ctx *c, const ns_step_params *p, int64_t nsteps, bool 
     }
     const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
     HIP_TRY(ns::overlap_fix_keys(c->ov, r, c->f32, c->sw.keys[0], c->n, c->keys_q24 ? c->qtab : nullptr,
-                                 (uint32_t)(c->n / ns::kQBuckets), c->stream));
+                                 ns::quantile_stride(c->n), c->stream));
     return NS_OK;
 }
 
@@ -989,8 +996,8 @@ int ns_gpu_set_option(ns_gpu_ctx *c, int option, int64_t value)
         c->graph_steps = (int)value;
         return NS_OK;
     case NS_OPT_SORT_KEYS:
-        if (value != 0 && value != 1)
-            return set_error(NS_ERR_ARG, "NS_OPT_SORT_KEYS takes 0 or 1");
+        if (value < 0 || value > 2)
+            return set_error(NS_ERR_ARG, "NS_OPT_SORT_KEYS takes 0, 1 or 2");
         c->sort_keys = (int)value;
         return NS_OK;
     case NS_OPT_INTEGRATOR:

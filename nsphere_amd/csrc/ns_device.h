@@ -90,7 +90,7 @@ __device__ inline void store_key(void *keys, int64_t, int64_t i, uint32_t k)
 // Between steps the float64 radii are sorted on a 24-bit key instead of
 // their 64 bits: the bucket of the radius in a table of kQBuckets quantiles
 // of the step's start radii (the radius keys of ranks 0, s, 2s, ... for the
-// stride s = n / kQBuckets), then kQSubBits of linear position inside the
+// stride s = quantile_stride(n)), then kQSubBits of linear position inside the
 // bucket.  The key is a non-decreasing function of the radius key, so a
 // stable sort on it followed by a stable sort of each run of equal keys on
 // the full radius key is the stable sort on the radius.  Quantile buckets
@@ -102,6 +102,15 @@ constexpr int kQSubBits = 12;
 constexpr uint32_t kQKeyMax = 0xFFFFFEu;  // keys stay below the deferral sentinel's low 24 bits
 constexpr int64_t kQMinN = 65536;         // smaller sorts keep the 64-bit keys
 constexpr int kQDBits = 20;               // the sub-bin offset is scaled to under 2^20 first
+
+// Start ranks per quantile bucket: ceil(n / kQBuckets) when the last
+// bucket's first rank still lies below n (the last bucket then holds no more
+// than the others), else n / kQBuckets (the last bucket takes the rest).
+__host__ __device__ inline uint32_t quantile_stride(int64_t n)
+{
+    const int64_t c = (n + kQBuckets - 1) / kQBuckets;
+    return (uint32_t)((int64_t)(kQBuckets - 1) * c <= n - 1 ? c : n / kQBuckets);
+}
 
 // One bucket: k = radius key of its first quantile; ms = (shift << 56) | mul,
 // sub-bin = (((key - k) >> shift) * mul) >> 32 (integers only, so the map is
@@ -290,6 +299,18 @@ struct SegTable {
     int64_t capacity = 0;           // tiles the device table can hold
 };
 
+// Device workspace of the bucket sort of quantile keys (ns_qsort.hip),
+// allocated by qsort_prepare for one n; cap == 0: not allocated (or n too
+// large for a bucket to fit a workgroup), the LSD passes sort quantile keys.
+struct QSortBufs {
+    uint2 *elem = nullptr;     // (kQBuckets + 1) regions of cap (key, position) pairs, one per bucket
+    uint2 *ovf = nullptr;      // n pairs: the elements past their region's cap
+    uint32_t *cnt = nullptr;   // kQBuckets + 1 bucket counters (zero between sorts)
+    uint32_t *ctl = nullptr;   // overflow count, bucket workgroups done (zero between sorts)
+    int cap = 0;               // elements per region (the bucket kernel's capacity)
+    int64_t n = 0;
+};
+
 // Device workspace of one radix sort.  Key buffers hold 64-bit keys, or
 // 32-bit keys in their first half.
 struct SortWork {
@@ -303,6 +324,7 @@ struct SortWork {
     uint32_t *counters;  // kKeyPasses tile tickets (+ spare)
     uint32_t epoch;      // lookback epoch of the next pass (host-owned)
     SegTable seg;        // segments (reduce-then-scan only)
+    QSortBufs qs;        // bucket sort of quantile keys
 };
 // Frees the device part of a segment table.
 void seg_table_release(SegTable &t);
@@ -334,7 +356,18 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
 // of r and input position: the stable sort on r.  Uses both key and payload
 // buffers, the tile-count buffer and counters[48].
 hipError_t radix_sort_perm_q24(SortWork &w, int64_t n, const double *r, uint32_t *perm_out, hipStream_t stream,
-                               Profiler *prof);
+                               Profiler *prof, bool buckets = true);
+// The bucket sort's workspace for n keys (ns_qsort.hip): allocated, and its
+// counters cleared, once per n; a no-op when n is too large for a bucket to
+// fit a workgroup.  Synchronous: call it outside any capture.
+hipError_t qsort_prepare(QSortBufs &b, int64_t n);
+void qsort_release(QSortBufs &b);
+// The stable sort of the quantile keys in keys (n >= kQMinN, b prepared for
+// n, deferred particles at the all-ones key): perm_out[k] = input position of
+// rank k by (radius key of r, position), the deferred last by position.
+// scratch: n entries no other argument aliases.
+hipError_t qsort_perm(QSortBufs &b, const uint32_t *keys, int64_t n, const double *r, uint32_t *perm_out,
+                      uint32_t *scratch, hipStream_t stream, Profiler *prof);
 
 // Kernel-family ids for profiling.
 enum KernelId : int {
@@ -362,6 +395,8 @@ enum KernelId : int {
     K_PLACE,
     K_SORT_FIXUP,
     K_INT_LC,
+    K_QS_SCATTER,
+    K_QS_BUCKET,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -669,7 +669,7 @@ hipError_t launch_quantile_table(const double *r0, const uint32_t *perm, int64_t
     if (n < kQMinN)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_quantile_table, dim3(kQBuckets / 256), dim3(256), 0, stream, r0, perm, n,
-                       (uint32_t)(n / kQBuckets), q);
+                       quantile_stride(n), q);
     return hipGetLastError();
 }
 

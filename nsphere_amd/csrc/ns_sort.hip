@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ns_device.h"
+#include "ns_radix.h"
 
 namespace ns {
 
@@ -54,53 +55,6 @@ __device__ inline void lb_store(uint64_t *p, uint64_t v)
 __device__ inline uint64_t lb_load(const uint64_t *p)
 {
     return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline uint64_t lanemask_lt()
-{
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// Lanes (within `valid`) whose 8-bit digit equals this lane's: one ballot per
-// digit bit, each folded in as peers &= (ballot ^ ~bit-mask) on both 32-bit
-// halves -- a single v_bitop3 per half on gfx950.
-__device__ inline uint64_t match_digit(uint32_t d, uint64_t valid)
-{
-    uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
-    const uint32_t nd = ~d;
-#pragma unroll
-    for (int b = 0; b < kRadixBits; b++) {
-        const uint32_t nm = (uint32_t)__builtin_amdgcn_sbfe((int)nd, b, 1);  // ~0 where bit b of d is 0
-        const uint64_t bb = __ballot(nm == 0);                                 // lanes whose bit b is 1
-        lo &= (uint32_t)bb ^ nm;
-        hi &= (uint32_t)(bb >> 32) ^ nm;
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Exclusive scan of x over the first 256 threads (one per digit; the first
-// four waves).  Every thread of the workgroup must call it (it synchronises);
-// threads past 256 get an unspecified value.
-__device__ inline uint32_t digit_exclusive_scan(uint32_t x, uint32_t *s_wave)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t incl = x;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off, 64);
-        if (lane >= off)
-            incl += y;
-    }
-    if (lane == 63 && wave < 4)
-        s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-#pragma unroll
-    for (int w = 0; w < 4; w++)
-        before += (w < wave) ? s_wave[w] : 0u;
-    __syncthreads();
-    return before + incl - x;
 }
 
 template <typename S>
@@ -870,11 +824,6 @@ struct TieRadius {
     __device__ T operator()(uint32_t p) const { return radius_key(r[p]); }
 };
 
-// The deferral sentinel (all ones, on no radius) marks a run of exact ties:
-// its tie keys are never read (the late kernel may still be writing those
-// particles' radii).
-constexpr uint32_t kSentinelWord = 0xFFFFFFFFu;
-
 template <class Lo>
 __global__ __launch_bounds__(64 * kFxWaves) void k_split_fixup(const uint32_t *__restrict__ hs, Lo lo,
                                                                  const uint32_t *__restrict__ perm_hi,
@@ -1205,10 +1154,13 @@ hipError_t radix_sort_perm(SortWork &w, int64_t n, int key_bits, uint32_t *perm_
 }
 
 hipError_t radix_sort_perm_q24(SortWork &w, int64_t n, const double *r, uint32_t *perm_out, hipStream_t stream,
-                               Profiler *prof)
+                               Profiler *prof, bool buckets)
 {
     if (n <= 0)
         return hipSuccess;
+    if (buckets && w.qs.cap > 0 && w.qs.n == n)
+        return qsort_perm(w.qs, reinterpret_cast<const uint32_t *>(w.keys[0]), n, r, perm_out, w.vals[0], stream,
+                          prof);
     uint32_t *count = w.counters + kLongRunCounter;
     // three passes on the low 24 bits; the last keeps its keys, which end in
     // keys[1] (keys[0] -> keys[1] -> keys[0] -> keys[1]), and its payload

@@ -1,6 +1,9 @@
 """NS_OPT_SORT_KEYS: between steps the float64 radii are sorted on 24-bit
-quantile keys (three radix passes, then each run of equal keys ordered by
-the full radius key) instead of the two-plane 64-bit keys.
+quantile keys instead of the two-plane 64-bit keys: by bucket (1: one
+scatter into per-bucket regions, then one workgroup per bucket ordering it by
+sub-bin, radius and position; buckets too large for a workgroup, or with a
+run of equal sub-bins past its limit, go to a fallback kernel) or by three
+radix passes and a fix-up of each run of equal keys by the full radius key (2).
 
 The reference sorts the radii at the start of every method-5 step with a
 stable sort (nsphere.c:5784-5787, quadsort at nsphere.c:8356-8359); the
@@ -10,7 +13,9 @@ permutation after a run must equal the oracle's stable sort of the radii that
 step started from, and whole runs with either key form must agree bit for
 bit (eager and overlapped, the strict plain and the fast binned forms).
 Blocks of identical particles give runs of equal keys far longer than a
-wave's look-ahead, which reach the long-run kernel.
+wave's look-ahead, which reach the long-run kernel, and buckets past a
+workgroup's capacity or with runs past its limit, which reach the bucket
+sort's fallback.
 """
 import ctypes
 
@@ -61,10 +66,11 @@ def run_then_step(ns, state, keys, form, steps, overlap):
         return ctx.download(), r, ctx.permutation()
 
 
+@pytest.mark.parametrize("keys", [1, 2])
 @pytest.mark.parametrize("form,overlap", [(0, 0), (2, 0), (2, 1)])
-def test_quantile_keys_match_64bit_keys_and_oracle(ns, form, overlap):
+def test_quantile_keys_match_64bit_keys_and_oracle(ns, form, overlap, keys):
     state = ic(ns, 300_000, 71)
-    fin_q, r_q, perm_q = run_then_step(ns, state, 1, form, 6, overlap)
+    fin_q, r_q, perm_q = run_then_step(ns, state, keys, form, 6, overlap)
     fin_w, r_w, perm_w = run_then_step(ns, state, 0, form, 6, overlap)
     assert_state(fin_q, fin_w, "quantile vs 64-bit keys")
     np.testing.assert_array_equal(r_q.view(np.uint64), r_w.view(np.uint64))
@@ -74,22 +80,27 @@ def test_quantile_keys_match_64bit_keys_and_oracle(ns, form, overlap):
     assert np.count_nonzero(perm_q != np.arange(perm_q.size, dtype=np.uint32)) > perm_q.size // 2
 
 
-def test_quantile_keys_long_runs_of_exact_ties(ns):
+@pytest.mark.parametrize("keys", [1, 2])
+def test_quantile_keys_long_runs_of_exact_ties(ns, keys):
     """Blocks of identical particles: the first step's quantile table holds
     about 14 equal quantiles per block, so the block's new radii (split only
     by the rank-dependent enclosed mass, by about 1e-7) land in one or two
     sub-bins of one bucket, and the second step's sort meets runs of tens of
-    thousands of equal keys."""
+    thousands of equal keys (buckets far past a workgroup's capacity), and,
+    from the blocks of 100 and 200, runs below and above the bucket kernel's
+    limit of 128 in buckets it can hold."""
     r, v, ell, ids = ic(ns, 200_000, 72)
-    # 40 000 copies of one particle and 30 000 of another, 2 000 twin pairs
+    # 40 000 copies of one particle and 30 000 of another, 200 and 100 of
+    # two more, 2 000 twin pairs
     r, v, ell = r.copy(), v.copy(), ell.copy()
-    for lo, hi, src in ((1_000, 41_000, 0), (100_000, 130_000, 150_000)):
+    for lo, hi, src in ((1_000, 41_000, 0), (100_000, 130_000, 150_000), (160_000, 160_200, 170_000),
+                        (180_000, 180_100, 190_000)):
         r[lo:hi], v[lo:hi], ell[lo:hi] = r[src], v[src], ell[src]
     twins = np.arange(150_001, 154_001, 2)
     r[twins + 1], v[twins + 1], ell[twins + 1] = r[twins], v[twins], ell[twins]
     state = (r, v, ell, ids)
     for form, overlap in ((0, 0), (2, 1)):
-        fin_q, r_q, perm_q = run_then_step(ns, state, 1, form, 4, overlap)
+        fin_q, r_q, perm_q = run_then_step(ns, state, keys, form, 4, overlap)
         fin_w, _, perm_w = run_then_step(ns, state, 0, form, 4, overlap)
         assert_state(fin_q, fin_w, f"form {form}: quantile vs 64-bit keys")
         np.testing.assert_array_equal(perm_q, perm_w)
@@ -109,3 +120,17 @@ def test_quantile_keys_graph_replay(ns):
             ctx.run(p, 35)
             out.append(ctx.download())
     assert_state(out[0], out[1], "graph vs eager")
+
+
+def test_bucket_sort_large_regions(ns):
+    """At N = 2e7 a bucket holds about 4900 start radii and the last one
+    about 8200, past the smaller bucket kernel's 6144: the sort takes the
+    16384-entry regions and kernel.  Steps of the fast binned form, overlapped,
+    agree bit for bit with the 64-bit key sort, and the last permutation is
+    the oracle's stable sort."""
+    state = ic(ns, 20_000_000, 74)
+    fin_q, r_q, perm_q = run_then_step(ns, state, 1, 2, 3, 1)
+    fin_w, _, perm_w = run_then_step(ns, state, 0, 2, 3, 1)
+    assert_state(fin_q, fin_w, "bucket sort vs 64-bit keys at N = 2e7")
+    np.testing.assert_array_equal(perm_q, perm_w)
+    np.testing.assert_array_equal(perm_q, oracle.sort_perm(r_q))
