@@ -535,7 +535,7 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
     return None
 
 
-SIDE_STREAM = {"integrate_late", "overlap_plan"}
+SIDE_STREAM = {"integrate_late", "overlap_plan", "overlap_order"}
 
 
 def dominant_kernel(run: dict) -> str:

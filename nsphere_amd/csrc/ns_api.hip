@@ -67,7 +67,7 @@ struct ns_gpu_ctx {
     hipStream_t side = nullptr;
     // fork / join events of eager steps, and a second set that only graph
     // captures record (a captured record leaves its event unusable eagerly)
-    hipEvent_t ev[3] = {}, ev_cap[3] = {};
+    hipEvent_t ev[4] = {}, ev_cap[4] = {};
     bool uploaded = false;
     bool keys_valid = false;   // sw.keys[0] holds keys of r[cur]
     bool keys_q24 = false;     // ... quantile keys on qtab (else two-plane radius keys)
@@ -128,7 +128,7 @@ const char *kernel_name(int id)
                                          "snapshot_records", "potential", "density",
                                          "rank_files", "integrate_late", "overlap_plan",
                                          "overlap_place", "sort_fixup", "integrate_lc",
-                                         "qsort_scatter", "qsort_bucket"};
+                                         "qsort_scatter", "qsort_bucket", "overlap_order"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -285,7 +285,7 @@ void free_all(ns_gpu_ctx *c)
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++)
         for (hipEvent_t e : {c->ev[k], c->ev_cap[k]})
             if (e)
                 (void)hipEventDestroy(e);
@@ -316,7 +316,7 @@ int ensure_overlap(ns_gpu_ctx *c)
         e = dalloc(&c->ov.ctl, 8);
     if (e == hipSuccess && !c->side)
         e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++)
         for (hipEvent_t *ev : {&c->ev[k], &c->ev_cap[k]})
             if (e == hipSuccess && !*ev)
                 e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
@@ -638,19 +638,26 @@ bool overlap_applies(const ns_gpu_ctx *c, int64_t nsteps)
 }
 
 // One step of an overlapped run.  Context's stream: the sort (late keys at
-// the sentinel).  Second stream, after that sort and after the previous
-// step's late kernel: the plan, then this step's late kernel -- one chain
-// hands over to the next without a round trip through the context's stream.
-// Context's stream, after the plan: the placement of the early elements and
-// the bulk and deep kernels.  ev: the fork / join events (sorted, merged,
-// late), c->ev eagerly or c->ev_cap under a capture.
+// the sentinel).  Second stream, after the previous step's integrate kernels
+// on the context's stream and its own late kernel: the order of the late
+// particles, beside the sort; after the sort: the plan, then this step's late
+// kernel -- one chain hands over to the next without a round trip through
+// the context's stream.  Context's stream, after the plan: the placement of
+// the early elements and the bulk and deep kernels.  ev: the fork / join
+// events (sorted, merged, late, integrated), c->ev eagerly or c->ev_cap under
+// a capture.
 int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev)
 {
+    const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
+    HIP_TRY(hipEventRecord(ev[3], c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, ev[3], 0));
+    c->prof.begin(ns::K_ORDER, c->side);
+    HIP_TRY(ns::overlap_order(c->ov, r, c->f32, c->seg_off, c->nseg, c->n, c->side));
+    c->prof.end(ns::K_ORDER, c->side);
     if (int rc = sort_perm(c))
         return rc;
     HIP_TRY(hipEventRecord(ev[0], c->stream));
     HIP_TRY(hipStreamWaitEvent(c->side, ev[0], 0));
-    const void *r = c->f32 ? (const void *)c->rf[c->cur] : (const void *)c->r[c->cur];
     c->prof.begin(ns::K_MERGE, c->side);
     HIP_TRY(ns::overlap_plan(c->ov, c->perm, r, c->f32, c->seg_off, c->nseg, c->n,
                              c->prof.on ? c->work + ns::kWorkKernels * c->work_stride : nullptr, c->side));

@@ -397,6 +397,7 @@ enum KernelId : int {
     K_INT_LC,
     K_QS_SCATTER,
     K_QS_BUCKET,
+    K_ORDER,
     K_COUNT
 };
 const char *kernel_name(int id);

@@ -171,14 +171,20 @@ struct OverlapBufs {
 // order) the slow list is seeded from them, else left empty.
 hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t *seg_off, int nseg, int64_t n,
                          hipStream_t stream);
-// After the sort of r (state type S, keys of deferred ranks = sentinel) into
-// sorted, in two launches.  overlap_plan: the deferred ranks of the coming
-// step (the previous step's slow list at their new ranks) in def_pos /
-// def_src / ctl[1], and the late ranks' merged positions in perm -- all the
-// late kernel reads.  overlap_place: the early elements at their merged
-// positions in perm and the deferred ranks flagged -- what the bulk kernel
-// reads.  chain (profiling, else null): the kChainWords counters, whose
-// running maximum overlap_plan folds into the sum.
+// The merge of one overlapped step, in three launches.  overlap_order, once
+// the previous step's integrate kernels (both streams) have ended -- it does
+// not need this step's sort: the late particles (the previous step's
+// deferred ranks, old ranks def_pos) in (halo, key, old rank) order, and the
+// slow-list entries to defer next.  overlap_plan, after the sort of r (state
+// type S, keys of deferred ranks = sentinel) into sorted: the deferred ranks
+// of the coming step (the previous step's slow list at their new ranks) in
+// def_pos / def_src / ctl[1], and the late ranks' merged positions in perm --
+// all the late kernel reads.  overlap_place: the early elements at their
+// merged positions in perm and the deferred ranks flagged -- what the bulk
+// kernel reads.  chain (profiling, else null): the kChainWords counters,
+// whose running maximum overlap_plan folds into the sum.
+hipError_t overlap_order(const OverlapBufs &o, const void *r, bool f32, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream);
 hipError_t overlap_plan(const OverlapBufs &o, const uint32_t *sorted, const void *r, bool f32, const int64_t *seg_off,
                         int nseg, int64_t n, unsigned long long *chain, hipStream_t stream);
 hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int64_t *seg_off, int nseg, int64_t n,

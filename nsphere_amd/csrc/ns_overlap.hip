@@ -226,6 +226,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     __shared__ uint32_t s_old[kSlowCap];
     __shared__ uint32_t s_slot[kSlowCap];
     __shared__ uint32_t s_bin[kCostBins];  // cost histogram, then the first slot of each cost
+    __shared__ uint32_t s_wtot[3];
     const uint32_t tid = threadIdx.x;
     const uint32_t K = late_count(o, g.n);
     if (tid < (uint32_t)kCostBins)
@@ -287,13 +288,25 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
         within[u] = have ? atomicAdd(&s_bin[cost[u]], 1u) : ~0u;
     }
     __syncthreads();
-    if (tid == 0) {  // first slot of each cost, costliest first
-        uint32_t acc = 0;
-        for (int b = kCostBins - 1; b >= 0; b--) {
-            const uint32_t x = s_bin[b];
-            s_bin[b] = acc;
-            acc += x;
+    {   // first slot of each cost, costliest first: an exclusive scan over
+        // the bins from the top, thread q taking bin kCostBins - 1 - q
+        static_assert(kCostBins <= 192, "three waves scan the cost bins");
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+        const int bq = kCostBins - 1 - (int)tid;
+        const uint32_t x = bq >= 0 ? s_bin[bq] : 0u;
+        uint32_t incl = x;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off)
+                incl += y;
         }
+        if (lane == 63 && wave < 3)
+            s_wtot[wave] = incl;
+        __syncthreads();
+        const uint32_t before = (wave > 0 ? s_wtot[0] : 0u) + (wave > 1 ? s_wtot[1] : 0u);
+        if (bq >= 0)
+            s_bin[bq] = before + incl - x;
     }
     __syncthreads();
 #pragma unroll
@@ -549,13 +562,10 @@ template <typename S>
 hipError_t plan_typed(const OverlapBufs &o, const Halos &g, const uint32_t *sorted, const S *r,
                       unsigned long long *chain, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_overlap_order<S>, dim3(1), dim3(kPlanThreads), 0, stream, o, g, r);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return e;
     hipLaunchKernelGGL(k_overlap_search<S>, dim3(2 * kSlowCap / (kSearchThreads / 64)), dim3(kSearchThreads), 0,
                        stream, o, g, sorted, r);
-    if ((e = hipGetLastError()) != hipSuccess)
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_overlap_plan, dim3(1), dim3(kPlanThreads), 0, stream, o, g, chain);
     return hipGetLastError();
@@ -572,6 +582,23 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t
         return e;
     hipLaunchKernelGGL(k_overlap_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, o,
                        halos_of(seg_off, nseg, n), lvl);
+    return hipGetLastError();
+}
+
+hipError_t overlap_order(const OverlapBufs &o, const void *r, bool f32, const int64_t *seg_off, int nseg, int64_t n,
+                         hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    if (n > (int64_t)kIndexMask)
+        return hipErrorInvalidValue;
+    const Halos g = halos_of(seg_off, nseg, n);
+    if (f32)
+        hipLaunchKernelGGL(k_overlap_order<float>, dim3(1), dim3(kPlanThreads), 0, stream, o, g,
+                           static_cast<const float *>(r));
+    else
+        hipLaunchKernelGGL(k_overlap_order<double>, dim3(1), dim3(kPlanThreads), 0, stream, o, g,
+                           static_cast<const double *>(r));
     return hipGetLastError();
 }
 

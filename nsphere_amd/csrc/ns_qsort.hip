@@ -320,17 +320,20 @@ __device__ void qs_fallback(uint32_t b, uint32_t M, const uint2 *__restrict__ el
             A[e] = src[e];
 }
 
-// 2. One workgroup per bucket (the sentinel bucket kQBuckets among them):
-// its offset from the counters of the buckets below, elements k * THREADS +
-// tid of its region in registers, LDS counts per sub-bin (a slot each from
-// the atomic), their exclusive scan, the positions in sub-bin order, then each
-// element's rank in its run of equal sub-bins by (radius key, position).
-// STAGE: the elements of runs of two or more gather their radius keys once,
-// all at the same time, into LDS beside their positions, and rank against
-// those; otherwise (the larger regions, whose keys would not fit the LDS)
-// each comparison gathers the other element's radius.  Buckets past the
-// capacity or the run limit take qs_fallback.  Every workgroup then counts
-// itself done; the last one clears the counters (all others have read them).
+// 2. One workgroup per bucket (the sentinel bucket kQBuckets among them).
+// Everything it reads first comes in one round trip: the counters (its
+// offset is the sum of those below it, its count its own) and its whole
+// region (entries past the count are ignored).  It then counts itself done
+// (all its counter reads have arrived), so the last workgroup to do so may
+// clear the counters for the next sort while the others work on.  Its
+// elements k * THREADS + tid: LDS counts per sub-bin (a slot each from the
+// atomic) and, at the same time, their radius keys gathered from r; the
+// counts' exclusive scan; the positions in sub-bin order; then each element's
+// rank in its run of equal sub-bins by (radius key, position).  STAGE: the
+// elements of runs of two or more put their radius keys in LDS beside their
+// positions and rank against those; otherwise (the larger regions, whose keys
+// would not fit the LDS) each comparison gathers the other element's radius.
+// Buckets past the capacity or the run limit take qs_fallback.
 template <int ITEMS, bool STAGE>
 __global__ __launch_bounds__(kQsBucketThreads) void k_qs_bucket(const uint2 *__restrict__ elem, uint32_t cap,
                                                                const uint2 *__restrict__ ovf,
@@ -350,40 +353,56 @@ __global__ __launch_bounds__(kQsBucketThreads) void k_qs_bucket(const uint2 *__r
     constexpr int kWords = kSubWords + kPosWords + kRkWords > kFbWords ? kSubWords + kPosWords + kRkWords : kFbWords;
     __shared__ __align__(16) uint32_t s_lds[kWords];
     __shared__ uint32_t s_w[WAVES], s_m[WAVES];
+    __shared__ uint32_t s_mk[2];  // the bucket's count, the overflow count
     __shared__ uint32_t s_last;
     uint32_t *s_sub = s_lds;               // counts, then first slot of each sub-bin
     uint32_t *s_pos = s_lds + kSubWords;   // positions in sub-bin order, then in final order
     uint64_t *s_rk = reinterpret_cast<uint64_t *>(s_lds + kSubWords + kPosWords);  // radius keys in sub-bin order
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t b = xcd_contiguous_block(blockIdx.x, gridDim.x);
-    // offset: the counters of buckets 4 tid .. 4 tid + 3 below b
-    const uint4 c4 = reinterpret_cast<const uint4 *>(cnt)[tid];
+    const bool sentinel = b == (uint32_t)kQBuckets;
+    const uint4 c4 = reinterpret_cast<const uint4 *>(cnt)[tid];  // buckets 4 tid .. 4 tid + 3
+    if (tid == 0) {
+        s_mk[0] = cnt[b];
+        s_mk[1] = ctl[kQcOvf];
+    }
+    const uint2 *src = elem + (size_t)b * cap;
+    uint2 v[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++)
+        v[k] = src[k * THREADS + tid];  // inside the region (cap == CAP)
+    for (int i = tid; i < kQsSubs; i += THREADS)
+        s_sub[i] = 0;
     const uint32_t t4 = 4u * (uint32_t)tid;
     const uint32_t below = (t4 < b ? c4.x : 0u) + (t4 + 1 < b ? c4.y : 0u) + (t4 + 2 < b ? c4.z : 0u) +
                            (t4 + 3 < b ? c4.w : 0u);
-    const uint32_t lo = block_sum<WAVES>(below, s_w);
-    const uint32_t M = cnt[b];
-    const bool sentinel = b == (uint32_t)kQBuckets;
+    const uint32_t lo = block_sum<WAVES>(below, s_w);  // its barriers also publish s_mk and the cleared s_sub
+    const uint32_t M = s_mk[0], K = s_mk[1];
+    if (tid == 0)
+        s_last = atomicAdd(&ctl[kQcTicket], 1u) == gridDim.x - 1 ? 1u : 0u;
     bool fallback = M > (uint32_t)CAP;
     if (M > 0 && !fallback) {
-        for (int i = tid; i < kQsSubs; i += THREADS)
-            s_sub[i] = 0;
-        __syncthreads();
-        const uint2 *src = elem + (size_t)b * cap;
         uint32_t pos[ITEMS], sub[ITEMS], slot[ITEMS];
+        double rr[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t e = (uint32_t)(k * THREADS + tid);
             pos[k] = sub[k] = slot[k] = 0;
+            rr[k] = 0.0;
             if (e < M) {
-                const uint2 v = src[e];
-                pos[k] = v.y;
+                pos[k] = v[k].y;
                 // deferred particles: a sub-bin that grows with the position
-                sub[k] = sentinel ? (uint32_t)(((uint64_t)v.y << kQSubBits) / (uint64_t)n) : (v.x & (kQsSubs - 1));
-                slot[k] = atomicAdd(&s_sub[sub[k]], 1u);
+                sub[k] = sentinel ? (uint32_t)(((uint64_t)pos[k] << kQSubBits) / (uint64_t)n)
+                                  : (v[k].x & (kQsSubs - 1));
+                if (!sentinel)
+                    rr[k] = r[pos[k]];  // the radii arrive while the counts are taken and scanned
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++)
+            if ((uint32_t)(k * THREADS + tid) < M)
+                slot[k] = atomicAdd(&s_sub[sub[k]], 1u);
+        lds_barrier();
         uint32_t c[PER], sum = 0, mx = 0;
 #pragma unroll
         for (int j = 0; j < PER; j++) {
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(kQsBucketThreads) void k_qs_bucket(const uint2 *__r
             s_w[wave] = incl;
         if (lane == 0)
             s_m[wave] = mx;
-        __syncthreads();
+        lds_barrier();
         uint32_t ex = incl - sum, maxrun = 0;
 #pragma unroll
         for (int w = 0; w < WAVES; w++) {
@@ -417,33 +436,24 @@ __global__ __launch_bounds__(kQsBucketThreads) void k_qs_bucket(const uint2 *__r
             ex += c[j];
         }
         fallback = !sentinel && maxrun > kQsMaxRun;  // uniform: every thread read the same s_m
-        __syncthreads();
+        lds_barrier();
         if (!fallback) {
             // run of each element: [st, en) in sub-bin order
             uint32_t st[ITEMS], en[ITEMS];
+            uint64_t rk[ITEMS];
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 st[k] = en[k] = 0;
+                rk[k] = sentinel ? 0ull : radius_key(rr[k]);
                 if ((uint32_t)(k * THREADS + tid) < M) {
                     st[k] = s_sub[sub[k]];
                     en[k] = sub[k] + 1 < (uint32_t)kQsSubs ? s_sub[sub[k] + 1] : M;
                     s_pos[st[k] + slot[k]] = pos[k];
+                    if (STAGE && en[k] - st[k] > 1)
+                        s_rk[st[k] + slot[k]] = rk[k];
                 }
             }
-            uint64_t rk[ITEMS];
-#pragma unroll
-            for (int k = 0; k < ITEMS; k++) {
-                rk[k] = 0;
-                if (!sentinel && en[k] - st[k] > 1)
-                    rk[k] = radius_key(r[pos[k]]);
-            }
-            if constexpr (STAGE) {
-#pragma unroll
-                for (int k = 0; k < ITEMS; k++)
-                    if (en[k] - st[k] > 1)
-                        s_rk[st[k] + slot[k]] = rk[k];
-            }
-            __syncthreads();
+            lds_barrier();
             uint32_t fin[ITEMS];
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
@@ -462,23 +472,20 @@ __global__ __launch_bounds__(kQsBucketThreads) void k_qs_bucket(const uint2 *__r
                 }
                 fin[k] = st[k] + rank;
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int k = 0; k < ITEMS; k++)
                 if ((uint32_t)(k * THREADS + tid) < M)
                     s_pos[fin[k]] = pos[k];
-            __syncthreads();
+            lds_barrier();
             for (uint32_t e = tid; e < M; e += THREADS)
                 out[lo + e] = s_pos[e];
         }
     }
     if (fallback)  // uniform
-        qs_fallback<THREADS>(b, M, elem, cap, ovf, ctl[kQcOvf], r, n, out + lo, scratch + lo, s_lds);
-    __syncthreads();
-    if (tid == 0)
-        s_last = atomicAdd(&ctl[kQcTicket], 1u) == gridDim.x - 1 ? 1u : 0u;
-    __syncthreads();
-    if (s_last) {
+        qs_fallback<THREADS>(b, M, elem, cap, ovf, K, r, n, out + lo, scratch + lo, s_lds);
+    __syncthreads();  // s_last
+    if (s_last) {  // every other workgroup has read the counters
         for (int i = tid; i < kQCntWords; i += THREADS)
             cnt[i] = 0;
         if (tid == 0) {

@@ -39,6 +39,16 @@ __device__ inline uint64_t match_digit(uint32_t d, uint64_t valid)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Barrier for LDS traffic only: the workgroup's LDS writes before it are seen
+// by all its waves after it, while their global loads stay in flight (a
+// __syncthreads waits for those too).
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive scan of x over the first 256 threads (one per digit; the first
 // four waves).  Every thread of the workgroup must call it (it synchronises);
 // threads past 256 get an unspecified value.
