@@ -35,11 +35,11 @@ FLOPS_PER_FORCE_EVAL = 16     # FP64 flops per force evaluation + kick/drift, a 
 # over keys as wide as the state, so 8 passes of 64-bit keys for float64):
 # float64 304, float32 124.  This build's algorithm: between steps the
 # float64 sort buckets 24-bit quantile keys (one scatter of key and position
-# into per-bucket regions, 12 B, then one pass ordering each bucket, 16 B:
-# 28 B instead of 200 B), and the integrate writes a 4-byte key instead of 8,
-# so a float64 step moves 128 B.
+# into per-bucket regions, 12 B, then one pass ordering each bucket, 20 B:
+# 32 B instead of 200 B), and the integrate writes a 4-byte key instead of 8,
+# so a float64 step moves 132 B.
 CANONICAL_STEP_BYTES = {"f64": 304, "f32": 124}
-STEP_BYTES = {"f64": 128, "f32": 124}
+STEP_BYTES = {"f64": 132, "f32": 124}
 
 
 def bytes_per_particle(state_bytes: int) -> dict:
@@ -73,10 +73,10 @@ def bytes_per_particle(state_bytes: int) -> dict:
     if wide:
         out["sort_fixup"] = 16
         # bucket sort of the quantile keys: read the key, write (key, position);
-        # then per bucket read (key, position), the radius of the half of the
-        # positions that share their sub-bin, write the permutation
+        # then per bucket read (key, position) and the radius at the position,
+        # write the permutation
         out["qsort_scatter"] = 4 + 8
-        out["qsort_bucket"] = 8 + 4 + 4
+        out["qsort_bucket"] = 8 + 8 + 4
     return out
 
 
@@ -90,11 +90,10 @@ TRAFFIC_PROFILES = {
     ("f32", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f32_overlap.md",
                           {"sort_scatter": 171.4e6, "sort_upsweep": 50.04e6, "integrate_bulk": 496.6e6,
                            "integrate_lc": 7.51e6, "integrate_deep": 0.41e6, "integrate_late": 0.20e6}),
-    # sort_scatter: mean of the 3 passes' read + write
     ("f64", 12_000_000): ("profiles/hbm_traffic_plummer_n1.2e7_binned_fast_f64_overlap.md",
-                          {"sort_scatter": 180.1e6, "sort_upsweep": 50.05e6, "sort_fixup": 243.5e6,
-                           "integrate_bulk": 785.2e6, "integrate_lc": 10.01e6, "integrate_deep": 0.45e6,
-                           "integrate_late": 0.25e6}),
+                          {"qsort_scatter": 145.1e6, "qsort_bucket": 278.9e6,
+                           "integrate_bulk": 785.2e6, "integrate_lc": 9.99e6, "integrate_deep": 0.41e6,
+                           "integrate_late": 0.46e6}),
 }
 
 WORK_ROW = {"integrate": "bulk_or_plain", "integrate_bulk": "bulk_or_plain", "integrate_deep": "deep",
@@ -226,7 +225,7 @@ def graph_steps_for(warmup: int) -> int:
 
 
 def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, work_cap: int,
-            mass=None, halos=None, graph=False, autotune=0, mlp=None, overlap=True) -> dict:
+            mass=None, halos=None, graph=False, autotune=0, mlp=None, overlap=True, sort_keys=1) -> dict:
     """Time `steps` steps after `warmup`.  HIP-event timing of the roofline
     kernel families (ROOFLINE_KERNELS) runs over the timed region, unless
     graph: then the warmup captures the steps, the timed region replays them
@@ -244,6 +243,7 @@ def measure(ns, form: str, precision: str, state, p, steps: int, warmup: int, wo
         ctx.set_option(ns.NS_OPT_INTEGRATOR, FORMS[form])
         ctx.set_option(ns.NS_OPT_BUDGET, work_cap)
         ctx.set_option(ns.NS_OPT_OVERLAP, int(overlap))
+        ctx.set_option(ns.NS_OPT_SORT_KEYS, sort_keys)
         if graph:
             # every option before the warmup (setting one drops the captured
             # graph): the warmup run captures the graph, the timed region
@@ -601,6 +601,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="every kernel on one stream (NS_OPT_OVERLAP=0): the same results, for comparison")
+    ap.add_argument("--sort-keys", type=int, choices=[0, 1, 2], default=1,
+                    help="NS_OPT_SORT_KEYS of the chosen form: 1 (default) buckets the 24-bit quantile keys between "
+                         "steps, 2 sorts them with three radix passes and a fix-up, 0 sorts the 64-bit radius keys")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step's kernels eagerly (by default the timed region replays HIP "
                          "graphs of captured steps, NS_OPT_GRAPH, captured during the warmup)")
@@ -668,10 +671,11 @@ def main() -> int:
     if args.mlp:
         mlp, learned = train_level_predictor(ns, kind, args.mlp, rmax, lowl, args.seed, p, args.budget)
     chosen = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass, halos,
-                     not args.no_graph, args.autotune, mlp, overlap=not args.no_overlap)
+                     not args.no_graph, args.autotune, mlp, overlap=not args.no_overlap, sort_keys=args.sort_keys)
     if mlp is not None:  # the same form without the network
         without = measure(ns, args.form, args.precision, state, p, args.steps, args.warmup, args.budget, mass,
-                          halos, not args.no_graph, args.autotune, overlap=not args.no_overlap)
+                          halos, not args.no_graph, args.autotune, overlap=not args.no_overlap,
+                          sort_keys=args.sort_keys)
         learned.update({"value_without": without["value"], "ms_per_step_without": without["ms_per_step"],
                         "gain": chosen["value"] / without["value"],
                         "kernels_without": without["kernels"],
@@ -699,7 +703,8 @@ def main() -> int:
         "config": {"workload": args.workload, "n": n, "dt": p.dt, "form": args.form,
                    "state": args.precision, "lowl_frac": lowl, "lowl_scale": 1e-3, "seed": args.seed,
                    "masses": args.masses, "halos": None if halos is None else int(len(halos)),
-                   "graph": not args.no_graph, "overlap": not args.no_overlap and args.form != "plain"},
+                   "graph": not args.no_graph, "overlap": not args.no_overlap and args.form != "plain",
+                   "sort_keys": args.sort_keys},
         "autotune": chosen["autotune"],
         "roofline": roofline_for(chosen, n, dom),
         "roofline_sort": sort_roofline(chosen, n),
