@@ -432,6 +432,9 @@ typedef struct ns_tune_config {
     int32_t sort_alg, sort_threads, sort_items; /* NS_OPT_SORT_ALG / _TILE    */
     int32_t lane_levels;                        /* NS_OPT_LANE_LEVELS        */
     int32_t deep_blocks;                        /* NS_OPT_DEEP_BLOCKS        */
+    int32_t sort_keys;                          /* NS_OPT_SORT_KEYS: the
+                                                   digits of the sort
+                                                   between steps             */
     double ms_per_step;                         /* measured (HIP events)     */
 } ns_tune_config;
 #define NS_TUNE_MAX 32

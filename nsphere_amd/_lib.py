@@ -74,7 +74,8 @@ class TuneConfig(ctypes.Structure):
     """Mirror of ns_tune_config."""
 
     _fields_ = [("sort_alg", ctypes.c_int32), ("sort_threads", ctypes.c_int32), ("sort_items", ctypes.c_int32),
-                ("lane_levels", ctypes.c_int32), ("deep_blocks", ctypes.c_int32), ("ms_per_step", ctypes.c_double)]
+                ("lane_levels", ctypes.c_int32), ("deep_blocks", ctypes.c_int32), ("sort_keys", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

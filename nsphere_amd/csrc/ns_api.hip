@@ -1849,6 +1849,7 @@ int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_r
             lv++;
         t.lane_levels = lv;
         t.deep_blocks = c->deep_blocks;
+        t.sort_keys = c->sort_keys;
         return t;
     };
     auto apply = [&](const ns_tune_config &t) {
@@ -1857,6 +1858,7 @@ int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_r
         c->sw.config.items = t.sort_items;
         c->cap_N = (int64_t)1 << (t.lane_levels - 1);
         c->deep_blocks = t.deep_blocks;
+        c->sort_keys = t.sort_keys;
     };
     int rc = NS_OK;
     // one trial: restore, one untimed step, `steps` timed steps
@@ -1884,6 +1886,19 @@ int ns_gpu_autotune(ns_gpu_ctx *c, const ns_step_params *p, int steps, ns_tune_r
     const ns_tune_config orig = current();
     ns_tune_config best = orig;
     best.ms_per_step = 1e300;
+    // knob 0, where quantile keys apply: the digits of the sort between
+    // steps -- a 12-bit bucket digit and a 12-bit sub-bin (1), three 8-bit
+    // digits (2), or the 64-bit keys' four 8-bit digits of their high half
+    // and a fix-up (0)
+    if (!c->f32 && c->nseg == 0 && c->n >= ns::kQMinN)
+        for (int sk : {1, 2, 0}) {
+            if (rc)
+                break;
+            ns_tune_config t = best;
+            t.sort_keys = sk;
+            if ((rc = trial(t)) == NS_OK && t.ms_per_step < best.ms_per_step)
+                best = t;
+        }
     // knob 1: pass algorithm x tile shape
     static const int kTiles[][2] = {{256, 16}, {256, 24}, {512, 8}, {512, 12}, {512, 16}, {1024, 6}};
     constexpr int kNumTiles = (int)(sizeof kTiles / sizeof kTiles[0]);
