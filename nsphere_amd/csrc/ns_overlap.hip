@@ -34,7 +34,7 @@ namespace {
 constexpr int kPlanThreads = 1024;  // == kSlowCap: one thread per entry
 constexpr int kSearchThreads = 256;
 constexpr int kShiftThreads = 256;
-constexpr int kShiftItems = 8;
+constexpr int kShiftItems = 16;
 constexpr int kShiftTile = kShiftThreads * kShiftItems;
 constexpr int kShiftHalos = 64;  // halo bounds of a shift tile held in LDS
 static_assert(kPlanThreads == kSlowCap, "one plan thread per slow-list entry");
@@ -435,7 +435,46 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
     // of dependent global loads
     for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads)
         s_all[q] = o.ins[q];
+    if (threadIdx.x < 2)
+        s_rng[threadIdx.x] = 0;
     __syncthreads();
+    if (!g.off) {
+        // one halo: pairs (0, e), e ascending.  The tile's pairs are those
+        // with b0 <= e <= b1 - 1, [wlo, whi), counted by all threads at once
+        uint32_t clo = 0, chi = 0;
+        for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads) {
+            const uint32_t e = (uint32_t)s_all[q];
+            clo += e < b0 ? 1u : 0u;
+            chi += e <= b1 - 1 ? 1u : 0u;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            clo += __shfl_xor(clo, off, 64);
+            chi += __shfl_xor(chi, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&s_rng[0], clo);
+            atomicAdd(&s_rng[1], chi);
+        }
+        __syncthreads();
+        const uint32_t wlo = s_rng[0], whi = s_rng[1], E = g.n - K;  // E: the early elements
+#pragma unroll
+        for (int it = 0; it < kShiftItems; it++) {
+            const uint32_t p = b0 + (uint32_t)it * kShiftThreads + threadIdx.x;
+            if (p >= b1 || p >= E)
+                break;
+            uint32_t lo = wlo, hi = whi;  // the pairs with e <= p number lo
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((uint32_t)s_all[mid] <= p)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            o.perm[p + lo] = sorted[p];
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         const uint32_t h0 = g.of(b0), h1 = g.of(b1 - 1);
         const uint32_t i0 = count_lt(s_all, K, hl_pack(h0, 0));
