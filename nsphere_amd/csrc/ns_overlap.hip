@@ -26,6 +26,7 @@
 
 #include "ns_device.h"
 #include "ns_kernels.h"
+#include "ns_radix.h"
 
 namespace ns {
 
@@ -212,19 +213,22 @@ __device__ inline void late_run(const uint32_t *late_h, uint32_t K, uint32_t h, 
 }
 
 // One block.  The ctl[1] late particles (old ranks def_pos[j]) in (halo,
-// key, old rank) order by a bitonic sort in LDS: late_h / late_t /
-// late_slot[r] for the r-th.  And the slow-list entries to defer: the
-// costliest kSlowCap of both classes by a counting sort on the cost (the
+// key, old rank) order by a bitonic sort of one entry per thread held in
+// registers: stages whose partner (tid ^ j) lies in the same wave exchange
+// through shuffles, the others (j >= 64, ten stages for 1024 entries)
+// through two alternating LDS buffers, one barrier each.  Into late_h /
+// late_t / late_slot[r] for the r-th.  And the slow-list entries to defer:
+// the costliest kSlowCap of both classes by a counting sort on the cost (the
 // order within a cost is free: it decides only which wave runs which
 // particle), into sel / ctl[4].
 template <typename S>
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, Halos g, const S *__restrict__ r)
 {
     using KeyT = typename KeyOf<S>::type;
-    __shared__ uint32_t s_h[kSlowCap];
-    __shared__ KeyT s_key[kSlowCap];
-    __shared__ uint32_t s_old[kSlowCap];
-    __shared__ uint32_t s_slot[kSlowCap];
+    __shared__ uint32_t s_h[2][kSlowCap];
+    __shared__ KeyT s_key[2][kSlowCap];
+    __shared__ uint32_t s_old[2][kSlowCap];
+    __shared__ uint32_t s_slot[2][kSlowCap];
     __shared__ uint32_t s_bin[kCostBins];  // cost histogram, then the first slot of each cost
     __shared__ uint32_t s_wtot[3];
     const uint32_t tid = threadIdx.x;
@@ -234,46 +238,56 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     uint32_t P = 1;  // padded above every real entry to a power of two
     while (P < K)
         P <<= 1;
+    // entry tid; padding entries sort above every real one, and are distinct
+    // from each other (so that no comparison meets a tie)
+    uint32_t h = ~0u, t = ~0u - tid, slot = tid;
+    KeyT key = ~KeyT(0);
     if (tid < K) {
-        const uint32_t t = o.def_pos[tid];
-        s_h[tid] = g.of(t);
-        s_key[tid] = key_at(r, t);
-        s_old[tid] = t;
-    } else {
-        s_h[tid] = ~0u;
-        s_key[tid] = ~KeyT(0);
-        s_old[tid] = ~0u;
+        t = o.def_pos[tid];
+        h = g.of(t);
+        key = key_at(r, t);
     }
-    s_slot[tid] = tid;
-    __syncthreads();
+    int buf = 0;
     for (uint32_t k = 2; k <= P; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t l = tid ^ j;
-            if (tid < P && l > tid) {
-                const uint32_t ha = s_h[tid], hb = s_h[l];
-                const KeyT ka = s_key[tid], kb = s_key[l];
-                const uint32_t ta = s_old[tid], tb = s_old[l];
-                const bool a_above = ha > hb || (ha == hb && (ka > kb || (ka == kb && ta > tb)));
-                if (a_above == ((tid & k) == 0)) {  // ascending where bit k of tid is clear
-                    s_h[tid] = hb;
-                    s_h[l] = ha;
-                    s_key[tid] = kb;
-                    s_key[l] = ka;
-                    s_old[tid] = tb;
-                    s_old[l] = ta;
-                    const uint32_t x = s_slot[tid];
-                    s_slot[tid] = s_slot[l];
-                    s_slot[l] = x;
-                }
+            uint32_t hb, tb, sb;
+            KeyT kb;
+            if (j >= 64) {
+                s_h[buf][tid] = h;
+                s_key[buf][tid] = key;
+                s_old[buf][tid] = t;
+                s_slot[buf][tid] = slot;
+                lds_barrier();
+                const uint32_t l = tid ^ j;
+                hb = s_h[buf][l];
+                kb = s_key[buf][l];
+                tb = s_old[buf][l];
+                sb = s_slot[buf][l];
+                buf ^= 1;  // the next LDS stage writes the other buffer
+            } else {
+                hb = __shfl_xor(h, (int)j, 64);
+                kb = __shfl_xor(key, (int)j, 64);
+                tb = __shfl_xor(t, (int)j, 64);
+                sb = __shfl_xor(slot, (int)j, 64);
             }
-            __syncthreads();
+            const bool b_above = hb > h || (hb == h && (kb > key || (kb == key && tb > t)));
+            // ascending where bit k of tid is clear: the lower index of the
+            // pair keeps the smaller entry there, the upper one the larger
+            const bool keep_min = ((tid & j) == 0) == ((tid & k) == 0);
+            if (keep_min != b_above) {
+                h = hb;
+                key = kb;
+                t = tb;
+                slot = sb;
+            }
         }
     }
     if (tid < K) {
-        o.late_h[tid] = s_h[tid];
-        o.late_t[tid] = s_old[tid];
-        o.late_slot[tid] = s_slot[tid];
+        o.late_h[tid] = h;
+        o.late_t[tid] = t;
+        o.late_slot[tid] = slot;
     }
+    lds_barrier();  // the cleared cost bins (no LDS stage ran when P < 128)
     // selection: entry q of class 2 (q < m2) or class 1 (kSlowList + q, q < m1)
     constexpr int kPer = kSlowList / kPlanThreads;
     static_assert(kSlowList % kPlanThreads == 0 && kCoreList <= kPlanThreads, "entries per thread");
