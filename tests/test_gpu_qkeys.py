@@ -134,3 +134,20 @@ def test_bucket_sort_large_regions(ns):
     assert_state(fin_q, fin_w, "bucket sort vs 64-bit keys at N = 2e7")
     np.testing.assert_array_equal(perm_q, perm_w)
     np.testing.assert_array_equal(perm_q, oracle.sort_perm(r_q))
+
+
+@pytest.mark.parametrize("n", [65_536, 65_537, 70_001])
+def test_bucket_sort_smallest_sizes(ns, n):
+    """The smallest N that takes quantile keys: 16 or 17 start radii per bucket,
+    so that a 4096-position tile spans about 250 buckets of the scatter's
+    window; at 65 537 the stride falls back to floor(N / 4096) because
+    ceil(N / 4096) = 17 would put the last bucket's first rank past N.
+    Steps of the fast binned form, eager and overlapped, agree bit for bit
+    with the 64-bit key sort."""
+    state = ic(ns, n, 75)
+    for overlap in (0, 1):
+        fin_q, r_q, perm_q = run_then_step(ns, state, 1, 2, 5, overlap)
+        fin_w, _, perm_w = run_then_step(ns, state, 0, 2, 5, overlap)
+        assert_state(fin_q, fin_w, f"N={n} overlap={overlap}: bucket sort vs 64-bit keys")
+        np.testing.assert_array_equal(perm_q, perm_w)
+        np.testing.assert_array_equal(perm_q, oracle.sort_perm(r_q))
