@@ -554,19 +554,30 @@ int ensure_qtab(ns_gpu_ctx *c)
     return NS_OK;
 }
 
-// The quantile table of the step's start radii (buffer cur, in the order of
-// perm) for the keys the integrate writes, when they are quantile keys.
-int quantile_table(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm)
+// Set-up of a binned integrate, one launch: the quantile table of the
+// step's start radii (buffer cur, in the order of perm) when the integrate
+// writes quantile keys, the deferred ranks' flags in perm (flags: an
+// overlapped step, after its placement), and the queue counters cleared.
+int step_prep(ns_gpu_ctx *c, ns::StepArgs &a, const uint32_t *perm, bool flags)
 {
     a.qtab = nullptr;
     a.q_stride = 0;
-    if (!quantile_applies(c) || !a.keys)
-        return NS_OK;
-    if (int rc = ensure_qtab(c))
-        return rc;
-    HIP_TRY(ns::launch_quantile_table(c->r[c->cur], perm, c->n, c->qtab, c->stream));
-    a.qtab = c->qtab;
-    a.q_stride = ns::quantile_stride(c->n);
+    a.q_cleared = false;
+    const bool table = quantile_applies(c) && a.keys;
+    if (table)
+        if (int rc = ensure_qtab(c))
+            return rc;
+    if (!table && !flags && c->integrator == ns::FORM_PLAIN)
+        return NS_OK;  // the plain kernel uses no queues
+    HIP_TRY(ns::launch_step_prep(c->r[c->cur], perm, c->n, table ? c->qtab : nullptr,
+                                 c->integrator == ns::FORM_PLAIN ? nullptr : c->q_count,
+                                 flags ? c->ov.def_pos : nullptr, flags ? c->ov.ctl + 1 : nullptr,
+                                 flags ? c->ov.perm : nullptr, c->stream));
+    if (table) {
+        a.qtab = c->qtab;
+        a.q_stride = ns::quantile_stride(c->n);
+    }
+    a.q_cleared = c->integrator != ns::FORM_PLAIN;
     return NS_OK;
 }
 
@@ -618,7 +629,7 @@ int integrate(ns_gpu_ctx *c, const ns_step_params *p, bool gather)
         c->prof.end(ns::K_MLP);
         a.order = c->order;
     }
-    if (int rc = quantile_table(c, a, gather ? c->perm : nullptr))
+    if (int rc = step_prep(c, a, gather ? c->perm : nullptr, false))
         return rc;
     HIP_TRY(ns::launch_integrate(a, gather, c->integrator, c->f32, c->stream, &c->prof));
     if (gather)
@@ -678,7 +689,7 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev
     c->prof.begin(ns::K_PLACE);
     HIP_TRY(ns::overlap_place(c->ov, c->perm, c->seg_off, c->nseg, c->n, c->stream));
     c->prof.end(ns::K_PLACE);
-    if (int rc = quantile_table(c, a, c->ov.perm))
+    if (int rc = step_prep(c, a, c->ov.perm, true))
         return rc;
     HIP_TRY(ns::launch_integrate(a, true, c->integrator, c->f32, c->stream, &c->prof));
     c->last_perm = c->ov.perm;

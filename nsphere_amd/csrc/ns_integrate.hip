@@ -426,19 +426,32 @@ __global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__res
         out[0] = lf.rho + lf.v_end() + lf.t_end(0.0);
 }
 
-__global__ void k_quantile_table(const double *__restrict__ r0, const uint32_t *__restrict__ perm, int64_t n,
-                                 uint32_t stride, QEntry *__restrict__ q)
+// Blocks [0, tb): the quantile table (tb = kQBuckets / 256 with q, else 0);
+// the blocks after: the deferred ranks' flags; block 0 also clears the queue
+// counters.  The parts write disjoint words (the table reads perm masked, so
+// a flag set beside it changes nothing it reads).
+__global__ void k_step_prep(const double *__restrict__ r0, const uint32_t *__restrict__ perm, int64_t n,
+                            uint32_t stride, QEntry *__restrict__ q, uint32_t *__restrict__ q_count,
+                            const uint32_t *__restrict__ def_pos, const uint32_t *__restrict__ def_count,
+                            uint32_t *__restrict__ flag_perm)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= kQBuckets)
+    const int tb = q ? kQBuckets / 256 : 0;
+    if (blockIdx.x == 0 && q_count && threadIdx.x < (unsigned)kQueueCounters)
+        q_count[threadIdx.x] = 0;
+    if ((int)blockIdx.x < tb) {
+        const int b = blockIdx.x * blockDim.x + threadIdx.x;
+        const bool last = b == kQBuckets - 1;
+        const int64_t i0 = (int64_t)b * stride, i1 = last ? n - 1 : i0 + stride;
+        const int64_t p0 = perm ? (int64_t)(perm[i0] & kIndexMask) : i0;
+        const int64_t p1 = perm ? (int64_t)(perm[i1] & kIndexMask) : i1;
+        const uint64_t k0 = radius_key(r0[p0]), k1 = radius_key(r0[p1]);
+        q[b].k = k0;
+        q[b].ms = quantile_ms(k0, k1, last);
         return;
-    const bool last = b == kQBuckets - 1;
-    const int64_t i0 = (int64_t)b * stride, i1 = last ? n - 1 : i0 + stride;
-    const int64_t p0 = perm ? (int64_t)(perm[i0] & kIndexMask) : i0;
-    const int64_t p1 = perm ? (int64_t)(perm[i1] & kIndexMask) : i1;
-    const uint64_t k0 = radius_key(r0[p0]), k1 = radius_key(r0[p1]);
-    q[b].k = k0;
-    q[b].ms = quantile_ms(k0, k1, last);
+    }
+    const uint32_t j = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
+    if (flag_perm && j < *def_count)
+        flag_perm[def_pos[j]] |= kDeferBit;
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int n)
@@ -586,10 +599,12 @@ hipError_t launch_form(const StepArgs &a, int form, hipStream_t stream, Profiler
         end(K_INTEGRATE);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, kQueueCounters);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return e;
+    hipError_t e = hipSuccess;
+    if (!a.q_cleared) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.q_count, kQueueCounters);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
     begin(K_INT_BULK);
     const dim3 bulk_grid((unsigned)((a.n + kThreads * kBulkPPT - 1) / (kThreads * kBulkPPT)));
     hipLaunchKernelGGL((k_integrate_bulk<GATHER, FAST, S>), bulk_grid, dim3(kThreads), 0, stream, a);
@@ -664,12 +679,17 @@ hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *de
     return hipGetLastError();
 }
 
-hipError_t launch_quantile_table(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, hipStream_t stream)
+hipError_t launch_step_prep(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, uint32_t *q_count,
+                            const uint32_t *def_pos, const uint32_t *def_count, uint32_t *flag_perm,
+                            hipStream_t stream)
 {
-    if (n < kQMinN)
+    if (q && n < kQMinN)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_quantile_table, dim3(kQBuckets / 256), dim3(256), 0, stream, r0, perm, n,
-                       quantile_stride(n), q);
+    const int tb = q ? kQBuckets / 256 : 0;
+    const int fb = flag_perm ? kSlowCap / 256 : 0;
+    const int blocks = tb + fb > 0 ? tb + fb : 1;
+    hipLaunchKernelGGL(k_step_prep, dim3(blocks), dim3(256), 0, stream, r0, perm, n, q ? quantile_stride(n) : 1u, q,
+                       q_count, def_pos, def_count, flag_perm);
     return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ struct StepArgs {
                                  // than the lane levels (wave per particle)
     uint32_t *q_count;           // [0] length of q_lf, [1] of q_lc, [2] the deep
                                  // kernel's ticket, [3] length of q_lcw
+    bool q_cleared;              // q_count already cleared for this step (launch_step_prep)
     int64_t cap_N;               // deepest leapfrog N tried lane-per-particle
     int32_t deep_blocks;         // grid of the wave-per-particle kernel (256 threads each)
     // profiling only (else null): force evaluations spent, one slot per
@@ -180,8 +181,8 @@ hipError_t overlap_reset(const OverlapBufs &o, const uint8_t *lvl, const int64_t
 // of the coming step (the previous step's slow list at their new ranks) in
 // def_pos / def_src / ctl[1], and the late ranks' merged positions in perm --
 // all the late kernel reads.  overlap_place: the early elements at their
-// merged positions in perm and the deferred ranks flagged -- what the bulk
-// kernel reads.  chain (profiling, else null): the kChainWords counters,
+// merged positions in perm; with the deferred ranks then flagged
+// (launch_step_prep), what the bulk kernel reads.  chain (profiling, else null): the kChainWords counters,
 // whose running maximum overlap_plan folds into the sum.
 hipError_t overlap_order(const OverlapBufs &o, const void *r, bool f32, const int64_t *seg_off, int nseg, int64_t n,
                          hipStream_t stream);
@@ -196,9 +197,15 @@ hipError_t launch_late(const StepArgs &a, int form, bool f32, const uint32_t *de
                        const uint32_t *count, hipStream_t stream);
 // One wave running the late kernel's fast Levi-Civita micro-step `steps` times.
 hipError_t launch_chain_floor(int64_t steps, double *out, hipStream_t stream);
-// The quantile table of the start radii r0 in the order of perm (rank ->
-// position, kDeferBit ignored; null: r0 is in rank order), n >= kQMinN.
-hipError_t launch_quantile_table(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, hipStream_t stream);
+// What a step needs between its placement and its integrate, in one launch
+// of independent parts: the quantile table of the start radii r0 in the
+// order of perm (rank -> position, kDeferBit ignored; null: r0 is in rank
+// order; n >= kQMinN) when q is given; the deferred ranks def_pos[j < *def_count]
+// flagged (kDeferBit) in flag_perm when given; the binned integrator's queue
+// counters q_count cleared when given.
+hipError_t launch_step_prep(const double *r0, const uint32_t *perm, int64_t n, QEntry *q, uint32_t *q_count,
+                            const uint32_t *def_pos, const uint32_t *def_count, uint32_t *flag_perm,
+                            hipStream_t stream);
 // Radius keys of the deferred ranks of the last step (replacing the sentinel).
 hipError_t overlap_fix_keys(const OverlapBufs &o, const void *r, bool f32, void *keys, int64_t n,
                             const QEntry *qtab, uint32_t q_stride, hipStream_t stream);

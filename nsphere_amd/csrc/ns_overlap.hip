@@ -576,14 +576,6 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
     }
 }
 
-// The coming step's deferred ranks, flagged in the merged order.
-__global__ __launch_bounds__(kPlanThreads) void k_overlap_flag(OverlapBufs o)
-{
-    const uint32_t j = threadIdx.x;
-    if (j < o.ctl[1])
-        o.perm[o.def_pos[j]] |= kDeferBit;
-}
-
 template <typename S>
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_fix_keys(OverlapBufs o, const S *__restrict__ r,
                                                                    void *__restrict__ keys, int64_t n,
@@ -674,10 +666,6 @@ hipError_t overlap_place(const OverlapBufs &o, const uint32_t *sorted, const int
         return hipSuccess;
     hipLaunchKernelGGL(k_overlap_shift, dim3((unsigned)((n + kShiftTile - 1) / kShiftTile)), dim3(kShiftThreads), 0,
                        stream, o, halos_of(seg_off, nseg, n), sorted);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return e;
-    hipLaunchKernelGGL(k_overlap_flag, dim3(1), dim3(kPlanThreads), 0, stream, o);
     return hipGetLastError();
 }
 
