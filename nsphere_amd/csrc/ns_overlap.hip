@@ -37,6 +37,8 @@ constexpr int kSearchThreads = 256;
 constexpr int kShiftThreads = 256;
 constexpr int kShiftItems = 16;
 constexpr int kShiftTile = kShiftThreads * kShiftItems;
+constexpr int kShiftPairs = kSlowCap / kShiftThreads;  // insertion pairs per thread, at most
+static_assert(kSlowCap % kShiftThreads == 0, "whole insertion pairs per thread");
 constexpr int kShiftHalos = 64;  // halo bounds of a shift tile held in LDS
 static_assert(kPlanThreads == kSlowCap, "one plan thread per slow-list entry");
 
@@ -242,8 +244,21 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     // from each other (so that no comparison meets a tie)
     uint32_t h = ~0u, t = ~0u - tid, slot = tid;
     KeyT key = ~KeyT(0);
-    if (tid < K) {
+    if (tid < K)
         t = o.def_pos[tid];
+    // the selection's inputs (entry q of class 2, q < kSlowList, or of class
+    // 1, kSlowList + q): loaded now, used after the sort
+    constexpr int kPer = kSlowList / kPlanThreads;
+    static_assert(kSlowList % kPlanThreads == 0 && kCoreList == kPlanThreads, "entries per thread");
+    uint32_t rec[kPer + 1], ent[kPer + 1];
+#pragma unroll
+    for (int u = 0; u <= kPer; u++) {
+        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
+        rec[u] = o.slow_rec[q];
+        ent[u] = o.slow[q];
+    }
+    const uint32_t c2 = o.ctl[0], c1 = o.ctl[3];
+    if (tid < K) {
         h = g.of(t);
         key = key_at(r, t);
     }
@@ -289,16 +304,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     }
     lds_barrier();  // the cleared cost bins (no LDS stage ran when P < 128)
     // selection: entry q of class 2 (q < m2) or class 1 (kSlowList + q, q < m1)
-    constexpr int kPer = kSlowList / kPlanThreads;
-    static_assert(kSlowList % kPlanThreads == 0 && kCoreList <= kPlanThreads, "entries per thread");
-    const uint32_t m2 = o.ctl[0] < (uint32_t)kSlowList ? o.ctl[0] : (uint32_t)kSlowList;
-    const uint32_t m1 = o.ctl[3] < (uint32_t)kCoreList ? o.ctl[3] : (uint32_t)kCoreList;
+    const uint32_t m2 = c2 < (uint32_t)kSlowList ? c2 : (uint32_t)kSlowList;
+    const uint32_t m1 = c1 < (uint32_t)kCoreList ? c1 : (uint32_t)kCoreList;
     uint32_t cost[kPer + 1], within[kPer + 1];
 #pragma unroll
     for (int u = 0; u <= kPer; u++) {
         const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
         const bool have = u < kPer ? q < m2 : tid < m1;
-        cost[u] = have ? slow_cost(o.slow_rec[q], u < kPer) : 0u;
+        cost[u] = have ? slow_cost(rec[u], u < kPer) : 0u;
         within[u] = have ? atomicAdd(&s_bin[cost[u]], 1u) : ~0u;
     }
     __syncthreads();
@@ -327,10 +340,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     for (int u = 0; u <= kPer; u++) {
         if (within[u] == ~0u)
             continue;
-        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
         const uint32_t slot = s_bin[cost[u]] + within[u];
         if (slot < (uint32_t)kSlowCap)
-            o.sel[slot] = o.slow[q];
+            o.sel[slot] = ent[u];
     }
     if (tid == 0)
         o.ctl[4] = m2 + m1 < (uint32_t)kSlowCap ? m2 + m1 : (uint32_t)kSlowCap;
@@ -444,22 +456,41 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
     if (b0 >= g.n)
         return;
     const uint32_t b1 = b0 + (uint32_t)kShiftTile < g.n ? b0 + (uint32_t)kShiftTile : g.n;
-    // every insertion pair into LDS (coalesced, one round trip), so that the
+    // every insertion pair, then the tile's sorted positions, all loads in
+    // flight at once (one round trip); the pairs go to LDS, so that the
     // tile's range of pairs is found by searches in LDS rather than chains
     // of dependent global loads
-    for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads)
-        s_all[q] = o.ins[q];
+    uint64_t pr[kShiftPairs];
+#pragma unroll
+    for (int u = 0; u < kShiftPairs; u++) {
+        const uint32_t q = threadIdx.x + (uint32_t)u * kShiftThreads;
+        pr[u] = q < K ? o.ins[q] : ~0ull;
+    }
+    uint32_t val[kShiftItems];
+#pragma unroll
+    for (int it = 0; it < kShiftItems; it++) {
+        const uint32_t p = b0 + (uint32_t)it * kShiftThreads + threadIdx.x;
+        val[it] = p < b1 ? sorted[p] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kShiftPairs; u++) {
+        const uint32_t q = threadIdx.x + (uint32_t)u * kShiftThreads;
+        if (q < K)
+            s_all[q] = pr[u];
+    }
     if (threadIdx.x < 2)
         s_rng[threadIdx.x] = 0;
-    __syncthreads();
+    lds_barrier();  // the positions' loads stay in flight
     if (!g.off) {
         // one halo: pairs (0, e), e ascending.  The tile's pairs are those
         // with b0 <= e <= b1 - 1, [wlo, whi), counted by all threads at once
         uint32_t clo = 0, chi = 0;
-        for (uint32_t q = threadIdx.x; q < K; q += kShiftThreads) {
-            const uint32_t e = (uint32_t)s_all[q];
-            clo += e < b0 ? 1u : 0u;
-            chi += e <= b1 - 1 ? 1u : 0u;
+#pragma unroll
+        for (int u = 0; u < kShiftPairs; u++) {
+            const uint32_t q = threadIdx.x + (uint32_t)u * kShiftThreads;
+            const uint32_t e = (uint32_t)pr[u];
+            clo += q < K && e < b0 ? 1u : 0u;
+            chi += q < K && e <= b1 - 1 ? 1u : 0u;
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -470,7 +501,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
             atomicAdd(&s_rng[0], clo);
             atomicAdd(&s_rng[1], chi);
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t wlo = s_rng[0], whi = s_rng[1], E = g.n - K;  // E: the early elements
 #pragma unroll
         for (int it = 0; it < kShiftItems; it++) {
@@ -485,7 +516,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
                 else
                     hi = mid;
             }
-            o.perm[p + lo] = sorted[p];
+            o.perm[p + lo] = val[it];
         }
         return;
     }
@@ -571,7 +602,7 @@ __global__ __launch_bounds__(kShiftThreads) void k_overlap_shift(OverlapBufs o, 
                 else
                     hi = mid;
             }
-            o.perm[p + lo] = sorted[p];
+            o.perm[p + lo] = val[it];
         }
     }
 }

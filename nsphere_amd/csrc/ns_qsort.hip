@@ -6,7 +6,7 @@
 // bucket of the new radius among kQBuckets quantiles of the step's start
 // radii, then kQSubBits of position inside the bucket (ns_device.h).  The
 // key is a non-decreasing function of the radius key, and between steps a
-// particle moves a few buckets at most, so a workgroup's 4096 consecutive
+// particle moves a few buckets at most, so a workgroup's 3072 consecutive
 // positions hold keys of a handful of buckets.  Two kernels:
 //   1. k_qs_scatter: a workgroup ranks its keys per bucket in LDS (a window
 //      of buckets around the tile's expected first bucket), reserves one
@@ -44,8 +44,8 @@ namespace ns {
 namespace {
 
 constexpr int kQsThreads = 256;
-constexpr int kQsRounds = 4;                               // uint4 loads per thread
-constexpr int kQsTile = kQsThreads * 4 * kQsRounds;        // 4096 positions per workgroup
+constexpr int kQsRounds = 3;                               // uint4 loads per thread
+constexpr int kQsTile = kQsThreads * 4 * kQsRounds;        // 3072 positions per workgroup
 constexpr int kQsWin = 512;                                // buckets counted in LDS per tile
 constexpr int kQsWinBelow = 128;                           // ... below the tile's expected first bucket
 constexpr uint32_t kQsSent = kQsWin;                       // window index of the sentinel bucket
