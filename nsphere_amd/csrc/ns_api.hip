@@ -302,7 +302,7 @@ int ensure_overlap(ns_gpu_ctx *c)
     uint32_t **lists[] = {&c->ov.slow, &c->ov.slow_rec};
     for (uint32_t **b : lists)
         if (e == hipSuccess && !*b)
-            e = dalloc(b, ns::kSlowList + ns::kCoreList);
+            e = dalloc(b, ns::kSlowLists);
     uint32_t **small[] = {&c->ov.sel,    &c->ov.def_pos, &c->ov.def_src,
                           &c->ov.late_h, &c->ov.late_t,  &c->ov.late_slot};
     for (uint32_t **b : small)

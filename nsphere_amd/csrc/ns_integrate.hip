@@ -113,11 +113,37 @@ __device__ inline QWin stage_rank_window(const StepArgs &a, int64_t first, uint6
     return qwin_stage(a.qtab, w0, kQWin, sk, sms);
 }
 
+// Append (tag, word) to one class of the slow list for every lane with pred
+// set: the class's count in slow_count[ctr] (it goes on past the capacity
+// cap; entries past it are dropped), its entries from first.  One atomic per
+// wave, as a lane per particle may append tens of thousands a launch (on one
+// address they would serialise).  Must be reached by all active lanes.
+__device__ inline void slow_push(const StepArgs &a, bool pred, int ctr, uint32_t first, uint32_t cap, uint32_t tag,
+                                 uint32_t word)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(a.slow_count + ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const uint32_t k = base + (uint32_t)__popcll(m & lanemask_lt());
+    if (pred && k < cap) {
+        a.slow_list[first + k] = tag;
+        a.slow_rec[first + k] = word;
+    }
+}
+
 // slow_tag: what the slow list records for a slow result (the rank, or the
-// late kernel's slot | kDeferBit); local: the rank within its halo.
+// late kernel's slot | kDeferBit); local: the rank within its halo; spent:
+// the particle-step's force evaluations, where the kernel counts them.
 template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit,
-                                    uint32_t slow_tag, int64_t local, const QWin &qw = QWin())
+                                    uint32_t slow_tag, int64_t local, const QWin &qw = QWin(),
+                                    uint64_t spent = 0)
 {
     const S rs = (S)r;
     static_cast<S *>(a.r1)[i] = rs;
@@ -128,14 +154,12 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
         a.lvl[i] = (uint8_t)rec;
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
-    const int cls = a.slow_list ? slow_class(rec, local) : 0;
-    if (cls) {
-        const uint32_t k = atomicAdd(a.slow_count + (cls == 2 ? 0 : 3), 1u);
-        if (k < (uint32_t)(cls == 2 ? kSlowList : kCoreList)) {
-            const uint32_t q = (cls == 2 ? 0u : (uint32_t)kSlowList) + k;
-            a.slow_list[q] = slow_tag;
-            a.slow_rec[q] = (uint32_t)rec;
-        }
+    if (a.slow_list) {  // the same for every lane
+        const int cls = slow_class(rec, local, spent);
+        const uint32_t word = slow_word(rec, spent);
+        slow_push(a, cls == 2, 0, 0u, (uint32_t)kSlowList, slow_tag, word);
+        slow_push(a, cls == 1, 3, (uint32_t)kSlowList, (uint32_t)kCoreList, slow_tag, word);
+        slow_push(a, cls == 3, 5, (uint32_t)(kSlowList + kCoreList), (uint32_t)kSpentList, slow_tag, word);
     }
 }
 
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_lc_lanes(StepArgs a)
             if (rec == kEscalate)
                 escalate = true;
             else
-                write_result<S>(a, i, r_new, v_new, rec, bud.hit, i, rm.k.i, qw);
+                write_result<S>(a, i, r_new, v_new, rec, bud.hit, i, rm.k.i, qw, (uint64_t)(a.budget - bud.left));
             spent += (unsigned long long)(a.budget - bud.left);
         }
         wave_push(a.q_lcw, a.q_count + 3, escalate, i);
@@ -403,7 +427,7 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
         }
     }
     if ((threadIdx.x & 63) == 0)
-        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i, QWin(), (uint64_t)spent);
     if (a.work && (threadIdx.x & 63) == 0 && bud.chain > 0)
         atomicMax(a.work + kWorkKernels * a.work_stride + kChainMax, (unsigned long long)bud.chain);
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);

@@ -60,10 +60,11 @@ struct StepArgs {
     unsigned long long *work;
     int64_t work_stride;
     // overlapped run (else null): every result slow_class ranks appends its
-    // rank (the late kernel: its slot | kDeferBit) and its level record to
-    // slow_list / slow_rec -- class 2 at [0, kSlowList) counted in
+    // rank (the late kernel: its slot | kDeferBit) and its record word
+    // (slow_word) to slow_list / slow_rec -- class 2 at [0, kSlowList) counted in
     // slow_count[0], class 1 at [kSlowList, kSlowList + kCoreList) counted
-    // in slow_count[3] (the counts go on past the capacities).  In that mode
+    // in slow_count[3], class 3 at [kSlowList + kCoreList, kSlowLists)
+    // counted in slow_count[5] (the counts go on past the capacities).  In that mode
     // a perm entry with kDeferBit set marks a rank that the late kernel
     // integrates on the side stream: the bulk kernel only carries its ell and
     // id and writes the key sentinel (all ones, above every radius key).
@@ -86,6 +87,8 @@ constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 constexpr int kSlowCap = 1024;   // particles deferred per step
 constexpr int kSlowList = 4096;  // class-2 candidates kept per step
 constexpr int kCoreList = 1024;  // class-1 candidates kept per step
+constexpr int kSpentList = 1024; // class-3 candidates kept per step
+constexpr int kSlowLists = kSlowList + kCoreList + kSpentList;
 // Which particles are likely to need a wave of their own in the next step.
 // Class 2: the chain hit the cap, or it went past the lane kernels -- a
 // leapfrog level of kSlowLevel or more (past the bulk kernel's levels 0-3)
@@ -94,17 +97,36 @@ constexpr int kCoreList = 1024;  // class-1 candidates kept per step
 // the context's stream for the late kernel; a particle plunging towards the
 // centre climbs through these levels before it turns into a capped
 // Levi-Civita chain.  Class 1: otherwise, the particle sits among the
-// innermost kCoreRanks ranks of its halo, where the plunges end.  The
-// deferred particles are taken class 2 first, costliest first.
+// innermost kCoreRanks ranks of its halo, where the plunges end.  Class 3:
+// otherwise, the particle-step spent kSlowSpent force evaluations or more at
+// low levels (a Levi-Civita particle whose tau steps shrink near pericentre
+// takes many on one lane; such particles often need a wave in the next
+// step).  The deferred particles are taken class 2 first, then class 1, then
+// class 3, costliest first within a class; each class has a list of its own,
+// so that no class crowds out another.
 constexpr int kSlowLevel = 4;
 constexpr int kSlowLevelLc = 2;
+constexpr uint64_t kSlowSpent = 256;  // a Levi-Civita lane particle-step averages about 200
 constexpr int64_t kCoreRanks = 64;
-__host__ __device__ inline int slow_class(int rec, int64_t local_rank)
+__host__ __device__ inline int slow_class(int rec, int64_t local_rank, uint64_t spent = 0)
 {
     const int level = rec & NS_LVL_MASK;
     if ((rec & NS_LVL_NOCONV) || level >= ((rec & NS_LVL_LC) ? kSlowLevelLc : kSlowLevel))
         return 2;
-    return local_rank < kCoreRanks ? 1 : 0;
+    if (local_rank < kCoreRanks)
+        return 1;
+    return spent >= kSlowSpent ? 3 : 0;
+}
+// A slow-list entry's record word: the level record in bits 0-7 and, for a
+// particle-step that spent kSlowSpent force evaluations or more, 1 +
+// log2(spent / kSlowSpent) (at most 48) in bits 8-13, which orders class 3
+// when the deferred particles are chosen.
+__host__ __device__ inline uint32_t slow_word(int rec, uint64_t spent)
+{
+    uint32_t bin = 0;
+    for (uint64_t s = kSlowSpent; spent >= s && bin < 48u; s <<= 1)  // s stays below 2^57
+        bin++;
+    return (uint32_t)(rec & 0xFF) | (bin << 8);
 }
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
@@ -154,10 +176,11 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
 //   [1] deferred ranks of the coming step (entries of def_pos)
 //   [2] ranks deferred in the step just sorted (sorted last, by key sentinel)
 //   [4] slow-list entries selected for deferral (entries of sel)
+//   [5] class-3 results of this step (all kernels)
 struct OverlapBufs {
     uint32_t *perm;      // n: the merged order, kDeferBit on deferred ranks
-    uint32_t *slow;      // kSlowList + kCoreList: StepArgs::slow_list
-    uint32_t *slow_rec;  // kSlowList + kCoreList: the level record of each entry
+    uint32_t *slow;      // kSlowLists: StepArgs::slow_list
+    uint32_t *slow_rec;  // kSlowLists: the record word of each entry (slow_word)
     uint32_t *sel;       // kSlowCap: the slow-list entries deferred, costliest first
     uint32_t *def_pos;   // kSlowCap: deferred ranks, costliest first
     uint32_t *def_src;   // kSlowCap: their positions in the previous order

@@ -18,8 +18,9 @@
 //   * an early element of h at local sorted position e moves up by the number
 //     of late particles of h inserted at or before e.
 // Which particles are deferred: those whose previous step was slow
-// (slow_class: capped or deep, else among the innermost ranks of their halo;
-// the capped chains persist from step to step), found through the slow list
+// (slow_class: capped or deep, else among the innermost ranks of their halo,
+// else costly in force evaluations; the capped chains persist from step to
+// step), found through the slow list
 // the integrate kernels append to.  The choice affects only the time, never
 // the results.
 #include <hip/hip_runtime.h>
@@ -112,8 +113,8 @@ __global__ void k_overlap_seed(OverlapBufs o, Halos g, const uint8_t *__restrict
     if (t >= g.n)
         return;
     const uint8_t rec = lvl[t];
-    const int cls = slow_class(rec, t - g.start(g.of(t)));
-    if (!cls)
+    const int cls = slow_class(rec, t - g.start(g.of(t)));  // no force evaluations here: never class 3
+    if (cls != 1 && cls != 2)
         return;
     const uint32_t k = atomicAdd(o.ctl + (cls == 2 ? 0 : 3), 1u);
     if (k < (uint32_t)(cls == 2 ? kSlowList : kCoreList)) {
@@ -182,12 +183,32 @@ __device__ inline uint32_t selected_count(const OverlapBufs &o)
     return m > (uint32_t)kSlowCap ? (uint32_t)kSlowCap : m;
 }
 
-// Cost order of the deferred ranks: class 2 before class 1, each capped
-// chains first, then by level.
+// Cost order of the deferred ranks (the record word of slow_word): class 2
+// first (capped chains first, then by level), then class 1 (by level), then
+// class 3 (by spent bin), so that class 3 fills only the places the others
+// leave.
 constexpr int kCostBins = 130;
-__device__ inline uint32_t slow_cost(uint32_t rec, bool class2)
+__device__ inline uint32_t slow_cost(uint32_t word, int cls)
 {
-    return (class2 ? 65u : 0u) + ((rec & NS_LVL_NOCONV) ? 64u : (rec & NS_LVL_MASK));
+    const uint32_t level = word & NS_LVL_MASK, bin = (word >> 8) & 63u;
+    if (cls == 2)
+        return 65u + ((word & NS_LVL_NOCONV) ? 64u : level);
+    if (cls == 1)
+        return 32u + (level < 32u ? level : 32u);
+    return bin < 31u ? bin : 31u;
+}
+
+// The selection's entries per thread of k_overlap_order: kPer2 of class 2,
+// one of class 1, one of class 3; the u-th of thread tid.
+constexpr int kPer2 = kSlowList / kPlanThreads;
+constexpr int kSelPer = kPer2 + 2;
+static_assert(kSlowList % kPlanThreads == 0 && kCoreList == kPlanThreads && kSpentList == kPlanThreads,
+              "whole entries per thread");
+__device__ inline uint32_t entry_of(int u, uint32_t tid)
+{
+    if (u < kPer2)
+        return (uint32_t)u * kPlanThreads + tid;
+    return (uint32_t)(u == kPer2 ? kSlowList : kSlowList + kCoreList) + tid;
 }
 
 // Late particles of halo h in the (halo, key, rank)-sorted late list
@@ -220,7 +241,7 @@ __device__ inline void late_run(const uint32_t *late_h, uint32_t K, uint32_t h, 
 // through shuffles, the others (j >= 64, ten stages for 1024 entries)
 // through two alternating LDS buffers, one barrier each.  Into late_h /
 // late_t / late_slot[r] for the r-th.  And the slow-list entries to defer:
-// the costliest kSlowCap of both classes by a counting sort on the cost (the
+// the costliest kSlowCap of the three classes by a counting sort on the cost (the
 // order within a cost is free: it decides only which wave runs which
 // particle), into sel / ctl[4].
 template <typename S>
@@ -246,18 +267,16 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     KeyT key = ~KeyT(0);
     if (tid < K)
         t = o.def_pos[tid];
-    // the selection's inputs (entry q of class 2, q < kSlowList, or of class
-    // 1, kSlowList + q): loaded now, used after the sort
-    constexpr int kPer = kSlowList / kPlanThreads;
-    static_assert(kSlowList % kPlanThreads == 0 && kCoreList == kPlanThreads, "entries per thread");
-    uint32_t rec[kPer + 1], ent[kPer + 1];
+    // the selection's inputs (entry q of class 2, q < kSlowList, of class 1,
+    // kSlowList + q, and of class 3, kSlowList + kCoreList + q; u-th of a
+    // thread: entry_of): loaded now, used after the sort
+    uint32_t rec[kSelPer], ent[kSelPer];
 #pragma unroll
-    for (int u = 0; u <= kPer; u++) {
-        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
-        rec[u] = o.slow_rec[q];
-        ent[u] = o.slow[q];
+    for (int u = 0; u < kSelPer; u++) {
+        rec[u] = o.slow_rec[entry_of(u, tid)];
+        ent[u] = o.slow[entry_of(u, tid)];
     }
-    const uint32_t c2 = o.ctl[0], c1 = o.ctl[3];
+    const uint32_t c2 = o.ctl[0], c1 = o.ctl[3], c3 = o.ctl[5];
     if (tid < K) {
         h = g.of(t);
         key = key_at(r, t);
@@ -303,15 +322,17 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
         o.late_slot[tid] = slot;
     }
     lds_barrier();  // the cleared cost bins (no LDS stage ran when P < 128)
-    // selection: entry q of class 2 (q < m2) or class 1 (kSlowList + q, q < m1)
+    // selection: the entries each class holds (its count, at most its list)
     const uint32_t m2 = c2 < (uint32_t)kSlowList ? c2 : (uint32_t)kSlowList;
     const uint32_t m1 = c1 < (uint32_t)kCoreList ? c1 : (uint32_t)kCoreList;
-    uint32_t cost[kPer + 1], within[kPer + 1];
+    const uint32_t m3 = c3 < (uint32_t)kSpentList ? c3 : (uint32_t)kSpentList;
+    uint32_t cost[kSelPer], within[kSelPer];
 #pragma unroll
-    for (int u = 0; u <= kPer; u++) {
-        const uint32_t q = u < kPer ? (uint32_t)u * kPlanThreads + tid : (uint32_t)kSlowList + tid;
-        const bool have = u < kPer ? q < m2 : tid < m1;
-        cost[u] = have ? slow_cost(rec[u], u < kPer) : 0u;
+    for (int u = 0; u < kSelPer; u++) {
+        const int cls = u < kPer2 ? 2 : (u == kPer2 ? 1 : 3);
+        const uint32_t k = cls == 2 ? (uint32_t)u * kPlanThreads + tid : tid;  // index within its class
+        const bool have = k < (cls == 2 ? m2 : (cls == 1 ? m1 : m3));
+        cost[u] = have ? slow_cost(rec[u], cls) : 0u;
         within[u] = have ? atomicAdd(&s_bin[cost[u]], 1u) : ~0u;
     }
     __syncthreads();
@@ -337,7 +358,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u <= kPer; u++) {
+    for (int u = 0; u < kSelPer; u++) {
         if (within[u] == ~0u)
             continue;
         const uint32_t slot = s_bin[cost[u]] + within[u];
@@ -345,7 +366,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
             o.sel[slot] = ent[u];
     }
     if (tid == 0)
-        o.ctl[4] = m2 + m1 < (uint32_t)kSlowCap ? m2 + m1 : (uint32_t)kSlowCap;
+        o.ctl[4] = m2 + m1 + m3 < (uint32_t)kSlowCap ? m2 + m1 + m3 : (uint32_t)kSlowCap;
 }
 
 // One wave per query: wave w < kSlowCap is the w-th late particle (its
@@ -386,7 +407,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o
 // pairs, ascending, into ins; the selected entries (the previous step's
 // slow list, costliest first) at their new ranks, with their old positions,
 // into def_pos / def_src; their count into ctl[1], the late count into
-// ctl[2]; clears the class counts ctl[0] and ctl[3].
+// ctl[2]; clears the class counts ctl[0], ctl[3] and ctl[5].
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Halos g, unsigned long long *chain)
 {
     // the previous late launch has ended (same stream): fold its longest chain
@@ -430,6 +451,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Ha
     if (tid == 0) {
         o.ctl[0] = 0;
         o.ctl[3] = 0;
+        o.ctl[5] = 0;
         o.ctl[1] = m;
         o.ctl[2] = K;
     }
