@@ -677,7 +677,7 @@ int overlapped_step(ns_gpu_ctx *c, const ns_step_params *p, const hipEvent_t *ev
     ns::StepArgs a = state_args(c, p, true, c->ov.perm);
     a.slow_list = c->ov.slow;
     a.slow_rec = c->ov.slow_rec;
-    a.slow_count = c->ov.ctl;  // ctl[0] and ctl[3]
+    a.slow_count = c->ov.ctl;  // the region counts (slow_region_ctl)
     ns::StepArgs late = a;
     late.keys = nullptr;  // the sentinel stays until the next merge
     late.qtab = nullptr;

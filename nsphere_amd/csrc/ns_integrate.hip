@@ -113,37 +113,14 @@ __device__ inline QWin stage_rank_window(const StepArgs &a, int64_t first, uint6
     return qwin_stage(a.qtab, w0, kQWin, sk, sms);
 }
 
-// Append (tag, word) to one class of the slow list for every lane with pred
-// set: the class's count in slow_count[ctr] (it goes on past the capacity
-// cap; entries past it are dropped), its entries from first.  One atomic per
-// wave, as a lane per particle may append tens of thousands a launch (on one
-// address they would serialise).  Must be reached by all active lanes.
-__device__ inline void slow_push(const StepArgs &a, bool pred, int ctr, uint32_t first, uint32_t cap, uint32_t tag,
-                                 uint32_t word)
-{
-    const uint64_t m = __ballot(pred);
-    if (m == 0)
-        return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader)
-        base = atomicAdd(a.slow_count + ctr, (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    const uint32_t k = base + (uint32_t)__popcll(m & lanemask_lt());
-    if (pred && k < cap) {
-        a.slow_list[first + k] = tag;
-        a.slow_rec[first + k] = word;
-    }
-}
-
 // slow_tag: what the slow list records for a slow result (the rank, or the
 // late kernel's slot | kDeferBit); local: the rank within its halo; spent:
-// the particle-step's force evaluations, where the kernel counts them.
+// the particle-step's force evaluations, where the kernel counts them; soon:
+// a leapfrog particle about to turn Levi-Civita (lc_soon).
 template <typename S>
 __device__ inline void write_result(const StepArgs &a, int64_t i, double r, double v, int rec, bool hit,
                                     uint32_t slow_tag, int64_t local, const QWin &qw = QWin(),
-                                    uint64_t spent = 0)
+                                    uint64_t spent = 0, bool soon = false)
 {
     const S rs = (S)r;
     static_cast<S *>(a.r1)[i] = rs;
@@ -155,11 +132,11 @@ __device__ inline void write_result(const StepArgs &a, int64_t i, double r, doub
     if (hit)
         atomicAdd(a.budget_hits, 1ull);
     if (a.slow_list) {  // the same for every lane
-        const int cls = slow_class(rec, local, spent);
+        const int g = slow_region(rec, local, spent, soon);
         const uint32_t word = slow_word(rec, spent);
-        slow_push(a, cls == 2, 0, 0u, (uint32_t)kSlowList, slow_tag, word);
-        slow_push(a, cls == 1, 3, (uint32_t)kSlowList, (uint32_t)kCoreList, slow_tag, word);
-        slow_push(a, cls == 3, 5, (uint32_t)(kSlowList + kCoreList), (uint32_t)kSpentList, slow_tag, word);
+#pragma unroll
+        for (int r = 0; r < kSlowRegions; r++)
+            slow_push(a.slow_count, a.slow_list, a.slow_rec, r, g == r, slow_tag, word);
     }
 }
 
@@ -286,7 +263,9 @@ __global__ __launch_bounds__(kThreads) void k_integrate_bulk(StepArgs a)
                 if (rec == kEscalate)
                     to_lf = true;
                 else
-                    write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i, qw);
+                    write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i, qw, 0,
+                                    a.slow_list && (rec & NS_LVL_MASK) >= kSoonLevel &&
+                                        lc_soon(a, rm, r_new, v_new, ell[k]));
                 spent += (unsigned long long)(a.budget - bud.left);
             }
         }
@@ -378,7 +357,9 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deep(StepArgs a)
         const int rec = lc ? adaptive_lc<FAST, true>(ph, a, r, v, r_new, v_new, bud)
                            : adaptive_leap_wave<FAST>(ph, a, r, v, r_new, v_new, bud);
         if (lane == 0) {
-            write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i);
+            write_result<S>(a, i, r_new, v_new, rec, bud.hit, (uint32_t)i, rm.k.i, QWin(), 0,
+                            !lc && a.slow_list && (rec & NS_LVL_MASK) >= kSoonLevel &&
+                                lc_soon(a, rm, r_new, v_new, ell));
             spent += (unsigned long long)(a.budget - bud.left);  // the wave's particle, once
         }
     }
@@ -427,7 +408,8 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
         }
     }
     if ((threadIdx.x & 63) == 0)
-        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i, QWin(), (uint64_t)spent);
+        write_result<S>(a, i, r_new, v_new, rec, bud.hit, j | kDeferBit, rm.k.i, QWin(), (uint64_t)spent,
+                        !(rec & NS_LVL_LC) && (rec & NS_LVL_MASK) >= kSoonLevel && lc_soon(a, rm, r_new, v_new, ell));
     if (a.work && (threadIdx.x & 63) == 0 && bud.chain > 0)
         atomicMax(a.work + kWorkKernels * a.work_stride + kChainMax, (unsigned long long)bud.chain);
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);

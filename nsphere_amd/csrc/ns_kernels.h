@@ -59,12 +59,11 @@ struct StepArgs {
     // then the kChainWords chain counters
     unsigned long long *work;
     int64_t work_stride;
-    // overlapped run (else null): every result slow_class ranks appends its
-    // rank (the late kernel: its slot | kDeferBit) and its record word
-    // (slow_word) to slow_list / slow_rec -- class 2 at [0, kSlowList) counted in
-    // slow_count[0], class 1 at [kSlowList, kSlowList + kCoreList) counted
-    // in slow_count[3], class 3 at [kSlowList + kCoreList, kSlowLists)
-    // counted in slow_count[5] (the counts go on past the capacities).  In that mode
+    // overlapped run (else null): every result that slow_region places
+    // appends its rank (the late kernel: its slot | kDeferBit) and its
+    // record word (slow_word) to that region of slow_list / slow_rec, counted
+    // in slow_count[slow_region_ctl(region)] (the overlap's control words;
+    // the counts go on past the capacities).  In that mode
     // a perm entry with kDeferBit set marks a rank that the late kernel
     // integrates on the side stream: the bulk kernel only carries its ell and
     // id and writes the key sentinel (all ones, above every radius key).
@@ -85,10 +84,6 @@ constexpr int kQueueCounters = 4;
 constexpr uint32_t kDeferBit = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 constexpr int kSlowCap = 1024;   // particles deferred per step
-constexpr int kSlowList = 4096;  // class-2 candidates kept per step
-constexpr int kCoreList = 1024;  // class-1 candidates kept per step
-constexpr int kSpentList = 1024; // class-3 candidates kept per step
-constexpr int kSlowLists = kSlowList + kCoreList + kSpentList;
 // Which particles are likely to need a wave of their own in the next step.
 // Class 2: the chain hit the cap, or it went past the lane kernels -- a
 // leapfrog level of kSlowLevel or more (past the bulk kernel's levels 0-3)
@@ -102,20 +97,44 @@ constexpr int kSlowLists = kSlowList + kCoreList + kSpentList;
 // low levels (a Levi-Civita particle whose tau steps shrink near pericentre
 // takes many on one lane; such particles often need a wave in the next
 // step).  The deferred particles are taken class 2 first, then class 1, then
-// class 3, costliest first within a class; each class has a list of its own,
-// so that no class crowds out another.
+// class 3, costliest first within a class.
 constexpr int kSlowLevel = 4;
 constexpr int kSlowLevelLc = 2;
+constexpr int kSlowLevelHigh = 8;     // leapfrog levels from here on: the long class-2 chains
 constexpr uint64_t kSlowSpent = 256;  // a Levi-Civita lane particle-step averages about 200
 constexpr int64_t kCoreRanks = 64;
-__host__ __device__ inline int slow_class(int rec, int64_t local_rank, uint64_t spent = 0)
+// The slow list is cut into regions, each with its own capacity and count
+// (a control word of the overlap), so that no kind of candidate crowds out
+// another: the first come keep their places, and a workload can produce tens
+// of thousands of one kind per step while the chains that matter most -- the
+// capped ones and the deep ones, which end last -- append last.
+//   region 0, class 2: capped (not converged)
+//   region 1, class 2: Levi-Civita, leapfrog level kSlowLevelHigh or more,
+//             or a falling leapfrog particle about to turn Levi-Civita (soon)
+//   region 2, class 2: the other leapfrog levels
+//   region 3, class 1; region 4, class 3.
+constexpr int kSlowRegions = 5;
+constexpr int kSlowLists = 7 * kSlowCap;  // entries of all regions
+__host__ __device__ constexpr uint32_t slow_region_first(int g)
+{
+    return (uint32_t)kSlowCap * (g == 0 ? 0u : g == 1 ? 1u : g == 2 ? 3u : g == 3 ? 5u : 6u);
+}
+__host__ __device__ constexpr uint32_t slow_region_cap(int g) { return (uint32_t)kSlowCap * (g == 1 || g == 2 ? 2u : 1u); }
+__host__ __device__ constexpr int slow_region_ctl(int g) { return g == 0 ? 6 : g == 1 ? 0 : g == 2 ? 7 : g == 3 ? 3 : 5; }
+__host__ __device__ constexpr int slow_region_class(int g) { return g <= 2 ? 2 : (g == 3 ? 1 : 3); }
+static_assert(slow_region_first(4) + slow_region_cap(4) == (uint32_t)kSlowLists, "regions fill the list");
+// The region of a result (-1: not slow).
+__host__ __device__ inline int slow_region(int rec, int64_t local_rank, uint64_t spent = 0, bool soon = false)
 {
     const int level = rec & NS_LVL_MASK;
-    if ((rec & NS_LVL_NOCONV) || level >= ((rec & NS_LVL_LC) ? kSlowLevelLc : kSlowLevel))
-        return 2;
+    const bool lc = (rec & NS_LVL_LC) != 0;
+    if (rec & NS_LVL_NOCONV)
+        return 0;
+    if (level >= (lc ? kSlowLevelLc : kSlowLevel))
+        return (lc || level >= kSlowLevelHigh || soon) ? 1 : 2;
     if (local_rank < kCoreRanks)
-        return 1;
-    return spent >= kSlowSpent ? 3 : 0;
+        return 3;
+    return spent >= kSlowSpent ? 4 : -1;
 }
 // A slow-list entry's record word: the level record in bits 0-7 and, for a
 // particle-step that spent kSlowSpent force evaluations or more, 1 +
@@ -127,6 +146,30 @@ __host__ __device__ inline uint32_t slow_word(int rec, uint64_t spent)
     for (uint64_t s = kSlowSpent; spent >= s && bin < 48u; s <<= 1)  // s stays below 2^57
         bin++;
     return (uint32_t)(rec & 0xFF) | (bin << 8);
+}
+// Append (tag, word) to region g of the slow list (entries list / words,
+// count ctl[slow_region_ctl(g)], which goes on past the capacity; entries
+// past it are dropped) for every lane with pred set.  One atomic per wave,
+// as a lane per particle may append tens of thousands a launch (on one
+// address they would serialise).  Must be reached by all active lanes.
+__device__ inline void slow_push(uint32_t *ctl, uint32_t *list, uint32_t *words, int g, bool pred, uint32_t tag,
+                                 uint32_t word)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(ctl + slow_region_ctl(g), (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const uint64_t below = lane ? (m << (64 - lane)) : 0ull;  // lanes of m under this one
+    const uint32_t k = base + (uint32_t)__popcll(below);
+    if (pred && k < slow_region_cap(g)) {
+        list[slow_region_first(g) + k] = tag;
+        words[slow_region_first(g) + k] = word;
+    }
 }
 // slots per row: enough for the waves of any integrate launch over n particles
 int64_t work_stride_for(int64_t n);
@@ -172,11 +215,11 @@ hipError_t launch_integrate(const StepArgs &a, bool gather, int form, bool f32, 
                             Profiler *prof);
 
 // Overlapped run (ns_overlap.hip).  Device control words of ctl:
-//   [0], [3] class-2 and class-1 results of this step (all kernels)
+//   [0], [3], [5], [6], [7] the slow list's region counts of this step (all
+//       kernels; slow_region_ctl)
 //   [1] deferred ranks of the coming step (entries of def_pos)
 //   [2] ranks deferred in the step just sorted (sorted last, by key sentinel)
 //   [4] slow-list entries selected for deferral (entries of sel)
-//   [5] class-3 results of this step (all kernels)
 struct OverlapBufs {
     uint32_t *perm;      // n: the merged order, kDeferBit on deferred ranks
     uint32_t *slow;      // kSlowLists: StepArgs::slow_list

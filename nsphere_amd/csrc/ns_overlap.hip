@@ -18,7 +18,7 @@
 //   * an early element of h at local sorted position e moves up by the number
 //     of late particles of h inserted at or before e.
 // Which particles are deferred: those whose previous step was slow
-// (slow_class: capped or deep, else among the innermost ranks of their halo,
+// (slow_region: capped or deep, else among the innermost ranks of their halo,
 // else costly in force evaluations; the capped chains persist from step to
 // step), found through the slow list
 // the integrate kernels append to.  The choice affects only the time, never
@@ -110,18 +110,12 @@ __global__ void k_overlap_reset(uint32_t *__restrict__ ctl)
 __global__ void k_overlap_seed(OverlapBufs o, Halos g, const uint8_t *__restrict__ lvl)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.n)
-        return;
-    const uint8_t rec = lvl[t];
-    const int cls = slow_class(rec, t - g.start(g.of(t)));  // no force evaluations here: never class 3
-    if (cls != 1 && cls != 2)
-        return;
-    const uint32_t k = atomicAdd(o.ctl + (cls == 2 ? 0 : 3), 1u);
-    if (k < (uint32_t)(cls == 2 ? kSlowList : kCoreList)) {
-        const uint32_t q = (cls == 2 ? 0u : (uint32_t)kSlowList) + k;
-        o.slow[q] = t;
-        o.slow_rec[q] = rec;
-    }
+    const bool in = t < g.n;  // every lane reaches the pushes
+    const uint8_t rec = in ? lvl[t] : (uint8_t)0;
+    const int reg = in ? slow_region(rec, t - g.start(g.of(t))) : -1;  // no force evaluations here: never class 3
+#pragma unroll
+    for (int r = 0; r < kSlowRegions; r++)
+        slow_push(o.ctl, o.slow, o.slow_rec, r, reg == r, t, (uint32_t)rec);
 }
 
 template <typename S>
@@ -198,17 +192,16 @@ __device__ inline uint32_t slow_cost(uint32_t word, int cls)
     return bin < 31u ? bin : 31u;
 }
 
-// The selection's entries per thread of k_overlap_order: kPer2 of class 2,
-// one of class 1, one of class 3; the u-th of thread tid.
-constexpr int kPer2 = kSlowList / kPlanThreads;
-constexpr int kSelPer = kPer2 + 2;
-static_assert(kSlowList % kPlanThreads == 0 && kCoreList == kPlanThreads && kSpentList == kPlanThreads,
-              "whole entries per thread");
-__device__ inline uint32_t entry_of(int u, uint32_t tid)
+// The selection's entries per thread of k_overlap_order: entry u * kPlanThreads
+// + tid of the slow list for u < kSelPer, in region region_of_block(u).
+constexpr int kSelPer = kSlowLists / kPlanThreads;
+static_assert(kSlowLists % kPlanThreads == 0 && kPlanThreads == kSlowCap, "whole blocks of entries per thread");
+__host__ __device__ constexpr int region_of_block(int u)
 {
-    if (u < kPer2)
-        return (uint32_t)u * kPlanThreads + tid;
-    return (uint32_t)(u == kPer2 ? kSlowList : kSlowList + kCoreList) + tid;
+    int g = 0;
+    while (g + 1 < kSlowRegions && slow_region_first(g + 1) <= (uint32_t)u * kPlanThreads)
+        g++;
+    return g;
 }
 
 // Late particles of halo h in the (halo, key, rank)-sorted late list
@@ -241,7 +234,7 @@ __device__ inline void late_run(const uint32_t *late_h, uint32_t K, uint32_t h, 
 // through shuffles, the others (j >= 64, ten stages for 1024 entries)
 // through two alternating LDS buffers, one barrier each.  Into late_h /
 // late_t / late_slot[r] for the r-th.  And the slow-list entries to defer:
-// the costliest kSlowCap of the three classes by a counting sort on the cost (the
+// the costliest kSlowCap of all regions by a counting sort on the cost (the
 // order within a cost is free: it decides only which wave runs which
 // particle), into sel / ctl[4].
 template <typename S>
@@ -267,16 +260,17 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
     KeyT key = ~KeyT(0);
     if (tid < K)
         t = o.def_pos[tid];
-    // the selection's inputs (entry q of class 2, q < kSlowList, of class 1,
-    // kSlowList + q, and of class 3, kSlowList + kCoreList + q; u-th of a
-    // thread: entry_of): loaded now, used after the sort
-    uint32_t rec[kSelPer], ent[kSelPer];
+    // the selection's inputs (entry u * kPlanThreads + tid of the slow list,
+    // and the region counts): loaded now, used after the sort
+    uint32_t rec[kSelPer], ent[kSelPer], cnt[kSlowRegions];
 #pragma unroll
     for (int u = 0; u < kSelPer; u++) {
-        rec[u] = o.slow_rec[entry_of(u, tid)];
-        ent[u] = o.slow[entry_of(u, tid)];
+        rec[u] = o.slow_rec[(uint32_t)u * kPlanThreads + tid];
+        ent[u] = o.slow[(uint32_t)u * kPlanThreads + tid];
     }
-    const uint32_t c2 = o.ctl[0], c1 = o.ctl[3], c3 = o.ctl[5];
+#pragma unroll
+    for (int r = 0; r < kSlowRegions; r++)
+        cnt[r] = o.ctl[slow_region_ctl(r)];
     if (tid < K) {
         h = g.of(t);
         key = key_at(r, t);
@@ -322,17 +316,17 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
         o.late_slot[tid] = slot;
     }
     lds_barrier();  // the cleared cost bins (no LDS stage ran when P < 128)
-    // selection: the entries each class holds (its count, at most its list)
-    const uint32_t m2 = c2 < (uint32_t)kSlowList ? c2 : (uint32_t)kSlowList;
-    const uint32_t m1 = c1 < (uint32_t)kCoreList ? c1 : (uint32_t)kCoreList;
-    const uint32_t m3 = c3 < (uint32_t)kSpentList ? c3 : (uint32_t)kSpentList;
-    uint32_t cost[kSelPer], within[kSelPer];
+    // selection: the entries each region holds (its count, at most its capacity)
+    uint32_t cost[kSelPer], within[kSelPer], m_all = 0;
+#pragma unroll
+    for (int r = 0; r < kSlowRegions; r++)
+        m_all += cnt[r] < slow_region_cap(r) ? cnt[r] : slow_region_cap(r);
 #pragma unroll
     for (int u = 0; u < kSelPer; u++) {
-        const int cls = u < kPer2 ? 2 : (u == kPer2 ? 1 : 3);
-        const uint32_t k = cls == 2 ? (uint32_t)u * kPlanThreads + tid : tid;  // index within its class
-        const bool have = k < (cls == 2 ? m2 : (cls == 1 ? m1 : m3));
-        cost[u] = have ? slow_cost(rec[u], cls) : 0u;
+        const int r = region_of_block(u);
+        const uint32_t k = (uint32_t)u * kPlanThreads + tid - slow_region_first(r);  // index within its region
+        const bool have = k < cnt[r] && k < slow_region_cap(r);
+        cost[u] = have ? slow_cost(rec[u], slow_region_class(r)) : 0u;
         within[u] = have ? atomicAdd(&s_bin[cost[u]], 1u) : ~0u;
     }
     __syncthreads();
@@ -366,7 +360,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_order(OverlapBufs o, H
             o.sel[slot] = ent[u];
     }
     if (tid == 0)
-        o.ctl[4] = m2 + m1 + m3 < (uint32_t)kSlowCap ? m2 + m1 + m3 : (uint32_t)kSlowCap;
+        o.ctl[4] = m_all < (uint32_t)kSlowCap ? m_all : (uint32_t)kSlowCap;
 }
 
 // One wave per query: wave w < kSlowCap is the w-th late particle (its
@@ -407,7 +401,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_overlap_search(OverlapBufs o
 // pairs, ascending, into ins; the selected entries (the previous step's
 // slow list, costliest first) at their new ranks, with their old positions,
 // into def_pos / def_src; their count into ctl[1], the late count into
-// ctl[2]; clears the class counts ctl[0], ctl[3] and ctl[5].
+// ctl[2]; clears the slow list's region counts.
 __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Halos g, unsigned long long *chain)
 {
     // the previous late launch has ended (same stream): fold its longest chain
@@ -449,9 +443,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_overlap_plan(OverlapBufs o, Ha
     }
     __syncthreads();
     if (tid == 0) {
-        o.ctl[0] = 0;
-        o.ctl[3] = 0;
-        o.ctl[5] = 0;
+#pragma unroll
+        for (int r = 0; r < kSlowRegions; r++)
+            o.ctl[slow_region_ctl(r)] = 0;
         o.ctl[1] = m;
         o.ctl[2] = K;
     }

@@ -120,6 +120,22 @@ __device__ inline bool takes_levi_civita(const StepArgs &a, int64_t i, double r,
     return takes_levi_civita(a, rank_mass(a, i), r, ell);
 }
 
+// Whether a falling leapfrog particle's straight path (r + s v dt, at its
+// enclosed mass now) reaches the Levi-Civita regime within kSoonSteps
+// steps.  Leapfrog particles at level kSoonLevel or more that do turn into
+// Levi-Civita chains of thousands of micro-steps a tau step on their way to
+// pericentre (in a cusp); the deferral of the next step ranks them beside
+// the deepest ones.
+constexpr double kSoonSteps = 2.0;
+constexpr int kSoonLevel = 6;
+__device__ inline bool lc_soon(const StepArgs &a, const RankMass &rm, double r, double v, double ell)
+{
+    if (!(v < 0.0))
+        return false;
+    const double ahead = fma(kSoonSteps * a.dt, v, r);
+    return ahead <= 1.0e-30 || takes_levi_civita(a, rm, ahead, ell);
+}
+
 // h / 2^e for the power-of-two divisors of the leapfrog passes: exact, as
 // the division it replaces (no result here comes near the subnormals).
 __device__ inline double div_pow2(double h, int e) { return ldexp(h, -e); }
