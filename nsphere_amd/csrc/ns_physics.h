@@ -446,16 +446,19 @@ __device__ int adaptive_leap_wave(const Phys &ph, const StepArgs &a, double r_in
 // faster on one wave (23.3 against 24.6 ns per micro-step).
 struct LcFast {
     double hd, lh, gh;  // dtau/2, hd dtau ell^2, hd dtau gm
+    double ihd;         // 1 / hd
     double rho, u, u_prev, T;
     double g, inv2;     // force terms at the current rho
 
     // init with the dtau-dependent constants (LcPassConst) and 1/rho_in^2
     // given: the same operations as init
-    __device__ inline void init_from(double hd_, double lh_, double gh_, double rho_in, double inv2_in, double v_in)
+    __device__ inline void init_from(double hd_, double lh_, double gh_, double ihd_, double rho_in, double inv2_in,
+                                     double v_in)
     {
         hd = hd_;
         lh = lh_;
         gh = gh_;
+        ihd = ihd_;
         rho = rho_in;
         T = 0.0;
         inv2 = inv2_in;
@@ -466,6 +469,7 @@ struct LcFast {
     __device__ inline void init(const Phys &ph, double dtau, double rho_in, double v_in)
     {
         hd = 0.5 * dtau;
+        ihd = 1.0 / hd;
         const double sc = hd * dtau;
         lh = ph.l2 * sc;
         gh = (ph.idg ? 0.0 : ph.gm) * sc;
@@ -490,8 +494,8 @@ struct LcFast {
         rho = rn;
     }
     // velocity after the steps taken: the closing half kick on the u held
-    // before the last full kick
-    __device__ inline double v_end() const { return fma(0.5 * g, inv2, u_prev) / hd; }
+    // before the last full kick (u is v scaled by hd)
+    __device__ inline double v_end() const { return fma(0.5 * g, inv2, u_prev) * ihd; }
     __device__ inline double t_end(double t_in) const { return fma(0.5 * hd, T, t_in); }
 };
 
@@ -529,6 +533,22 @@ __device__ inline void micro_lc(const Phys &ph, double rho_in, double v_in, doub
     rho_out = rho;
     v_out = v;
     t_out = t;
+}
+
+// The reference's Levi-Civita convergence test (nsphere.c:2125-2130): the
+// change of rho relative to r = rho_F^2 and of v relative to v_F below the
+// tolerances.  The fast form multiplies instead of dividing, which differs
+// from it only where a quotient rounds across the tolerance.
+template <bool FAST>
+__device__ inline bool lc_converged(const StepArgs &a, double rhoC, double vC, double rhoF, double vF)
+{
+    const double rF = rhoF * rhoF;
+    if (FAST)
+        return fabs(rhoF - rhoC) < a.radius_tol * (fabs(rF) + 1.0e-30) &&
+               fabs(vF - vC) < a.velocity_tol * (fabs(vF) + 1.0e-30);
+    const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
+    const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
+    return (rhodif < a.radius_tol) && (vdif < a.velocity_tol);
 }
 
 __device__ inline void lc_output(int out_type, double rhoC, double vC, double tC, double rhoF, double vF,
@@ -584,10 +604,7 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
         }
         micro_lc<FAST>(ph, rho_in, v_in, t_in, 4 * N + 1, h, rhoF, vF, tF);
         last = level_of(N);
-        const double rF = rhoF * rhoF;
-        const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
-        const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
-        if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
+        if (lc_converged<FAST>(a, rhoC, vC, rhoF, vF)) {
             lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
             return last;
         }
@@ -707,10 +724,7 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
         const double rn = rho_now(), vn = v_now(), tn = t_now();
         const double rhoC = __shfl(rn, src_c, 64), vC = __shfl(vn, src_c, 64), tC = __shfl(tn, src_c, 64);
         const double rhoF = __shfl(rn, k, 64), vF = __shfl(vn, k, 64), tF = __shfl(tn, k, 64);
-        const double rF = rhoF * rhoF;
-        const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
-        const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
-        if ((rhodif < a.radius_tol) && (vdif < a.velocity_tol)) {
+        if (lc_converged<FAST>(a, rhoC, vC, rhoF, vF)) {
             lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
             bud.chain += it;
             return level_of(Nk);
@@ -804,24 +818,19 @@ __device__ int adaptive_lc(const Phys &ph, const StepArgs &a, double r_in, doubl
 // micro-steps over h (the same operations, done once per particle-step
 // instead of once per tau step).
 struct LcPassConst {
-    double hd, lh, gh;
+    double hd, lh, gh, ihd;
     __device__ inline void set(const Phys &ph, double h, int64_t sub)
     {
         const double dtau = h / (double)sub;
         hd = 0.5 * dtau;
+        ihd = 1.0 / hd;
         const double sc = hd * dtau;
         lh = ph.l2 * sc;
         gh = (ph.idg ? 0.0 : ph.gm) * sc;
     }
 };
 
-__device__ inline bool lc_converged(const StepArgs &a, double rhoC, double vC, double rhoF, double vF)
-{
-    const double rF = rhoF * rhoF;
-    const double rhodif = fabs(rhoF - rhoC) / (fabs(rF) + 1.0e-30);
-    const double vdif = fabs(vF - vC) / (fabs(vF) + 1.0e-30);
-    return (rhodif < a.radius_tol) && (vdif < a.velocity_tol);
-}
+
 
 // single_tau_lc<true> with cap_N = 2 (levels 0 and 1), for max_subdiv >= 4,
 // written out: the coarse and fine passes of level 0 (3 and 5 micro-steps)
@@ -840,8 +849,8 @@ __device__ inline int lc_tau_lane_fast(const StepArgs &a, const LcPassConst K[3]
     const double inv = fast_rcp(rho_in);
     const double inv2 = inv * inv;
     LcFast c, f;
-    c.init_from(K[0].hd, K[0].lh, K[0].gh, rho_in, inv2, v_in);
-    f.init_from(K[1].hd, K[1].lh, K[1].gh, rho_in, inv2, v_in);
+    c.init_from(K[0].hd, K[0].lh, K[0].gh, K[0].ihd, rho_in, inv2, v_in);
+    f.init_from(K[1].hd, K[1].lh, K[1].gh, K[1].ihd, rho_in, inv2, v_in);
     c.step();
     f.step();
     c.step();
@@ -858,7 +867,7 @@ __device__ inline int lc_tau_lane_fast(const StepArgs &a, const LcPassConst K[3]
         return NS_LVL_NOCONV;
     }
     const double rhoF = f.rho, vF = f.v_end(), tF = f.t_end(t_in);
-    if (lc_converged(a, rhoC, vC, rhoF, vF)) {
+    if (lc_converged<true>(a, rhoC, vC, rhoF, vF)) {
         lc_output(a.out_type, rhoC, vC, tC, rhoF, vF, tF, rho_out, v_out, t_out);
         return 0;
     }
@@ -869,12 +878,12 @@ __device__ inline int lc_tau_lane_fast(const StepArgs &a, const LcPassConst K[3]
         return NS_LVL_NOCONV;
     }
     LcFast d;
-    d.init_from(K[2].hd, K[2].lh, K[2].gh, rho_in, inv2, v_in);
+    d.init_from(K[2].hd, K[2].lh, K[2].gh, K[2].ihd, rho_in, inv2, v_in);
 #pragma unroll
     for (int s = 0; s < 9; s++)
         d.step();
     const double rhoD = d.rho, vD = d.v_end(), tD = d.t_end(t_in);
-    if (lc_converged(a, rhoF, vF, rhoD, vD)) {
+    if (lc_converged<true>(a, rhoF, vF, rhoD, vD)) {
         lc_output(a.out_type, rhoF, vF, tF, rhoD, vD, tD, rho_out, v_out, t_out);
         return 1;
     }
