@@ -449,6 +449,7 @@ struct LcFast {
     double ihd;         // 1 / hd
     double rho, u, u_prev, T;
     double g, inv2;     // force terms at the current rho
+    double inv;         // 1 / rho (set by init and step, read by step_seeded)
 
     // init with the dtau-dependent constants (LcPassConst) and 1/rho_in^2
     // given: the same operations as init
@@ -461,7 +462,7 @@ struct LcFast {
         ihd = ihd_;
         rho = rho_in;
         T = 0.0;
-        inv2 = inv2_in;
+        inv2 = inv2_in;  // inv stays unset: step, not step_seeded, comes first
         g = fma(lh, inv2, gh);
         u = fma(0.5 * g, inv2, hd * v_in);
         u_prev = u;
@@ -475,7 +476,7 @@ struct LcFast {
         gh = (ph.idg ? 0.0 : ph.gm) * sc;
         rho = rho_in;
         T = 0.0;
-        const double inv = fast_rcp(rho);
+        inv = fast_rcp(rho);
         inv2 = inv * inv;
         g = fma(lh, inv2, gh);
         u = fma(0.5 * g, inv2, hd * v_in);  // hd (v + hd forceLC(rho))
@@ -484,7 +485,26 @@ struct LcFast {
     __device__ inline void step()
     {
         const double rn = fma(rho, u, rho);
-        const double inv = fast_rcp(rn);
+        inv = fast_rcp(rn);
+        inv2 = inv * inv;
+        g = fma(lh, inv2, gh);
+        const double s = rho + rn;
+        T = fma(s, s, T);
+        u_prev = u;
+        u = fma(g, inv2, u);
+        rho = rn;
+    }
+    // step with 1/rn from the reciprocal of the previous rho instead of the
+    // hardware reciprocal: rn = rho (1 + u), so y0 = inv (1 - u) has the
+    // relative error u^2 and one Newton-Raphson step y0 (2 - rn y0) leaves
+    // u^4 (below the double rounding for |u| < 1e-4, the relative change of
+    // rho in one micro-step).  Two FMAs in place of the reciprocal, which
+    // issues at a quarter of the FMA rate.
+    __device__ inline void step_seeded()
+    {
+        const double rn = fma(rho, u, rho);
+        const double y0 = fma(-inv, u, inv);
+        inv = fma(y0, fma(-rn, y0, 1.0), y0);
         inv2 = inv * inv;
         g = fma(lh, inv2, gh);
         const double s = rho + rn;
@@ -633,6 +653,8 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
 // 4*2^k+1 iterations lanes k-1 (or 63) and k hold coarse and fine of level k
 // and the reference's test is evaluated.  Same results as the lane form, after
 // 4*2^K+1 iterations instead of about 8*2^K.
+constexpr int kSeedLevel = 8;
+
 template <bool FAST>
 __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
                                   double h, double &rho_out, double &v_out, double &t_out, Budget &bud)
@@ -693,6 +715,10 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     // Iterations [4*2^(k-1)+1, 4*2^k+1) belong to lanes k..kmax+1, whose
     // passes all run through them, so the lane mask is set once per level.
     // Level 0 (iterations [0, 5)) also carries lane 63's 3-step coarse pass.
+    // The fast form's iterations from level kSeedLevel on take the seeded
+    // reciprocal (LcFast::step_seeded): the passes running there have at
+    // least 4*2^kSeedLevel+1 micro-steps, each moving rho by a small part of
+    // its change over the tau step.
     int64_t it = 0;
     auto run_level = [&](int k) {
         const int64_t end = 4 * ((int64_t)1 << k) + 1;
@@ -704,7 +730,10 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
                 for (int s = 3; s < 5; s++)
                     micro_step();
         } else if (lane >= k && lane <= kmax + 1) {
-            repeat_uniform(end - it, micro_step);  // 2^(k+1) steps
+            if (FAST && k >= kSeedLevel)
+                repeat_uniform(end - it, [&]() { lf.step_seeded(); });  // 2^(k+1) steps
+            else
+                repeat_uniform(end - it, micro_step);  // 2^(k+1) steps
         }
         it = end;
     };
