@@ -531,6 +531,14 @@ def roofline_for(run: dict, n: int, kernel: str) -> dict | None:
                               "ns_per_micro_step": round(ns_step, 2), "floor_ns": floor,
                               "floor_source": "ns_gpu_chain_floor, one wave alone, measured in this run",
                               "frac": round(floor / ns_step, 3) if floor else None}
+            if floor and run.get("ms_per_step"):
+                # the timed region's step period against that chain at the
+                # floor: the late kernel bounds the step when this is near 1
+                chain_ms = run["chain"] / launches * floor * 1e-6
+                out["latency"]["timed_step_vs_chain_floor"] = {
+                    "chain_at_floor_ms": round(chain_ms, 4), "ms_per_step": round(run["ms_per_step"], 4),
+                    "frac": round(chain_ms / run["ms_per_step"], 3),
+                    "note": "chain length from the profiled steps, step period from the timed region"}
         return out
     return None
 
