@@ -479,10 +479,12 @@ int ns_gpu_force_evals(ns_gpu_ctx *ctx, uint64_t out[3]);
 int ns_gpu_chain_steps(ns_gpu_ctx *ctx, uint64_t *out);
 
 /* The floor of those chains (a measurement aid with no counterpart in the
- * reference): one wave alone on `device` running the late kernel's fast
- * Levi-Civita micro-step micro_steps times (1e3 .. 2^34) on a scalar trip
- * counter; ns_per_step receives its duration per micro-step.  Run it while the
- * device is otherwise idle. */
+ * reference): one wave alone on `device` running the fast Levi-Civita
+ * micro-steps of the late kernel's long chains (one with a Newton-refined
+ * reciprocal, then three with the series one) micro_steps times (1e3 .. 2^34,
+ * rounded down to a multiple of 4) on a scalar trip counter; ns_per_step
+ * receives its duration per micro-step.  Run it while the device is otherwise
+ * idle. */
 int ns_gpu_chain_floor(int device, int64_t micro_steps, double *ns_per_step);
 
 /* Seeded initial conditions in G = M = a = 1 units (host, deterministic per

@@ -2178,7 +2178,7 @@ int ns_gpu_chain_floor(int device, int64_t micro_steps, double *ns_per_step)
     if (s) (void)hipStreamDestroy(s);
     if (e != hipSuccess)
         return set_error(NS_ERR_HIP, "ns_gpu_chain_floor: %s", hipGetErrorString(e));
-    *ns_per_step = (double)ms * 1.0e6 / (double)micro_steps;
+    *ns_per_step = (double)ms * 1.0e6 / (double)(micro_steps & ~(int64_t)3);  // whole groups of four ran
     return NS_OK;
 }
 

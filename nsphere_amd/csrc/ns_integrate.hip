@@ -415,11 +415,12 @@ __global__ __launch_bounds__(64) void k_integrate_late(StepArgs a, const uint32_
     add_work(a, 2, (threadIdx.x & 63) == 0 ? spent : 0ull);
 }
 
-// One wave running the fast Levi-Civita micro-step with the seeded
-// reciprocal (LcFast::step_seeded, the loop body of the late kernel's long
-// chains) `steps` times on a scalar trip counter, as the wave form does; out
-// keeps the result live, the elapsed-time sum included (the chains need it,
-// so it is part of their floor).
+// One wave running the fast Levi-Civita micro-steps of the late kernel's
+// long chains (the levels from kSeriesLevel on: LcFast::step_seeded, then
+// three LcFast::step_series) `steps` times (rounded down to a multiple of 4)
+// on a scalar trip counter, as the wave form does; out keeps the result live,
+// the elapsed-time sum included (the chains need it, so it is part of their
+// floor).
 __global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__restrict__ out)
 {
     Phys ph;
@@ -428,7 +429,12 @@ __global__ __launch_bounds__(64) void k_chain_floor(int64_t steps, double *__res
     ph.idg = false;
     LcFast lf;
     lf.init(ph, 1.0e-3 / (double)(steps > 0 ? steps : 1), 0.1, -0.2);
-    repeat_uniform(steps, [&]() { lf.step_seeded(); });
+    repeat_uniform(steps / 4, [&]() {
+        lf.step_seeded();
+        lf.step_series();
+        lf.step_series();
+        lf.step_series();
+    });
     if ((threadIdx.x & 63) == 0)
         out[0] = lf.rho + lf.v_end() + lf.t_end(0.0);
 }

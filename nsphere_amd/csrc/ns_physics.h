@@ -513,6 +513,21 @@ struct LcFast {
         u = fma(g, inv2, u);
         rho = rn;
     }
+    // step with 1/rn = inv (1 - u + u^2), the series of 1/(1 + u) to second
+    // order: relative error u^3 on top of the error inv brings, corrected by
+    // the Newton step of the next step_seeded (one FMA fewer than it).
+    __device__ inline void step_series()
+    {
+        const double rn = fma(rho, u, rho);
+        inv = fma(inv, fma(u, u, -u), inv);
+        inv2 = inv * inv;
+        g = fma(lh, inv2, gh);
+        const double s = rho + rn;
+        T = fma(s, s, T);
+        u_prev = u;
+        u = fma(g, inv2, u);
+        rho = rn;
+    }
     // velocity after the steps taken: the closing half kick on the u held
     // before the last full kick (u is v scaled by hd)
     __device__ inline double v_end() const { return fma(0.5 * g, inv2, u_prev) * ihd; }
@@ -653,7 +668,12 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
 // 4*2^k+1 iterations lanes k-1 (or 63) and k hold coarse and fine of level k
 // and the reference's test is evaluated.  Same results as the lane form, after
 // 4*2^K+1 iterations instead of about 8*2^K.
+// The fast form's wave passes from level kSeedLevel on take the reciprocal
+// from the previous micro-step's (LcFast::step_seeded), and from level
+// kSeriesLevel on three micro-steps of four take the second-order series
+// instead (LcFast::step_series) between Newton-refined ones.
 constexpr int kSeedLevel = 8;
+constexpr int kSeriesLevel = 9;
 
 template <bool FAST>
 __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
@@ -716,9 +736,10 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
     // passes all run through them, so the lane mask is set once per level.
     // Level 0 (iterations [0, 5)) also carries lane 63's 3-step coarse pass.
     // The fast form's iterations from level kSeedLevel on take the seeded
-    // reciprocal (LcFast::step_seeded): the passes running there have at
-    // least 4*2^kSeedLevel+1 micro-steps, each moving rho by a small part of
-    // its change over the tau step.
+    // reciprocal (LcFast::step_seeded, and from kSeriesLevel on the series
+    // one three times in four): the passes running there have at least
+    // 4*2^kSeedLevel+1 micro-steps, each moving rho by a small part of its
+    // change over the tau step.
     int64_t it = 0;
     auto run_level = [&](int k) {
         const int64_t end = 4 * ((int64_t)1 << k) + 1;
@@ -730,7 +751,14 @@ __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_
                 for (int s = 3; s < 5; s++)
                     micro_step();
         } else if (lane >= k && lane <= kmax + 1) {
-            if (FAST && k >= kSeedLevel)
+            if (FAST && k >= kSeriesLevel)
+                repeat_uniform((end - it) / 4, [&]() {  // 2^(k+1) steps, a multiple of 4
+                    lf.step_seeded();
+                    lf.step_series();
+                    lf.step_series();
+                    lf.step_series();
+                });
+            else if (FAST && k >= kSeedLevel)
                 repeat_uniform(end - it, [&]() { lf.step_seeded(); });  // 2^(k+1) steps
             else
                 repeat_uniform(end - it, micro_step);  // 2^(k+1) steps
