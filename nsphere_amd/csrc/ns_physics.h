@@ -672,8 +672,8 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
 // from the previous micro-step's (LcFast::step_seeded), and from level
 // kSeriesLevel on three micro-steps of four take the second-order series
 // instead (LcFast::step_series) between Newton-refined ones.
-constexpr int kSeedLevel = 8;
-constexpr int kSeriesLevel = 9;
+constexpr int kSeedLevel = 5;
+constexpr int kSeriesLevel = 6;
 
 template <bool FAST>
 __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
