@@ -674,6 +674,8 @@ __device__ int single_tau_lc(const Phys &ph, const StepArgs &a, double rho_in, d
 // instead (LcFast::step_series) between Newton-refined ones.
 constexpr int kSeedLevel = 5;
 constexpr int kSeriesLevel = 6;
+static_assert(kSeedLevel >= 1 && kSeriesLevel >= kSeedLevel,
+              "level 0 runs the hardware reciprocal; a level k >= 1 adds 2^(k+1) iterations, whole groups of 4");
 
 template <bool FAST>
 __device__ int single_tau_lc_wave(const Phys &ph, const StepArgs &a, double rho_in, double v_in, double t_in,
