@@ -89,7 +89,11 @@ enum {
                                  particle on a whole wave, one level per lane;
                                  bit-identical to 0;
                               2 binned with the fast arithmetic form (one
-                                 reciprocal per force, fused multiply-adds)   */
+                                 reciprocal per force, fused multiply-adds;
+                                 the Levi-Civita wave passes from level 5
+                                 take the reciprocal from the previous
+                                 micro-step's, Newton-refined, or its
+                                 second-order series)                         */
     NS_OPT_SORT_TILE = 6,  /* radix tile: (threads << 16) | keys per thread,
                               one of 256x16, 256x24, 512x8, 512x12 (default),
                               512x16, 1024x6                                 */
